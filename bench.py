@@ -1,0 +1,219 @@
+"""Benchmark: RIRs/s of the batched simulation (4096 uneven rooms, one hipGraph replay
+per step) on one MI355X, with the plain drop-in rate, accuracy against the FP64 oracle,
+the roofline of the dominant kernel and the CPU oracle timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one pass of the simulation over the whole batch of rooms (40,709 RIRs):
+image amplitudes/delays, windowed-sinc accumulation, envelope and diffuse tail, with
+inputs already resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (packed); non-packed issue caps at half of it
+FLOP_PER_TAP = 12               # sub, x2pi, /Tw, cos, +1, x0.5, xpi, sin, /, xamp, xsinc, +acc
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    import numpy as np
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary (None if absent)."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rooms", type=int, default=4096)
+    ap.add_argument("--cpu-sample-rooms", type=int, default=6)
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus must be 1)")
+
+    import numpy as np
+    import torch
+
+    import gpurir_amd as gpuRIR
+    from gpurir_amd import torch_ops as T
+    from gpurir_amd import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    fs = W.FS
+
+    # ---------------- batched: 4096 uneven rooms, one graph replay per step
+    log(f"[bench] generating {args.rooms} uneven rooms")
+    rooms = W.uneven_rooms(n_rooms=args.rooms, seed=0)
+    plan = T.BatchPlan(rooms, fs, seed=1234, call_offset=0)
+    n_rirs = plan.n_rirs
+    log(f"[bench] plan: {n_rirs} RIRs, {plan.out.numel() * 4 / 1e9:.2f} GB output")
+    taps = plan.count_taps()
+    log(f"[bench] in-window taps W = {taps:.4e}")
+    for _ in range(args.warmup):
+        plan.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    rirs_per_s = n_rirs / (ms_step / 1e3)
+    log(f"[bench] batched: {ms_step:.3f} ms/step, {rirs_per_s:.4e} RIRs/s")
+
+    # dominant kernel: accumulation, timed with HIP events on its launch stream
+    prof = [plan.profile() for _ in range(5)]
+    ms_acc = float(np.mean([p[0] for p in prof]))
+    ms_fin = float(np.mean([p[1] for p in prof]))
+    achieved = FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12
+    log(f"[bench] accumulate {ms_acc:.3f} ms, finalize {ms_fin:.3f} ms, {achieved:.2f} TFLOP/s algorithmic")
+
+    # accuracy of the batch against the FP64 oracle (same Philox stream)
+    from oracle import oracle as orc
+
+    plan.run()
+    torch.cuda.synchronize()
+    errs = []
+    for k in range(3):
+        r = rooms[k]
+        o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
+                             Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
+        errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
+    log(f"[bench] batch rel-L2 vs FP64 (3 rooms): {max(errs):.3e}")
+
+    # ---------------- plain drop-in: one room, numpy in / numpy out, FP32 exact sinc
+    c1 = W.plain_room()
+    gpuRIR.activateLUT(False)
+    kw = dict(room_sz=c1["room_sz"], beta=c1["beta"], pos_src=c1["pos_src"], pos_rcv=c1["pos_rcv"],
+              nb_img=c1["nb_img"], Tmax=c1["Tmax"], fs=fs, Tdiff=c1["Tdiff"])
+    for _ in range(5):
+        gpuRIR.simulateRIR(**kw)
+    n_plain = 50
+    t0 = time.perf_counter()
+    for _ in range(n_plain):
+        r1 = gpuRIR.simulateRIR(**kw)
+    t1 = time.perf_counter()
+    plain_rirs_s = 2 * n_plain / (t1 - t0)
+    gpuRIR.set_seed(1234, 0)
+    r1 = gpuRIR.simulateRIR(**kw)
+    err_plain = rel_l2(r1, orc.simulate_rir(**kw, call_offset=0))
+    gpuRIR.activateLUT(True)
+    log(f"[bench] plain drop-in: {plain_rirs_s:.4e} RIRs/s, rel-L2 {err_plain:.3e}")
+
+    # ---------------- mid room (4 x 32, directional), device resident
+    c2 = W.mid_room()
+    kw2 = dict(room_sz=c2["room_sz"], beta=c2["beta"], pos_src=c2["pos_src"], pos_rcv=c2["pos_rcv"],
+               nb_img=c2["nb_img"], Tmax=c2["Tmax"], fs=fs, Tdiff=c2["Tdiff"], spkr_pattern="hypcard",
+               mic_pattern="card", orV_src=c2["orV_src"], orV_rcv=c2["orV_rcv"])
+    ps = torch.as_tensor(c2["pos_src"], device=dev)
+    pr = torch.as_tensor(c2["pos_rcv"], device=dev)
+    kw2d = dict(kw2, pos_src=ps, pos_rcv=pr, orV_src=torch.as_tensor(c2["orV_src"], device=dev),
+                orV_rcv=torch.as_tensor(c2["orV_rcv"], device=dev))
+    out2 = T.simulate_rir(**kw2d)
+    for _ in range(3):
+        T.simulate_rir(**kw2d, out=out2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        T.simulate_rir(**kw2d, out=out2)
+    torch.cuda.synchronize()
+    ms_mid = (time.perf_counter() - t0) * 1e3 / 20
+    err_mid = rel_l2(out2.cpu().numpy(), orc.simulate_rir(**kw2))
+    log(f"[bench] mid room: {ms_mid:.3f} ms/call ({128 / ms_mid * 1e3:.4e} RIRs/s), rel-L2 {err_mid:.3e}")
+
+    # ---------------- trajectory: 128 points x 8 mics, full ISM 0.5 s, 10 s of noise
+    c5 = W.trajectory()
+    rirs5 = T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs)
+    sig5 = torch.as_tensor(c5["signal"], device=dev)
+    for _ in range(2):
+        T.trajectory(sig5, rirs5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        y5 = T.trajectory(sig5, rirs5)
+    torch.cuda.synchronize()
+    ms_traj = (time.perf_counter() - t0) * 1e3 / 10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs, out=rirs5)
+    torch.cuda.synchronize()
+    ms_traj_rirs = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms, convolution+OLA {ms_traj:.3f} ms")
+    del y5
+
+    # ---------------- CPU baseline: the FP64 oracle on a bounded sample of the same batch
+    cpu = None
+    if not args.skip_cpu:
+        nthreads = orc.max_threads()
+        sample = rooms[:args.cpu_sample_rooms]
+        n_s = sum(len(r["pos_rcv"]) for r in sample)
+        t0 = time.perf_counter()
+        for k, r in enumerate(sample):
+            orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
+                             Tdiff=r["Tdiff"], room_index=k)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n_s / dt, "unit": "RIRs/s", "cores": nthreads, "kind": "port",
+               "sample": f"first {len(sample)} rooms of the 4096-room batch ({n_s} RIRs), FP64 oracle, "
+                         f"windowed scatter, {nthreads} OpenMP threads, {dt:.1f} s"}
+        log(f"[bench] CPU oracle: {cpu['value']:.3e} RIRs/s on {nthreads} threads")
+
+    traffic = read_traffic("traffic_ism_accumulate_uneven_rooms.txt")
+    line = {
+        "metric": "RIRs/s batched vs plain at <=1e-3 rel-L2 vs FP64; fwd+bwd grads/s",
+        "value": rirs_per_s,
+        "unit": "RIRs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.rooms} uneven rooms in one launch (3-10 x 3-8 x 2.5-4 m, T60 0.2-1.2 s, "
+                               "1 src x 4-16 rcv, fs 16 kHz, Tmax 1.1 T60, Tdiff at 15 dB), FP32 exact sinc, "
+                               "hipGraph replay", "rirs": n_rirs, "rooms": args.rooms, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "ism_accumulate_kernel", "kernel_ms": ms_acc,
+                     "work": f"{taps} in-window taps x {FLOP_PER_TAP} flop"},
+        "cpu_baseline": cpu,
+        "accuracy": {"batch_rel_l2_max_3_rooms": max(errs), "plain_rel_l2": err_plain, "mid_room_rel_l2": err_mid},
+        "plain_rirs_per_s": plain_rirs_s,
+        "batched_over_plain": rirs_per_s / plain_rirs_s,
+        "mid_room_ms": ms_mid,
+        "finalize_ms": ms_fin,
+        "trajectory": {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

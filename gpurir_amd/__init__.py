@@ -1,0 +1,176 @@
+"""Room impulse responses (image source method) on AMD MI355X -- drop-in for gpuRIR.
+
+``import gpurir_amd as gpuRIR`` gives the API of gpuRIR/__init__.py (DavidDiazGuerra/gpuRIR
+v1.2.0): the same functions, arguments, defaults, polar patterns, assertion messages and
+output layouts.  The simulation runs in hand-written gfx950 HIP kernels reached through
+the C ABI of libgpurir.so (include/gpurir.h); there is no CPU fallback.
+
+Differences that follow from the design (see the repository documentation):
+  * the diffuse tail draws its noise from Philox4x32-10 with the library's seed
+    (1234, as the reference seeds cuRAND) and a per-call counter, not from cuRAND;
+  * activateMixedPrecision keeps the reference's messages and mutual exclusion with
+    the lookup table, but the half2 path is not provided (FP32 is used instead).
+"""
+from __future__ import division
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["polar_patterns", "beta_SabineEstimation", "att2t_SabineEstimator", "t2n", "simulateRIR",
+           "simulateTrajectory", "activateMixedPrecision", "activateLUT", "activate_mixed_precision", "activate_lut"]
+
+# gpuRIR/__init__.py:13-20
+polar_patterns = {
+    "omni": 0,
+    "homni": 1,
+    "card": 2,
+    "hypcard": 3,
+    "subcard": 4,
+    "bidir": 5,
+}
+
+
+def beta_SabineEstimation(room_sz, T60, abs_weights=[1.0] * 6):
+    """Reflection coefficients giving the requested T60 by Sabine's formula (gpuRIR/__init__.py:22-54).
+
+    Same objective, start point (0.5) and bounds ([0, 1]) as the reference, solved with
+    scipy.optimize.minimize; returns float32 [beta_x0, beta_x1, beta_y0, beta_y1, beta_z0, beta_z1].
+    """
+    from scipy.optimize import minimize
+
+    def t60error(x, T60, room_sz, abs_weights):
+        alpha = x * abs_weights
+        Sa = (alpha[0] + alpha[1]) * room_sz[1] * room_sz[2] + \
+            (alpha[2] + alpha[3]) * room_sz[0] * room_sz[2] + \
+            (alpha[4] + alpha[5]) * room_sz[0] * room_sz[1]
+        V = np.prod(room_sz)
+        if Sa == 0:
+            return T60 - 0  # anechoic chamber
+        return abs(T60 - 0.161 * V / Sa)  # Sabine's formula
+
+    abs_weights /= np.array(abs_weights).max()
+    result = minimize(t60error, 0.5, args=(T60, room_sz, abs_weights), bounds=[[0, 1]])
+    return np.sqrt(1 - result.x * abs_weights).astype(np.float32)
+
+
+def att2t_SabineEstimator(att_dB, T60):
+    """Time for the RIR to reach att_dB of attenuation (gpuRIR/__init__.py:56-72)."""
+    return att_dB / 60.0 * T60
+
+
+def t2n(T, rooms_sz, c=343.0):
+    """Images per dimension needed for RIRs of length T (gpuRIR/__init__.py:74-93)."""
+    nb_img = 2 * T / (np.array(rooms_sz) / c)
+    return [int(n) for n in np.ceil(nb_img)]
+
+
+def _f32c(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def simulateRIR(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, spkr_pattern="omni",
+                mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0):
+    """RIRs of a shoebox room by the image source method (gpuRIR/__init__.py:95-175).
+
+    Returns float32 [n_src, n_rcv, nSamples], nSamples = ceil(Tmax*fs) rounded up to even.
+    The early part (t < Tdiff) is the windowed-sinc sum over images; the rest is the
+    diffuse model.  Uses the lookup-table state set by activateLUT (on by default).
+    """
+    assert not ((pos_src >= room_sz).any() or (pos_src <= 0).any()), "The sources must be inside the room"
+    assert not ((pos_rcv >= room_sz).any() or (pos_rcv <= 0).any()), "The receivers must be inside the room"
+    assert Tdiff is None or Tdiff <= Tmax, "Tmax must be equal or greater than Tdiff"
+    assert mic_pattern in polar_patterns, "mic_pattern must be omni, homni, card, hypcard, subcard or bidir"
+    assert mic_pattern == "omni" or orV_rcv is not None, "the mics are not omni but their orientation is undefined"
+    assert spkr_pattern in spkr_pattern, "spkr_pattern must be omni, homni, card, hypcard, subcard or bidir"
+    assert spkr_pattern == "omni" or orV_src is not None, "the sources are not omni but their orientation is undefined"
+
+    pos_src = _f32c(pos_src)
+    pos_rcv = _f32c(pos_rcv)
+    if Tdiff is None:
+        Tdiff = Tmax
+    if mic_pattern is None:
+        mic_pattern = "omni"
+    if spkr_pattern is None:
+        spkr_pattern = "omni"
+    orV_rcv = np.zeros_like(pos_rcv) if orV_rcv is None else _f32c(orV_rcv)
+    orV_src = np.zeros_like(pos_src) if orV_src is None else _f32c(orV_src)
+    spkr = polar_patterns[spkr_pattern]  # KeyError for an unknown pattern, as the reference
+    mic = polar_patterns[mic_pattern]
+
+    room = _f32c(room_sz).reshape(3)
+    b = _f32c(beta).reshape(6)
+    nb = np.ascontiguousarray(np.asarray(nb_img, dtype=np.int32).reshape(3))
+    assert pos_src.ndim == 2 and pos_src.shape[1] == 3 and pos_rcv.ndim == 2 and pos_rcv.shape[1] == 3
+    assert orV_src.shape == pos_src.shape and orV_rcv.shape == pos_rcv.shape
+    lib = _lib.load()
+    ns = lib.gpurir_n_samples(float(Tmax), float(fs))
+    out = np.empty((pos_src.shape[0], pos_rcv.shape[0], ns), dtype=np.float32)
+    _lib.check(lib.gpurir_simulate_rir(_lib.fptr(room), _lib.fptr(b), _lib.fptr(pos_src), pos_src.shape[0],
+                                       _lib.fptr(pos_rcv), pos_rcv.shape[0], _lib.fptr(orV_src), _lib.fptr(orV_rcv),
+                                       spkr, mic, _lib.iptr(nb), float(Tdiff), float(Tmax), float(fs), float(c),
+                                       _lib.fptr(out)))
+    return out
+
+
+def simulateTrajectory(source_signal, RIRs, timestamps=None, fs=None):
+    """Signals of a moving source recorded by a microphone array (gpuRIR/__init__.py:177-221).
+
+    RIRs is [n_pts, n_rcv, len] (numpy, or a torch tensor already on the GPU, which then
+    stays there).  Segmentation, batched FFT convolution and the float64 overlap-add all
+    run on the GPU.  Returns float64 [len(source_signal) + len - 1, n_rcv].
+    """
+    import torch
+
+    nSamples = len(source_signal)
+    nPts = RIRs.shape[0]
+    nRcv = RIRs.shape[1]
+    lenRIR = RIRs.shape[2]
+
+    assert timestamps is None or fs is not None, "fs must be indicated for custom timestamps"
+    assert timestamps is None or timestamps[0] == 0, "The first timestamp must be 0"
+    if timestamps is None:
+        fs = nSamples / nPts
+        timestamps = np.arange(nPts)
+
+    w_ini = np.append((np.asarray(timestamps) * fs).astype(int), nSamples)
+    w_ini = np.ascontiguousarray(w_ini, dtype=np.int32)
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    if isinstance(RIRs, torch.Tensor):
+        d_rirs = RIRs.detach().to(device=dev, dtype=torch.float32).contiguous()
+    else:
+        d_rirs = torch.from_numpy(np.ascontiguousarray(RIRs, dtype=np.float32)).to(dev)
+    if isinstance(source_signal, torch.Tensor):
+        d_sig = source_signal.detach().to(device=dev, dtype=torch.float32).contiguous()
+    else:
+        d_sig = torch.from_numpy(np.ascontiguousarray(np.asarray(source_signal), dtype=np.float32)).to(dev)
+    d_out = torch.empty((nSamples + lenRIR - 1, nRcv), dtype=torch.float64, device=dev)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(lib.gpurir_trajectory_device(d_sig.data_ptr(), nSamples, d_rirs.data_ptr(), nPts, nRcv, lenRIR,
+                                            _lib.iptr(w_ini), d_out.data_ptr(), stream))
+    return d_out.cpu().numpy()
+
+
+def activateMixedPrecision(activate=True):
+    """Mixed-precision toggle (gpuRIR/__init__.py:223-232); mutually exclusive with the table."""
+    _lib.load().gpurir_activate_mixed_precision(int(bool(activate)))
+
+
+def activateLUT(activate=True):
+    """Lookup-table toggle for the sinc evaluation (gpuRIR/__init__.py:234-243)."""
+    _lib.load().gpurir_activate_lut(int(bool(activate)))
+
+
+def set_seed(seed=1234, next_call_offset=0):
+    """Noise stream of the drop-in simulateRIR: Philox key and the counter word of the next call.
+
+    No reference counterpart: the reference seeds its generator once with 1234
+    (src/gpuRIR_cuda.cu:1046-1047) and lets the stream advance from call to call."""
+    _lib.load().gpurir_set_seed(int(seed), int(next_call_offset))
+
+
+# the reference lists these spellings in __all__ (gpuRIR/__init__.py:11)
+activate_mixed_precision = activateMixedPrecision
+activate_lut = activateLUT

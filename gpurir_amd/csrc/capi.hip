@@ -1,0 +1,472 @@
+// C ABI of libgpurir (declared in include/gpurir.h): argument checks, the
+// drop-in state of the reference's simulator object, device workspaces and the
+// batch-plan object with hipGraph replay.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpurir.h"
+#include "internal.h"
+
+namespace gpurir {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+// ---- rounding rules (src/gpuRIR_cuda.cu:629, :831-835)
+int n_samples_rule(float T, float fs) {
+    const float p = T * fs;
+    int n = (int)std::ceil(p);
+    n += n % 2;
+    return n;
+}
+
+int tw_rule(float fs) { return (int)std::round(8e-3f * fs); }
+
+// ---- workspaces
+namespace {
+constexpr int MAX_DEV = 64;
+struct Ws {
+    void* ptr = nullptr;
+    size_t size = 0;
+};
+std::mutex g_ws_mu;
+Ws g_ws[MAX_DEV][WS_COUNT];
+
+struct Stage {
+    void* host = nullptr;
+    size_t size = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+std::mutex g_stage_mu;
+Stage g_stage[MAX_DEV];
+
+int current_device(int* dev) {
+    GPURIR_HIP(hipGetDevice(dev));
+    GPURIR_CHECK(*dev >= 0 && *dev < MAX_DEV, "device index out of range");
+    return 0;
+}
+}  // namespace
+
+int workspace(int slot, size_t bytes, void** ptr) {
+    int dev = 0;
+    if (int rc = current_device(&dev)) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Ws& w = g_ws[dev][slot];
+    if (bytes == 0) bytes = 256;
+    if (w.size < bytes) {
+        if (w.ptr) GPURIR_HIP(hipFree(w.ptr));  // hipFree waits for work that still uses it
+        w.ptr = nullptr;
+        w.size = 0;
+        const size_t want = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&w.ptr, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            w.ptr = nullptr;
+            set_error("out of device memory for a workspace of " + std::to_string(want) + " bytes");
+            return -3;
+        }
+        w.size = want;
+    }
+    *ptr = w.ptr;
+    return 0;
+}
+
+int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) return 0;
+    int dev = 0;
+    if (int rc = current_device(&dev)) return rc;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    Stage& s = g_stage[dev];
+    if (s.pending) {
+        GPURIR_HIP(hipEventSynchronize(s.ev));  // the previous copy has read the buffer
+        s.pending = false;
+    }
+    if (!s.ev) GPURIR_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    if (s.size < bytes) {
+        if (s.host) GPURIR_HIP(hipHostFree(s.host));
+        s.host = nullptr;
+        s.size = 0;
+        const size_t want = bytes + bytes / 4 + 4096;
+        GPURIR_HIP(hipHostMalloc(&s.host, want, hipHostMallocDefault));
+        s.size = want;
+    }
+    std::memcpy(s.host, src, bytes);
+    GPURIR_HIP(hipMemcpyAsync(dst, s.host, bytes, hipMemcpyHostToDevice, stream));
+    GPURIR_HIP(hipEventRecord(s.ev, stream));
+    s.pending = true;
+    return 0;
+}
+
+// ---- drop-in state (src/gpuRIR_cuda.h:29-33; constructor defaults src/python_bind.cpp:15, :135:
+//      mixed precision off, lookup table on; generator seeded with 1234, src/gpuRIR_cuda.cu:1047)
+namespace {
+std::mutex g_state_mu;
+bool g_mixed_precision = false;
+bool g_lookup_table = true;
+unsigned long long g_seed = 1234ULL;
+unsigned g_next_call = 0;
+
+RoomHost room_from(const float room_sz[3], const float beta[6], const int nb[3], int M_src, int M_rcv, float Tdiff,
+                   float Tmax, unsigned room_id, float level) {
+    RoomHost h;
+    std::memset(&h, 0, sizeof(h));
+    for (int d = 0; d < 3; d++) {
+        h.L[d] = room_sz[d];
+        h.nb[d] = nb[d];
+    }
+    for (int d = 0; d < 6; d++) h.beta[d] = beta[d];
+    h.n_src = M_src;
+    h.n_rcv = M_rcv;
+    h.Tdiff = Tdiff;
+    h.Tmax = Tmax;
+    h.room_id = room_id;
+    h.level = level;
+    return h;
+}
+
+int check_common(int M_src, int M_rcv, int spkr, int mic, const int nb[3]) {
+    GPURIR_CHECK(M_src > 0 && M_rcv > 0, "at least one source and one receiver are needed");
+    GPURIR_CHECK(spkr >= 0 && spkr <= 5, "spkr_pattern must be omni, homni, card, hypcard, subcard or bidir");
+    GPURIR_CHECK(mic >= 0 && mic <= 5, "mic_pattern must be omni, homni, card, hypcard, subcard or bidir");
+    GPURIR_CHECK(nb != nullptr && nb[0] > 0 && nb[1] > 0 && nb[2] > 0, "nb_img must be positive in every dimension");
+    return 0;
+}
+
+SimOptions sim_options(const gpurir_options_t* o, int spkr, int mic, float fs, float c) {
+    SimOptions s;
+    std::memset(&s, 0, sizeof(s));
+    s.precision = o ? o->precision : GPURIR_PREC_FP32;
+    s.noise = o ? o->noise : GPURIR_NOISE_LOGISTIC;
+    s.seed = o ? o->seed : 1234ULL;
+    s.call_offset = o ? o->call_offset : 0u;
+    s.chunk_images = o ? o->chunk_images : 0;
+    s.tile_samples = o ? o->tile_samples : 0;
+    s.spkr = spkr;
+    s.mic = mic;
+    s.fs = fs;
+    s.c = c;
+    return s;
+}
+}  // namespace
+
+}  // namespace gpurir
+
+using namespace gpurir;
+
+struct gpurir_plan {
+    SimLaunch* sim = nullptr;
+    void* d_plan = nullptr;
+    float* d_part = nullptr;
+    const float *pos_src = nullptr, *pos_rcv = nullptr, *orv_src = nullptr, *orv_rcv = nullptr;
+    float* out = nullptr;
+    long long out_elems = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int gpurir_version(void) { return 100; }
+
+const char* gpurir_last_error(void) { return gpurir::last_error(); }
+
+int gpurir_n_samples(float T, float fs) { return n_samples_rule(T, fs); }
+
+int gpurir_activate_mixed_precision(int activate) {
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    if (activate && g_lookup_table) {
+        std::printf("The mixed precision implementation is not compatible with the lookup table.\n");
+        std::printf("Disabling the lookup table.\n");
+        g_lookup_table = false;
+    }
+    if (activate) std::printf("This feature requires the packed-FP16 path, which this build does not provide.\n");
+    std::fflush(stdout);
+    g_mixed_precision = false;
+    return 0;
+}
+
+int gpurir_activate_lut(int activate) {
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    if (activate && g_mixed_precision) {
+        std::printf("The lookup table is not compatible with the mixed precision implementation.\n");
+        std::printf("Disabling the mixed precision implementation.\n");
+        std::fflush(stdout);
+        g_mixed_precision = false;
+    }
+    g_lookup_table = activate != 0;
+    return g_lookup_table ? 1 : 0;
+}
+
+int gpurir_set_seed(unsigned long long seed, unsigned int next_call_offset) {
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    g_seed = seed;
+    g_next_call = next_call_offset;
+    return 0;
+}
+
+int gpurir_simulate_rir_device(const float room_sz[3], const float beta[6], const float* d_pos_src, int M_src,
+                               const float* d_pos_rcv, int M_rcv, const float* d_orV_src, const float* d_orV_rcv,
+                               int spkr_pattern, int mic_pattern, const int nb_img[3], float Tdiff, float Tmax,
+                               float fs, float c, const gpurir_options_t* opt, float* d_out, void* stream) {
+    if (int rc = check_common(M_src, M_rcv, spkr_pattern, mic_pattern, nb_img)) return rc;
+    GPURIR_CHECK(room_sz && beta && d_pos_src && d_pos_rcv && d_out, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || d_orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || d_orV_rcv, "the mics are not omni but their orientation is undefined");
+    std::vector<RoomHost> rooms(1, room_from(room_sz, beta, nb_img, M_src, M_rcv, Tdiff, Tmax,
+                                             opt ? opt->room_index : 0u, opt ? opt->level : 1.0f));
+    SimOptions so = sim_options(opt, spkr_pattern, mic_pattern, fs, c);
+    return launch_simulation(rooms, d_pos_src, d_pos_rcv, d_orV_src, d_orV_rcv, so, d_out,
+                             static_cast<hipStream_t>(stream));
+}
+
+int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float* pos_src, int M_src,
+                        const float* pos_rcv, int M_rcv, const float* orV_src, const float* orV_rcv,
+                        int spkr_pattern, int mic_pattern, const int nb_img[3], float Tdiff, float Tmax,
+                        float fs, float c, float* out) {
+    if (int rc = check_common(M_src, M_rcv, spkr_pattern, mic_pattern, nb_img)) return rc;
+    GPURIR_CHECK(room_sz && beta && pos_src && pos_rcv && out, "null pointer argument");
+    GPURIR_CHECK(Tdiff <= Tmax, "Tmax must be equal or greater than Tdiff");
+    gpurir_options_t o;
+    std::memset(&o, 0, sizeof(o));
+    {
+        std::lock_guard<std::mutex> lk(g_state_mu);
+        o.precision = g_lookup_table ? GPURIR_PREC_LUT : GPURIR_PREC_FP32;
+        o.seed = g_seed;
+        o.call_offset = g_next_call++;
+    }
+    o.noise = GPURIR_NOISE_LOGISTIC;
+    o.level = 1.0f;
+    const int nS = n_samples_rule(Tmax, fs);
+    const size_t n_out = (size_t)M_src * M_rcv * nS;
+    const size_t ps = (size_t)M_src * 3, pr = (size_t)M_rcv * 3;
+    const size_t pad = 64;
+    const size_t total = (2 * ps + 2 * pr + 4 * pad) + n_out;
+    void* wsv = nullptr;
+    if (int rc = workspace(WS_STAGE, sizeof(float) * total, &wsv)) return rc;
+    float* w = static_cast<float*>(wsv);
+    float* d_ps = w;
+    float* d_pr = d_ps + ps + pad;
+    float* d_os = d_pr + pr + pad;
+    float* d_or = d_os + ps + pad;
+    float* d_out = d_or + pr + pad;
+    std::vector<float> zs(ps, 0.0f), zr(pr, 0.0f);
+    const hipStream_t st = nullptr;
+    GPURIR_HIP(hipMemcpyAsync(d_ps, pos_src, sizeof(float) * ps, hipMemcpyHostToDevice, st));
+    GPURIR_HIP(hipMemcpyAsync(d_pr, pos_rcv, sizeof(float) * pr, hipMemcpyHostToDevice, st));
+    GPURIR_HIP(hipMemcpyAsync(d_os, orV_src ? orV_src : zs.data(), sizeof(float) * ps, hipMemcpyHostToDevice, st));
+    GPURIR_HIP(hipMemcpyAsync(d_or, orV_rcv ? orV_rcv : zr.data(), sizeof(float) * pr, hipMemcpyHostToDevice, st));
+    if (int rc = gpurir_simulate_rir_device(room_sz, beta, d_ps, M_src, d_pr, M_rcv, d_os, d_or, spkr_pattern,
+                                            mic_pattern, nb_img, Tdiff, Tmax, fs, c, &o, d_out, st))
+        return rc;
+    GPURIR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * n_out, hipMemcpyDeviceToHost, st));
+    GPURIR_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gpurir_image_list_device(const float room_sz[3], const float beta[6], const float* d_pos_src, int M_src,
+                             const float* d_pos_rcv, int M_rcv, const float* d_orV_src, const float* d_orV_rcv,
+                             int spkr_pattern, int mic_pattern, const int nb_img[3], float fs, float c, int* d_n0,
+                             float* d_frac, float* d_amp, void* stream) {
+    if (int rc = check_common(M_src, M_rcv, spkr_pattern, mic_pattern, nb_img)) return rc;
+    GPURIR_CHECK(room_sz && beta && d_pos_src && d_pos_rcv && d_n0 && d_frac && d_amp, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || d_orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || d_orV_rcv, "the mics are not omni but their orientation is undefined");
+    RoomHost h = room_from(room_sz, beta, nb_img, M_src, M_rcv, 0.0f, 0.0f, 0u, 1.0f);
+    SimOptions so = sim_options(nullptr, spkr_pattern, mic_pattern, fs, c);
+    return launch_image_list(h, d_pos_src, d_pos_rcv, d_orV_src, d_orV_rcv, so, d_n0, d_frac, d_amp,
+                             static_cast<hipStream_t>(stream));
+}
+
+static int batch_rooms(int n_rooms, const float* room_sz, const float* beta, const int* nb_img, const int* n_src,
+                       const int* n_rcv, const float* Tdiff, const float* Tmax, const float* level, float fs,
+                       const gpurir_options_t* opt, const long long* out_offsets, std::vector<RoomHost>& rooms,
+                       long long* out_elems) {
+    GPURIR_CHECK(n_rooms > 0, "n_rooms must be positive");
+    GPURIR_CHECK(room_sz && beta && nb_img && n_src && n_rcv && Tdiff && Tmax, "null pointer argument");
+    rooms.clear();
+    rooms.reserve(n_rooms);
+    int so = 0, ro = 0;
+    long long oo = 0, end = 0;
+    for (int k = 0; k < n_rooms; k++) {
+        GPURIR_CHECK(n_src[k] > 0 && n_rcv[k] > 0, "each room needs at least one source and one receiver");
+        RoomHost h = room_from(room_sz + 3 * k, beta + 6 * k, nb_img + 3 * k, n_src[k], n_rcv[k], Tdiff[k], Tmax[k],
+                               (opt ? opt->room_index : 0u) + (unsigned)k, level ? level[k] : 1.0f);
+        h.src_off = so;
+        h.rcv_off = ro;
+        const long long sz = (long long)n_src[k] * n_rcv[k] * n_samples_rule(Tmax[k], fs);
+        h.out_off = out_offsets ? out_offsets[k] : oo;
+        GPURIR_CHECK(h.out_off >= 0, "negative output offset");
+        end = std::max(end, h.out_off + sz);
+        oo += sz;
+        so += n_src[k];
+        ro += n_rcv[k];
+        rooms.push_back(h);
+    }
+    *out_elems = end;
+    return 0;
+}
+
+int gpurir_simulate_batch_device(int n_rooms, const float* room_sz, const float* beta, const int* nb_img,
+                                 const int* n_src, const int* n_rcv, const float* Tdiff, const float* Tmax,
+                                 const float* level, const float* d_pos_src, const float* d_pos_rcv,
+                                 const float* d_orV_src, const float* d_orV_rcv, int spkr_pattern, int mic_pattern,
+                                 float fs, float c, const gpurir_options_t* opt, float* d_out,
+                                 const long long* out_offsets, void* stream) {
+    GPURIR_CHECK(spkr_pattern >= 0 && spkr_pattern <= 5 && mic_pattern >= 0 && mic_pattern <= 5, "invalid polar pattern");
+    GPURIR_CHECK(d_pos_src && d_pos_rcv && d_out, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || d_orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || d_orV_rcv, "the mics are not omni but their orientation is undefined");
+    std::vector<RoomHost> rooms;
+    long long elems = 0;
+    if (int rc = batch_rooms(n_rooms, room_sz, beta, nb_img, n_src, n_rcv, Tdiff, Tmax, level, fs, opt, out_offsets,
+                             rooms, &elems))
+        return rc;
+    SimOptions so = sim_options(opt, spkr_pattern, mic_pattern, fs, c);
+    return launch_simulation(rooms, d_pos_src, d_pos_rcv, d_orV_src, d_orV_rcv, so, d_out,
+                             static_cast<hipStream_t>(stream));
+}
+
+int gpurir_batch_plan_create(int n_rooms, const float* room_sz, const float* beta, const int* nb_img,
+                             const int* n_src, const int* n_rcv, const float* Tdiff, const float* Tmax,
+                             const float* level, const float* d_pos_src, const float* d_pos_rcv,
+                             const float* d_orV_src, const float* d_orV_rcv, int spkr_pattern, int mic_pattern,
+                             float fs, float c, const gpurir_options_t* opt, float* d_out,
+                             const long long* out_offsets, gpurir_plan_t* plan) {
+    GPURIR_CHECK(plan != nullptr, "null plan pointer");
+    *plan = nullptr;
+    GPURIR_CHECK(spkr_pattern >= 0 && spkr_pattern <= 5 && mic_pattern >= 0 && mic_pattern <= 5, "invalid polar pattern");
+    GPURIR_CHECK(d_pos_src && d_pos_rcv && d_out, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || d_orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || d_orV_rcv, "the mics are not omni but their orientation is undefined");
+    std::vector<RoomHost> rooms;
+    long long elems = 0;
+    if (int rc = batch_rooms(n_rooms, room_sz, beta, nb_img, n_src, n_rcv, Tdiff, Tmax, level, fs, opt, out_offsets,
+                             rooms, &elems))
+        return rc;
+    SimOptions so = sim_options(opt, spkr_pattern, mic_pattern, fs, c);
+    gpurir_plan* p = new gpurir_plan();
+    p->sim = sim_new();
+    if (int rc = sim_prepare(rooms, so, *p->sim)) {
+        sim_delete(p->sim);
+        delete p;
+        return rc;
+    }
+    auto fail = [&](int rc) {
+        if (p->d_plan) (void)hipFree(p->d_plan);
+        if (p->d_part) (void)hipFree(p->d_part);
+        sim_delete(p->sim);
+        delete p;
+        return rc;
+    };
+    if (hipMalloc(&p->d_plan, sim_plan_bytes(*p->sim)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for the batch plan");
+        return fail(-3);
+    }
+    if (sim_partial_bytes(*p->sim) > 0 &&
+        hipMalloc(reinterpret_cast<void**>(&p->d_part), sim_partial_bytes(*p->sim)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for the batch plan's partial sums");
+        return fail(-3);
+    }
+    if (hipMemcpy(p->d_plan, sim_blob(*p->sim), sim_plan_bytes(*p->sim), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("plan upload failed");
+        return fail(-2);
+    }
+    p->pos_src = d_pos_src;
+    p->pos_rcv = d_pos_rcv;
+    p->orv_src = d_orV_src;
+    p->orv_rcv = d_orV_rcv;
+    p->out = d_out;
+    p->out_elems = elems;
+    *plan = p;
+    return 0;
+}
+
+int gpurir_batch_plan_run(gpurir_plan_t p, void* stream) {
+    GPURIR_CHECK(p != nullptr, "null plan");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!p->exec) {
+        GPURIR_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        GPURIR_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = sim_enqueue(*p->sim, p->d_plan, p->d_part, p->pos_src, p->pos_rcv, p->orv_src, p->orv_rcv, p->out,
+                             p->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+        if (rc) return rc;
+        GPURIR_HIP(e);
+        p->graph = g;
+        GPURIR_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    }
+    GPURIR_HIP(hipGraphLaunch(p->exec, st));
+    return 0;
+}
+
+int gpurir_batch_plan_count_taps(gpurir_plan_t p, long long* taps, void* stream) {
+    GPURIR_CHECK(p != nullptr && taps != nullptr, "null argument");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    void* w = nullptr;
+    if (int rc = workspace(WS_GRAD, sizeof(unsigned long long), &w)) return rc;
+    GPURIR_HIP(hipMemsetAsync(w, 0, sizeof(unsigned long long), st));
+    if (int rc = sim_count_taps(*p->sim, p->d_plan, p->pos_src, p->pos_rcv, static_cast<unsigned long long*>(w), st))
+        return rc;
+    unsigned long long h = 0;
+    GPURIR_HIP(hipMemcpyAsync(&h, w, sizeof(h), hipMemcpyDeviceToHost, st));
+    GPURIR_HIP(hipStreamSynchronize(st));
+    *taps = (long long)h;
+    return 0;
+}
+
+int gpurir_batch_plan_profile(gpurir_plan_t p, void* stream, float* ms_accumulate, float* ms_finalize) {
+    GPURIR_CHECK(p != nullptr && ms_accumulate != nullptr && ms_finalize != nullptr, "null argument");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1, e2;
+    GPURIR_HIP(hipEventCreate(&e0));
+    GPURIR_HIP(hipEventCreate(&e1));
+    GPURIR_HIP(hipEventCreate(&e2));
+    int rc = 0;
+    if (hipEventRecord(e0, st) != hipSuccess) rc = -2;
+    if (!rc) rc = sim_enqueue_stage(*p->sim, 1, p->d_plan, p->d_part, p->pos_src, p->pos_rcv, p->orv_src, p->orv_rcv,
+                                    p->out, st);
+    if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = -2;
+    if (!rc) rc = sim_enqueue_stage(*p->sim, 2, p->d_plan, p->d_part, p->pos_src, p->pos_rcv, p->orv_src, p->orv_rcv,
+                                    p->out, st);
+    if (!rc && hipEventRecord(e2, st) != hipSuccess) rc = -2;
+    if (!rc && hipEventSynchronize(e2) != hipSuccess) rc = -2;
+    if (!rc) {
+        (void)hipEventElapsedTime(ms_accumulate, e0, e1);
+        (void)hipEventElapsedTime(ms_finalize, e1, e2);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (rc == -2) set_error("HIP error while profiling the plan");
+    return rc;
+}
+
+long long gpurir_batch_plan_out_elems(gpurir_plan_t p) { return p ? p->out_elems : -1; }
+long long gpurir_batch_plan_n_rirs(gpurir_plan_t p) { return p ? sim_n_rirs(*p->sim) : -1; }
+
+int gpurir_batch_plan_destroy(gpurir_plan_t p) {
+    if (!p) return 0;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->d_plan) (void)hipFree(p->d_plan);
+    if (p->d_part) (void)hipFree(p->d_part);
+    sim_delete(p->sim);
+    delete p;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,351 @@
+// Stage (d): batched FFT convolution and trajectory overlap-add for gfx950.
+//
+// Reference path (DavidDiazGuerra/gpuRIR v1.2.0):
+//   gpuRIR/__init__.py:177-221 simulateTrajectory (segmentation, float64 OLA on the host)
+//   src/gpuRIR_cuda.cu:933-1023 cuda_convolutions (cuFFT R2C/C2R, N_fft = pow2roundup(seg+L-1))
+//   src/gpuRIR_cuda.cu:463-482  complexPointwiseMulAndScale
+//
+// Design: a real length-N transform is a complex length-H = N/2 transform of
+// z[n] = x[2n] + i x[2n+1] plus an O(H) split.  One workgroup holds one
+// length-H complex sequence in LDS (H <= 16384, 128 KiB) and runs
+//   forward: radix-2 decimation in frequency (natural in, bit-reversed out),
+//   split + product with the segment spectrum + inverse split (pairs k, H-k),
+//   inverse: radix-2 decimation in time (bit-reversed in, natural out),
+// so no permutation pass is needed.  Segment spectra are computed once per
+// trajectory point and shared by that point's receivers through L2.  The
+// overlap-add is a gather: each output sample adds the segments that cover it
+// in ascending segment order, in FP64 (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/gpurir.h"
+#include "internal.h"
+
+namespace gpurir {
+namespace {
+
+constexpr int CWG = 512;
+
+struct ConvArgs {
+    const float* src;       // segment source (signal or segment matrix)
+    const int* seg_start;   // [P] element offset of segment p in src
+    const int* seg_len;     // [P] samples of segment p
+    const float* rirs;      // [P][R][L]
+    const float2* spec;     // [P][H+1] segment spectra (natural order)
+    float2* spec_out;
+    const float2* tw;       // [H/2] e^{-2 pi i m / H}
+    const float2* twN;      // [H+1] e^{-2 pi i k / N}
+    float* conv;            // [P][R][conv_stride]
+    int P, R, L, H, logH;
+    int conv_stride;
+};
+
+__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+__device__ inline int brev(int k, int logH) { return logH ? (int)(__brev((unsigned)k) >> (32 - logH)) : 0; }
+
+// In-place radix-2 DIF: natural order in, bit-reversed order out.
+__device__ void fft_dif(float2* x, const float2* __restrict__ tw, int H, int logH) {
+    for (int s = 0; s < logH; s++) {
+        const int h = H >> (s + 1);
+        const int tstride = 1 << s;  // W_{2h}^j = W_H^{j * H/(2h)}
+        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
+            const int j = b & (h - 1);
+            const int i = ((b >> (logH - s - 1)) << (logH - s)) + j;
+            const float2 a = x[i], c = x[i + h];
+            x[i] = cadd(a, c);
+            x[i + h] = cmul(csub(a, c), tw[j * tstride]);
+        }
+        __syncthreads();
+    }
+}
+
+// In-place radix-2 DIT with conjugate twiddles: bit-reversed in, natural out (unnormalized inverse).
+__device__ void ifft_dit(float2* x, const float2* __restrict__ tw, int H, int logH) {
+    for (int s = logH - 1; s >= 0; s--) {
+        const int h = H >> (s + 1);
+        const int tstride = 1 << s;
+        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
+            const int j = b & (h - 1);
+            const int i = ((b >> (logH - s - 1)) << (logH - s)) + j;
+            const float2 a = x[i];
+            const float2 c = cmulc(x[i + h], tw[j * tstride]);
+            x[i] = cadd(a, c);
+            x[i + h] = csub(a, c);
+        }
+        __syncthreads();
+    }
+}
+
+// z[n] = x[2n] + i x[2n+1] for x = src[0:len] zero-padded to 2H.
+__device__ void load_real(float2* z, const float* __restrict__ src, int len, int H) {
+    for (int n = threadIdx.x; n < H; n += blockDim.x) {
+        const int e = 2 * n;
+        z[n] = make_float2(e < len ? src[e] : 0.0f, e + 1 < len ? src[e + 1] : 0.0f);
+    }
+    __syncthreads();
+}
+
+// X[k] and X[H-k] of the real transform from the bit-reversed half-length spectrum.
+__device__ inline void split_pair(const float2* z, const float2* __restrict__ twN, int k, int H, int logH, float2& Xk,
+                                  float2& Xj) {
+    const int j = H - k;
+    const float2 A = z[brev(k & (H - 1), logH)];
+    const float2 Bc = conjf2(z[brev(j & (H - 1), logH)]);
+    const float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
+    const float2 D = csub(A, Bc);
+    const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);  // -i D / 2
+    const float2 WO = cmul(twN[k], O);
+    Xk = cadd(E, WO);
+    Xj = conjf2(csub(E, WO));
+}
+
+__global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
+    extern __shared__ __align__(16) float2 z[];
+    const int p = blockIdx.x;
+    load_real(z, a.src + a.seg_start[p], a.seg_len[p], a.H);
+    fft_dif(z, a.tw, a.H, a.logH);
+    float2* S = a.spec_out + (long long)p * (a.H + 1);
+    for (int k = threadIdx.x; k <= a.H / 2; k += blockDim.x) {
+        float2 Xk, Xj;
+        split_pair(z, a.twN, k, a.H, a.logH, Xk, Xj);
+        S[k] = Xk;
+        S[a.H - k] = Xj;
+    }
+}
+
+// One workgroup per (trajectory point p, receiver r): conv[p][r][0 : seg_len[p] + L - 1].
+__global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
+    extern __shared__ __align__(16) float2 z[];
+    const int p = blockIdx.x / a.R, r = blockIdx.x - p * a.R;
+    const int H = a.H, logH = a.logH;
+    load_real(z, a.rirs + ((long long)p * a.R + r) * a.L, a.L, H);
+    fft_dif(z, a.tw, H, logH);
+    const float2* S = a.spec + (long long)p * (H + 1);
+    const float scale = 1.0f / (float)H;
+    // product and inverse split, pairs (k, H-k) handled by one thread (in place)
+    for (int k = threadIdx.x; k <= H / 2; k += blockDim.x) {
+        const int j = H - k;
+        float2 Xk, Xj;
+        split_pair(z, a.twN, k, H, logH, Xk, Xj);
+        float2 Yk = cmul(Xk, S[k]), Yj = cmul(Xj, S[j]);
+        Yk = make_float2(Yk.x * scale, Yk.y * scale);
+        Yj = make_float2(Yj.x * scale, Yj.y * scale);
+        const float2 Yjc = conjf2(Yj);
+        const float2 E = make_float2(0.5f * (Yk.x + Yjc.x), 0.5f * (Yk.y + Yjc.y));
+        const float2 Dm = csub(Yk, Yjc);
+        const float2 O = cmulc(make_float2(0.5f * Dm.x, 0.5f * Dm.y), a.twN[k]);  // (Yk - conj Yj) W^-k / 2
+        const float2 Zk = make_float2(E.x - O.y, E.y + O.x);                      // E + i O
+        const float2 Zj = make_float2(E.x + O.y, -E.y + O.x);                     // conj(E) + i conj(O)
+        // both positions were read above by this thread only (pairs are disjoint)
+        if (k == 0) {
+            z[0] = Zk;
+        } else if (k == H / 2) {
+            z[brev(k, logH)] = Zk;
+        } else {
+            z[brev(k, logH)] = Zk;
+            z[brev(j, logH)] = Zj;
+        }
+    }
+    __syncthreads();
+    ifft_dit(z, a.tw, H, logH);
+    float* out = a.conv + ((long long)p * a.R + r) * a.conv_stride;
+    const int len = a.seg_len[p] + a.L - 1;
+    for (int t = threadIdx.x; t < len; t += blockDim.x) {
+        const float2 v = z[t >> 1];
+        out[t] = (t & 1) ? v.y : v.x;
+    }
+}
+
+// out[t][r] = sum over segments p covering t of conv[p][r][t - w_ini[p]], FP64, ascending p.
+__global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv, const int* __restrict__ w_ini,
+                                                  int P, int R, int L, int conv_stride, long long n_out,
+                                                  double* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_out) return;
+    // last p with w_ini[p] <= t
+    int lo = 0, hi = P - 1, ph = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)w_ini[mid] <= t) {
+            ph = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    // first p with w_ini[p+1] + L - 1 > t
+    lo = 0;
+    hi = P - 1;
+    int pl = P;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)w_ini[mid + 1] + L - 1 > t) {
+            pl = mid;
+            hi = mid - 1;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    for (int r = 0; r < R; r++) {
+        double acc = 0.0;
+        for (int p = pl; p <= ph; p++) acc += (double)conv[((long long)p * R + r) * conv_stride + (t - w_ini[p])];
+        out[t * R + r] = acc;
+    }
+}
+
+struct FftTables {
+    std::vector<float2> tw, twN;
+};
+
+FftTables make_fft_tables(int H) {
+    FftTables T;
+    T.tw.resize(std::max(H / 2, 1));
+    for (int m = 0; m < H / 2; m++) {
+        const double a = -2.0 * M_PI * (double)m / (double)H;
+        T.tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    T.twN.resize(H + 1);
+    for (int k = 0; k <= H; k++) {
+        const double a = -2.0 * M_PI * (double)k / (2.0 * (double)H);
+        T.twN[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    return T;
+}
+
+unsigned pow2roundup(unsigned x) {
+    unsigned p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+size_t al(size_t x) { return (x + 255) / 256 * 256; }
+
+// Shared driver: segments of src given by (seg_start, seg_len) per point.
+// Writes conv [P][R][conv_stride] into *conv_out (workspace) and returns the stride.
+int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::vector<int>& seg_len, const float* d_rirs,
+             int P, int R, int L, hipStream_t stream, float** conv_out, int* conv_stride, void** extra,
+             size_t extra_bytes) {
+    int seg_max = 0;
+    for (int v : seg_len) seg_max = std::max(seg_max, v);
+    const unsigned N = pow2roundup((unsigned)std::max(seg_max + L - 1, 2));
+    const int H = (int)(N / 2);
+    GPURIR_CHECK(H <= 16384, "convolution too long for one LDS-resident FFT (N_fft > 32768)");
+    int logH = 0;
+    while ((1 << logH) < H) logH++;
+    FftTables T = make_fft_tables(H);
+    const int stride = (int)((seg_max + L - 1 + 63) / 64 * 64);
+    const size_t b_ss = al(sizeof(int) * P), b_sl = al(sizeof(int) * P);
+    const size_t b_tw = al(sizeof(float2) * T.tw.size()), b_twN = al(sizeof(float2) * T.twN.size());
+    const size_t b_spec = al(sizeof(float2) * (size_t)P * (H + 1));
+    const size_t b_conv = al(sizeof(float) * (size_t)P * R * stride);
+    const size_t b_small = b_ss + b_sl + b_tw + b_twN;
+    void* ws = nullptr;
+    if (int rc = workspace(WS_CONV, b_small + b_spec + b_conv + al(extra_bytes), &ws)) return rc;
+    unsigned char* w = static_cast<unsigned char*>(ws);
+    std::vector<unsigned char> host(b_small, 0);
+    std::memcpy(host.data(), seg_start.data(), sizeof(int) * P);
+    std::memcpy(host.data() + b_ss, seg_len.data(), sizeof(int) * P);
+    std::memcpy(host.data() + b_ss + b_sl, T.tw.data(), sizeof(float2) * T.tw.size());
+    std::memcpy(host.data() + b_ss + b_sl + b_tw, T.twN.data(), sizeof(float2) * T.twN.size());
+    if (int rc = upload_async(w, host.data(), b_small, stream)) return rc;
+    ConvArgs a;
+    a.src = d_src;
+    a.seg_start = reinterpret_cast<const int*>(w);
+    a.seg_len = reinterpret_cast<const int*>(w + b_ss);
+    a.tw = reinterpret_cast<const float2*>(w + b_ss + b_sl);
+    a.twN = reinterpret_cast<const float2*>(w + b_ss + b_sl + b_tw);
+    a.spec_out = reinterpret_cast<float2*>(w + b_small);
+    a.spec = a.spec_out;
+    a.conv = reinterpret_cast<float*>(w + b_small + b_spec);
+    a.rirs = d_rirs;
+    a.P = P;
+    a.R = R;
+    a.L = L;
+    a.H = H;
+    a.logH = logH;
+    a.conv_stride = stride;
+    const size_t lds = sizeof(float2) * (size_t)H;
+    if (lds > 64 * 1024) {
+        GPURIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&segment_spectrum_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        GPURIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    hipLaunchKernelGGL(segment_spectrum_kernel, dim3(P), dim3(CWG), lds, stream, a);
+    GPURIR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(conv_kernel, dim3((unsigned)P * R), dim3(CWG), lds, stream, a);
+    GPURIR_HIP(hipGetLastError());
+    *conv_out = a.conv;
+    *conv_stride = stride;
+    if (extra) *extra = (extra_bytes > 0) ? (w + b_small + b_spec + b_conv) : nullptr;
+    return 0;
+}
+
+}  // namespace
+}  // namespace gpurir
+
+using namespace gpurir;
+
+extern "C" {
+
+int gpurir_conv(const float* segments, int M_src, int seg_len, const float* rirs, int M_rcv, int rir_len, float* out) {
+    GPURIR_CHECK(segments && rirs && out, "null pointer argument");
+    GPURIR_CHECK(M_src > 0 && M_rcv > 0 && seg_len > 0 && rir_len > 0, "sizes must be positive");
+    const hipStream_t st = nullptr;
+    const size_t nseg = (size_t)M_src * seg_len, nrir = (size_t)M_src * M_rcv * rir_len;
+    void* sv = nullptr;
+    if (int rc = workspace(WS_STAGE, sizeof(float) * (nseg + nrir + 128), &sv)) return rc;
+    float* d_seg = static_cast<float*>(sv);
+    float* d_rir = d_seg + (nseg + 63) / 64 * 64;
+    GPURIR_HIP(hipMemcpyAsync(d_seg, segments, sizeof(float) * nseg, hipMemcpyHostToDevice, st));
+    GPURIR_HIP(hipMemcpyAsync(d_rir, rirs, sizeof(float) * nrir, hipMemcpyHostToDevice, st));
+    std::vector<int> ss(M_src), sl(M_src, seg_len);
+    for (int p = 0; p < M_src; p++) ss[p] = p * seg_len;
+    float* conv = nullptr;
+    int stride = 0;
+    if (int rc = run_conv(d_seg, ss, sl, d_rir, M_src, M_rcv, rir_len, st, &conv, &stride, nullptr, 0)) return rc;
+    const int len = seg_len + rir_len - 1;
+    GPURIR_HIP(hipMemcpy2DAsync(out, sizeof(float) * len, conv, sizeof(float) * stride, sizeof(float) * len,
+                                (size_t)M_src * M_rcv, hipMemcpyDeviceToHost, st));
+    GPURIR_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv,
+                             int rir_len, const int* w_ini, double* d_out, void* stream) {
+    GPURIR_CHECK(d_signal && d_rirs && w_ini && d_out, "null pointer argument");
+    GPURIR_CHECK(n_signal > 0 && n_pts > 0 && n_rcv > 0 && rir_len > 0, "sizes must be positive");
+    GPURIR_CHECK(w_ini[0] >= 0 && w_ini[n_pts] == n_signal, "w_ini must start at >= 0 and end at n_signal");
+    std::vector<int> ss(n_pts), sl(n_pts);
+    for (int p = 0; p < n_pts; p++) {
+        GPURIR_CHECK(w_ini[p + 1] >= w_ini[p], "segment starts must be non-decreasing");
+        ss[p] = w_ini[p];
+        sl[p] = w_ini[p + 1] - w_ini[p];
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    float* conv = nullptr;
+    int stride = 0;
+    void* extra = nullptr;
+    if (int rc = run_conv(d_signal, ss, sl, d_rirs, n_pts, n_rcv, rir_len, st, &conv, &stride, &extra,
+                          sizeof(int) * (n_pts + 1)))
+        return rc;
+    if (int rc = upload_async(extra, w_ini, sizeof(int) * (n_pts + 1), st)) return rc;
+    const long long n_out = (long long)n_signal + rir_len - 1;
+    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, st, conv,
+                       static_cast<const int*>(extra), n_pts, n_rcv, rir_len, stride, n_out, d_out);
+    GPURIR_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,859 @@
+// Image-source-method kernels for gfx950 (stages a, b and c of simulateRIR).
+//
+// Reference path (DavidDiazGuerra/gpuRIR v1.2.0, src/gpuRIR_cuda.cu):
+//   calcAmpTau_kernel :245-375   -> image_eval() below, fused into the accumulation
+//   generateRIR_kernel :377-390, reduceRIR_kernel :392-413, cuda_rirGenerator :618-674
+//                                 -> ism_accumulate_kernel + ism_finalize_kernel
+//   generateRIR_kernel_lut :559-572, create_sinc_texture_lut :578-616
+//                                 -> MODE == PREC_LUT (LDS table, FP32 lerp)
+//   envPred_kernel :415-443, diffRev_kernel :445-461 (+ cuRAND :872-886)
+//                                 -> ism_finalize_kernel (Philox4x32-10 noise)
+//
+// Design (one workgroup = one work item = one RIR x one image chunk x one time tile):
+//   * stage (a) per image in FP64 for the delay (image vector is exact for float
+//     inputs; sqrt and one multiply by fs/c), split into an integer sample n0 and
+//     a float fraction f;  amplitude, directivities and tap constants in FP32;
+//   * images whose window misses the tile are culled; live ones are compacted
+//     into an LDS queue with wave ballots;
+//   * stage (b) scatters each live image's taps into an LDS-resident partial RIR
+//     with ds_add_f32 (no global atomics).  Each lane owns one image and walks
+//     its taps; the windowed sinc uses the identities
+//        sin(pi (j - f))        = (-1)^(j+1) sin(pi f)
+//        cos(2 pi (j - f) / Tw) = cos(2 pi j/Tw) cos(2 pi f/Tw) + sin(2 pi j/Tw) sin(2 pi f/Tw)
+//     so a tap costs one subtraction, one reciprocal, two FMAs and one multiply;
+//   * the tile is flushed with coalesced stores, either straight into the output
+//     (one chunk per RIR) or into a partial row that ism_finalize_kernel adds up
+//     in a fixed order (deterministic across runs for a given plan);
+//   * ism_finalize_kernel also computes the Sabine envelope from the last 10 ms of
+//     the early part and writes the diffuse tail.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "internal.h"
+
+namespace gpurir {
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int NWAVE = WG / 64;
+constexpr int QCAP = 1024;
+
+struct AccArgs {
+    const RoomDev* rooms;
+    const Item* items;
+    const float* pos_src;
+    const float* pos_rcv;
+    const float* orv_src;
+    const float* orv_rcv;
+    const float2* jtab;   // per tap offset j: (sigma_j cos(2 pi j/Tw), sigma_j sin(2 pi j/Tw)), sigma_j = (-1)^(j+1)
+    const float2* lut;    // LUT mode: (T[i], T[i+1]-T[i])
+    float* out;
+    float* part;
+    int spkr, mic;
+    int J0, nJ, nJp;      // first tap offset, number of offsets, padded to a multiple of 8
+    float halfTw;         // support |t - tau| < halfTw
+    float inv_tw2;        // 2/Tw
+    int lut_len;
+    int ntab_max;         // max Nx+Ny+Nz over the rooms of the launch
+    int acc_len;          // floats of the LDS accumulator
+};
+
+struct FinArgs {
+    const RoomDev* rooms;
+    const int* rir_room;
+    const float* pos_src;
+    const float* pos_rcv;
+    float* out;
+    const float* part;
+    float w_sz;           // float(10e-3f * fs), src/gpuRIR_cuda.cu:421
+    double fs;
+    unsigned long long seed;
+    unsigned call_offset;
+    int noise;
+};
+
+// Integer power by squaring (e >= 0); 0^0 = 1 as powf(0,0) in the reference.
+__device__ inline double ipow(double b, int e) {
+    double r = 1.0;
+    while (e > 0) {
+        if (e & 1) r *= b;
+        b *= b;
+        e >>= 1;
+    }
+    return r;
+}
+
+// src/gpuRIR_cuda.cu:139-146 (cosTheta already computed)
+__device__ inline float pattern_gain(float c, int p) {
+    switch (p) {
+        case 1: return c > 0.0f ? 1.0f : 0.0f;
+        case 2: return 0.5f + 0.5f * c;
+        case 3: return 0.25f + 0.75f * c;
+        case 4: return 0.75f + 0.25f * c;
+        case 5: return c;
+        default: return 0.0f;
+    }
+}
+
+__device__ inline void lds_add(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Per-axis tables of one RIR: offs[e] = 2 m L + (1-2q) src - rcv (FP64, exact for
+// float inputs) and att[e] = (-b_lo)^|m-q| (-b_hi)^|m| (src/gpuRIR_cuda.cu:336-340,353).
+// Axis 0 occupies [0, Nx), axis 1 [Nx, Nx+Ny), axis 2 the rest.
+__device__ inline void build_axis_tables(const RoomDev& R, const float* ps, const float* pr, double* offs, float* att,
+                                         int tid, int nthreads) {
+    const int Nx = R.nb[0], Ny = R.nb[1];
+    const int ntab = Nx + Ny + R.nb[2];
+    for (int e = tid; e < ntab; e += nthreads) {
+        const int d = (e < Nx) ? 0 : (e < Nx + Ny ? 1 : 2);
+        const int n = e - (d == 0 ? 0 : (d == 1 ? Nx : Nx + Ny));
+        const int N = R.nb[d];
+        const int k = n - N / 2;
+        const int m = (k + 1) >> 1;  // ceil(k / 2)
+        const int q = k & 1;         // |k % 2|
+        const double sgn = (double)(1 - 2 * q);
+        const double av = 2.0 * (double)m * R.L[d];
+        const double bv = sgn * (double)ps[d];
+        offs[e] = (av + bv) - (double)pr[d];
+        const double b0 = -(double)R.beta[2 * d], b1 = -(double)R.beta[2 * d + 1];
+        att[e] = (float)(ipow(b0, abs(m - q)) * ipow(b1, abs(m)));
+    }
+}
+
+struct ImageVal {
+    double tau;
+    float amp;
+};
+
+// Stage (a) for image i of one RIR (src/gpuRIR_cuda.cu:336-369).
+__device__ inline ImageVal image_eval(const RoomDev& R, int i, const double* offs, const float* att, int spkr,
+                                      int mic, const float os[4], const float orr[4]) {
+    const int Nx = R.nb[0], Ny = R.nb[1], Nz = R.nb[2];
+    const int nyz = Ny * Nz;
+    const int nx = (int)(((double)i + 0.5) * R.inv_nyz);
+    const int rem = i - nx * nyz;
+    const int ny = (int)(((double)rem + 0.5) * R.inv_nz);
+    const int nz = rem - ny * Nz;
+    const double vx = offs[nx], vy = offs[Nx + ny], vz = offs[Nx + Ny + nz];
+    double d2 = vx * vx;
+    d2 = d2 + vy * vy;
+    d2 = d2 + vz * vz;
+    const double dist = sqrt(d2);
+    ImageVal r;
+    r.tau = dist * R.fs_over_c;
+    const float distf = (float)dist;
+    float amp = att[nx] * att[Nx + ny] * att[Nx + Ny + nz];
+    amp *= __builtin_amdgcn_rcpf(12.566370614359172f * distf);
+    if (spkr != 0 || mic != 0) {
+        const float inv_d = __builtin_amdgcn_rcpf(distf);
+        const float fx = (float)vx, fy = (float)vy, fz = (float)vz;
+        if (mic != 0) {
+            // doa = rcv -> image, receiver orientation as given
+            const float c = (fx * orr[0] + fy * orr[1] + fz * orr[2]) * inv_d * orr[3];
+            amp *= pattern_gain(c, mic);
+        }
+        if (spkr != 0) {
+            // doa = image -> rcv, source orientation mirrored per reflected axis
+            const int qx = (nx - Nx / 2) & 1, qy = (ny - Ny / 2) & 1, qz = (nz - Nz / 2) & 1;
+            const float ox = qx ? -os[0] : os[0], oy = qy ? -os[1] : os[1], oz = qz ? -os[2] : os[2];
+            const float c = -(fx * ox + fy * oy + fz * oz) * inv_d * os[3];
+            amp *= pattern_gain(c, spkr);
+        }
+    }
+    r.amp = amp;
+    return r;
+}
+
+__device__ inline void load_orient(const float* o, float out[4]) {
+    out[0] = o[0];
+    out[1] = o[1];
+    out[2] = o[2];
+    out[3] = __builtin_amdgcn_rsqf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+}
+
+// 8 consecutive taps of one image, exact windowed sinc.  jt (LDS) holds
+// (sigma_j cos(2 pi j/Tw), sigma_j sin(2 pi j/Tw)) of the block's offsets; xb is
+// j - f for the block's first offset, which is an even distance from J0.
+template <bool MASK>
+__device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float xb, float Bh, float BC, float BS,
+                                            float halfTw) {
+    const float4* j4 = reinterpret_cast<const float4*>(jt);
+    const float4 q0 = j4[0], q1 = j4[1], q2 = j4[2], q3 = j4[3];
+    const float cs[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const float x = xb + (float)u;
+        const float r = __builtin_amdgcn_rcpf(x);
+        const float w = fmaf(cs[2 * u], BC, fmaf(cs[2 * u + 1], BS, (u & 1) ? -Bh : Bh));
+        float v = w * r;
+        if (MASK) v = (fabsf(x) < halfTw) ? v : 0.0f;
+        lds_add(ap + u, v);
+    }
+}
+
+// 8 consecutive taps, lookup table with linear interpolation (1/16 sample grid);
+// lut (LDS) holds (T[i], T[i+1] - T[i]).
+template <bool MASK>
+__device__ __forceinline__ void taps8_lut(float* ap, const float2* lut, float xb, float amp, float center,
+                                          float half_int, int lut_len) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const float x = xb + (float)u;
+        const float uu = fmaf(x, 16.0f, center);
+        int i = (int)uu;
+        const float al = uu - floorf(uu);
+        if (MASK) i = min(max(i, 0), lut_len - 1);
+        const float2 t = lut[i];
+        float v = amp * fmaf(al, t.y, t.x);
+        if (MASK) v = (fabsf(x) < half_int) ? v : 0.0f;
+        lds_add(ap + u, v);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const Item it = a.items[blockIdx.x];
+    const RoomDev R = a.rooms[it.room];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    double* offs = reinterpret_cast<double*>(smem_raw);
+    float* att = reinterpret_cast<float*>(offs + a.ntab_max);
+    float* acc = att + a.ntab_max;
+    int* qa = reinterpret_cast<int*>(acc + a.acc_len);
+    float* qf = reinterpret_cast<float*>(qa + QCAP);
+    float* qB = qf + QCAP;
+    float* qC = qB + QCAP;
+    float* qS = qC + QCAP;
+    int* qcount = reinterpret_cast<int*>(qS + QCAP);
+    float2* jt = reinterpret_cast<float2*>(qcount + 4);   // [nJp] tap constants (exact mode)
+    float2* lut = jt + a.nJp;                              // [lut_len + 1] (LUT mode)
+
+    const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
+    const float* ps = a.pos_src + 3 * (R.src_off + s);
+    const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    build_axis_tables(R, ps, pr, offs, att, tid, WG);
+    for (int e = tid; e < a.acc_len; e += WG) acc[e] = 0.0f;
+    if (MODE == PREC_LUT)
+        for (int e = tid; e < a.lut_len + 1; e += WG) lut[e] = (e < a.lut_len) ? a.lut[e] : make_float2(0.0f, 0.0f);
+    else
+        for (int e = tid; e < a.nJp; e += WG) jt[e] = a.jtab[e];
+    float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
+    if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
+    if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
+
+    const int J1 = a.J0 + a.nJ - 1;
+    const int lo_n0 = it.t_lo - J1;        // images with n0 in [lo_n0, hi_n0] touch the tile
+    const int hi_n0 = it.t_hi - 1 - a.J0;
+    const int acc_shift = J1 - it.t_lo;    // acc index of the image's first tap = n0 + acc_shift
+    const float sigma0 = ((a.J0 & 1) != 0) ? 0.5f : -0.5f;  // 0.5 * (-1)^(J0+1)
+
+    for (int r0 = it.i0; r0 < it.i1; r0 += QCAP) {
+        if (tid == 0) *qcount = 0;
+        __syncthreads();
+        // ---- phase A: stage (a) for up to QCAP images, compaction of the live ones
+        for (int sub = 0; sub < QCAP / WG; sub++) {
+            const int i = r0 + sub * WG + tid;
+            bool live = false;
+            int n0 = 0;
+            float f = 0.0f, amp = 0.0f;
+            if (i < it.i1) {
+                const ImageVal iv = image_eval(R, i, offs, att, a.spkr, a.mic, os, orr);
+                const double fl = floor(iv.tau);
+                n0 = (int)fl;
+                f = (float)(iv.tau - fl);
+                amp = iv.amp;
+                live = (n0 >= lo_n0) && (n0 <= hi_n0) && (amp != 0.0f);
+            }
+            const unsigned long long mask = __ballot(live);
+            if (mask != 0ull) {
+                const int cnt = __popcll(mask);
+                const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                int base = 0;
+                if (lane == 0) base = atomicAdd(qcount, cnt);
+                base = __shfl(base, 0);
+                if (live) {
+                    const int q = base + pre;
+                    qa[q] = n0 + acc_shift;
+                    if (MODE == PREC_LUT) {
+                        qf[q] = f;
+                        qB[q] = amp;
+                    } else {
+                        // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
+                        // 2^-30 samples changes no tap by more than ~1e-9 relative.
+                        const float fc = fmaxf(f, 0x1p-30f);
+                        const float B = amp * sinpif(fc) * 0.31830988618379067f;
+                        const float ph = fc * a.inv_tw2;
+                        qf[q] = fc;
+                        qB[q] = B * sigma0;
+                        qC[q] = B * 0.5f * cospif(ph);
+                        qS[q] = B * 0.5f * sinpif(ph);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- phase B: every live image scatters its taps into the LDS tile
+        const int nq = *qcount;
+        for (int b = wave; b * 64 < nq; b += NWAVE) {
+            const int e = b * 64 + lane;
+            if (e < nq) {
+                float* ap = acc + qa[e];
+                const float f = qf[e];
+                const float xb = (float)a.J0 - f;
+                const int last = a.nJp - 8;
+                if (MODE == PREC_LUT) {
+                    const float amp = qB[e];
+                    const float cen = (float)(a.lut_len / 2);
+                    taps8_lut<true>(ap, lut, xb, amp, cen, a.halfTw, a.lut_len);
+                    for (int jb = 8; jb < last; jb += 8)
+                        taps8_lut<false>(ap + jb, lut, xb + (float)jb, amp, cen, a.halfTw, a.lut_len);
+                    if (last > 0) taps8_lut<true>(ap + last, lut, xb + (float)last, amp, cen, a.halfTw, a.lut_len);
+                } else {
+                    const float Bh = qB[e], BC = qC[e], BS = qS[e];
+                    taps8_exact<true>(ap, jt, xb, Bh, BC, BS, a.halfTw);
+                    for (int jb = 8; jb < last; jb += 8)
+                        taps8_exact<false>(ap + jb, jt + jb, xb + (float)jb, Bh, BC, BS, a.halfTw);
+                    if (last > 0) taps8_exact<true>(ap + last, jt + last, xb + (float)last, Bh, BC, BS, a.halfTw);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (it.i0 >= it.i1) __syncthreads();
+
+    // ---- flush the tile [t_lo, t_hi)
+    const float* src = acc + (J1 - a.J0);
+    const int nt = it.t_hi - it.t_lo;
+    float* dst;
+    if (R.n_chunks == 1)
+        dst = a.out + R.out_off + (long long)it.rir * R.nSamples + it.t_lo;
+    else
+        dst = a.part + R.part_off + ((long long)it.rir * R.n_chunks + it.chunk) * R.nISM + it.t_lo;
+    for (int t = tid; t < nt; t += WG) dst[t] = src[t];
+}
+
+__device__ inline void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1,
+                                     unsigned out[4]) {
+#pragma unroll
+    for (int rr = 0; rr < 10; rr++) {
+        const unsigned lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const unsigned lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const unsigned n0 = hi1 ^ c1 ^ k0;
+        const unsigned n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+__device__ inline double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < NWAVE; w++) t += red[w];
+    return t;
+}
+
+// One workgroup per RIR: fixed-order sum of the partial rows (if any), the
+// envelope of src/gpuRIR_cuda.cu:415-443 and the tail of :445-461.
+__global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
+    __shared__ double red[NWAVE];
+    const int g = blockIdx.x;
+    const RoomDev R = a.rooms[a.rir_room[g]];
+    const int m = g - R.rir_off;
+    const int s = m / R.n_rcv, r = m - s * R.n_rcv;
+    const int tid = threadIdx.x;
+    const int nISM = R.nISM, nS = R.nSamples, nDiff = nS - nISM;
+    float* o = a.out + R.out_off + (long long)m * nS;
+
+    // direct-path delay: image with k = 0 on every axis (src/gpuRIR_cuda.cu:341,368-369)
+    const float* ps = a.pos_src + 3 * (R.src_off + s);
+    const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    double v[3];
+    for (int d = 0; d < 3; d++) v[d] = (0.0 + (double)ps[d]) - (double)pr[d];
+    double d2 = v[0] * v[0];
+    d2 = d2 + v[1] * v[1];
+    d2 = d2 + v[2] * v[2];
+    const double tau_dp = sqrt(d2) * R.fs_over_c;
+    const int w_start = (int)ceil(fmax((double)nISM - (double)a.w_sz, tau_dp));
+
+    double pw = 0.0;
+    if (R.n_chunks > 1) {
+        const float* pp = a.part + R.part_off + (long long)m * R.n_chunks * nISM;
+        for (int t = tid; t < nISM; t += WG) {
+            float acc = 0.0f;
+            for (int c = 0; c < R.n_chunks; c++) acc += pp[(long long)c * nISM + t];
+            o[t] = acc;
+            if (t >= w_start) pw += (double)acc * (double)acc;
+        }
+    } else {
+        for (int t = w_start + tid; t < nISM; t += WG) {
+            const double x = o[t];
+            pw += x * x;
+        }
+    }
+    if (nDiff <= 0) return;
+    const double P = block_sum(pw, red) / (double)(nISM - w_start);
+
+    // Sabine T60 (src/gpuRIR_cuda.cu:123-130) and the envelope parameters
+    const double* L = R.L;
+    double b[6];
+    for (int k = 0; k < 6; k++) b[k] = R.beta[k];
+    const double Sa = ((1.0 - b[0] * b[0]) + (1.0 - b[1] * b[1])) * L[1] * L[2] +
+                      ((1.0 - b[2] * b[2]) + (1.0 - b[3] * b[3])) * L[0] * L[2] +
+                      ((1.0 - b[4] * b[4]) + (1.0 - b[5] * b[5])) * L[0] * L[1];
+    const double T60 = 0.161 * (L[0] * L[1] * L[2]) / Sa;
+    const double alpha = -13.8155 / (T60 * a.fs);
+    const double w_center = (double)w_start + (double)(nISM - w_start) / 2.0;
+    const double A = P / exp(alpha * (w_center - tau_dp));
+    const float gain = (float)((double)R.level * sqrt(A));
+    const float ha = (float)(0.5 * alpha);
+    const float base = (float)(0.5 * alpha * ((double)nISM - tau_dp));
+
+    const unsigned k0 = (unsigned)a.seed, k1 = (unsigned)(a.seed >> 32);
+    float* tail = o + nISM;
+    const int nblk = (nDiff + 3) >> 2;
+    for (int blk = tid; blk < nblk; blk += WG) {
+        unsigned w[4];
+        philox4x32_10((unsigned)blk, (unsigned)m, R.room_id, a.call_offset, k0, k1, w);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int sd = 4 * blk + u;
+            if (sd < nDiff) {
+                float z;
+                if (a.noise == 0) {
+                    // u = (x+1)/2^32, 1-u = (2^32-1-x)/2^32 (exact integers), log(u/(1-u+1e-6))
+                    const float num = (float)w[u] + 1.0f;
+                    const float den = (float)(0xFFFFFFFFu - w[u]) + 4294.967296f;
+                    z = 0.551329f * logf(num / den);
+                } else {
+                    const int pr2 = u >> 1;
+                    const float lnua = log1pf(-(float)(0xFFFFFFFFu - w[2 * pr2]) * 0x1p-32f);
+                    const float ub = (float)w[2 * pr2 + 1] * 0x1p-32f;
+                    const float rad = sqrtf(-2.0f * lnua);
+                    z = (u & 1) ? rad * sinpif(2.0f * ub) : rad * cospif(2.0f * ub);
+                }
+                const float env = expf(fmaf((float)sd, ha, base));
+                tail[sd] = gain * env * z;
+            }
+        }
+    }
+}
+
+// Stage (a) listing: n0, fraction and amplitude of every (RIR, image) of one room.
+__global__ __launch_bounds__(WG) void image_list_kernel(const RoomDev* rooms, const float* pos_src,
+                                                        const float* pos_rcv, const float* orv_src,
+                                                        const float* orv_rcv, int spkr, int mic, int ntab_max,
+                                                        int* n0_out, float* f_out, float* amp_out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const RoomDev R = rooms[0];
+    const int m = blockIdx.y;
+    const int s = m / R.n_rcv, r = m - s * R.n_rcv;
+    double* offs = reinterpret_cast<double*>(smem_raw);
+    float* att = reinterpret_cast<float*>(offs + ntab_max);
+    const float* ps = pos_src + 3 * s;
+    const float* pr = pos_rcv + 3 * r;
+    build_axis_tables(R, ps, pr, offs, att, threadIdx.x, WG);
+    float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
+    if (spkr != 0) load_orient(orv_src + 3 * s, os);
+    if (mic != 0) load_orient(orv_rcv + 3 * r, orr);
+    __syncthreads();
+    const long long N = (long long)R.nb[0] * R.nb[1] * R.nb[2];
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i < N) {
+        const ImageVal iv = image_eval(R, i, offs, att, spkr, mic, os, orr);
+        const double fl = floor(iv.tau);
+        n0_out[(long long)m * N + i] = (int)fl;
+        f_out[(long long)m * N + i] = (float)(iv.tau - fl);
+        amp_out[(long long)m * N + i] = iv.amp;
+    }
+}
+
+// Algorithmic work of a plan: the number of (RIR, image, t) with t in the item's
+// tile and |t - tau| < halfTw, summed over all items (stage (a) only).
+__global__ __launch_bounds__(WG) void count_taps_kernel(AccArgs a, unsigned long long* total) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ unsigned long long red[NWAVE];
+    const Item it = a.items[blockIdx.x];
+    const RoomDev R = a.rooms[it.room];
+    double* offs = reinterpret_cast<double*>(smem_raw);
+    float* att = reinterpret_cast<float*>(offs + a.ntab_max);
+    const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
+    build_axis_tables(R, a.pos_src + 3 * (R.src_off + s), a.pos_rcv + 3 * (R.rcv_off + r), offs, att, threadIdx.x,
+                      WG);
+    __syncthreads();
+    const float zero4[4] = {0, 0, 0, 0};
+    const double h = (double)a.halfTw;
+    unsigned long long cnt = 0;
+    for (int i = it.i0 + (int)threadIdx.x; i < it.i1; i += WG) {
+        const ImageVal iv = image_eval(R, i, offs, att, 0, 0, zero4, zero4);
+        long long t0 = (long long)floor(iv.tau - h) + 1;
+        long long t1 = (long long)ceil(iv.tau + h) - 1;
+        t0 = t0 > it.t_lo ? t0 : it.t_lo;
+        t1 = t1 < it.t_hi - 1 ? t1 : it.t_hi - 1;
+        if (t1 >= t0) cnt += (unsigned long long)(t1 - t0 + 1);
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < NWAVE; w++) t += red[w];
+        atomicAdd(total, t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct TapTables {
+    int J0 = 0, nJ = 0, nJp = 0;
+    float halfTw = 0.0f;
+    std::vector<float2> jtab;
+    std::vector<float2> lut;
+};
+
+// Tap offsets j = t - floor(tau) that can lie inside the window, and their
+// per-offset constants.  Exact mode: |j - f| < Tw/2 for some f in [0,1).  LUT
+// mode: |j - f| < Tw/2 (integer division), src/gpuRIR_cuda.cu:637.
+TapTables make_tap_tables(int precision, int Tw) {
+    TapTables T;
+    if (precision == PREC_LUT) {
+        const int half = Tw / 2;
+        T.J0 = -half + 1;
+        T.nJ = 2 * half;
+        T.halfTw = (float)half;
+        int len = Tw * 16;
+        len += (len % 2) ? 0 : 1;
+        const int center = len / 2;
+        std::vector<double> tab(len + 1, 0.0);
+        for (int i = 0; i <= center; i++) {
+            const double x = (double)i / 16.0;
+            const double sn = (i == 0) ? 1.0 : std::sin(M_PI * x) / (M_PI * x);
+            const double h = 0.5 * (1.0 + std::cos(2.0 * M_PI * x / Tw));
+            tab[center + i] = tab[center - i] = h * sn;
+        }
+        T.lut.resize(len);
+        for (int i = 0; i < len; i++) T.lut[i] = make_float2((float)tab[i], (float)(tab[i + 1] - tab[i]));
+        T.nJp = (T.nJ + 7) / 8 * 8;
+        T.jtab.assign(T.nJp, make_float2(0.0f, 0.0f));
+    } else {
+        const double h = Tw / 2.0;
+        T.halfTw = (float)h;
+        T.J0 = (int)std::floor(-h) + 1;
+        const int J1 = (int)std::ceil(h + 1.0) - 1;
+        T.nJ = J1 - T.J0 + 1;
+        T.nJp = (T.nJ + 7) / 8 * 8;
+        T.jtab.resize(T.nJp);
+        for (int u = 0; u < T.nJp; u++) {
+            const int j = T.J0 + u;
+            const double sg = (j & 1) ? 1.0 : -1.0;  // (-1)^(j+1)
+            const double th = 2.0 * M_PI * (double)j / (double)Tw;
+            T.jtab[u] = make_float2((float)(sg * std::cos(th)), (float)(sg * std::sin(th)));
+        }
+    }
+    return T;
+}
+
+struct Plan {
+    std::vector<RoomDev> rooms;
+    std::vector<Item> items;
+    std::vector<int> rir_room;
+    long long part_elems = 0;
+    int ntab_max = 0;
+    int tt_max = 0;
+};
+
+constexpr int TILE_DEFAULT = 8192;
+constexpr int MIN_CHUNK = 512;
+constexpr int TARGET_ITEMS = 2048;
+
+Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt) {
+    Plan P;
+    const int tile_cap = opt.tile_samples > 0 ? opt.tile_samples : TILE_DEFAULT;
+    double total = 0.0;
+    long long total_rirs = 0;
+    for (const RoomHost& h : rh) {
+        const long long N = (long long)h.nb[0] * h.nb[1] * h.nb[2];
+        const int nISM = n_samples_rule(h.Tdiff, opt.fs);
+        const int tiles = nISM > 0 ? (nISM + tile_cap - 1) / tile_cap : 0;
+        total += (double)h.n_src * h.n_rcv * (double)N * tiles;
+        total_rirs += (long long)h.n_src * h.n_rcv;
+    }
+    long long per_item = (long long)std::ceil(total / (double)TARGET_ITEMS);
+    if (opt.chunk_images > 0) per_item = opt.chunk_images;
+    per_item = std::max<long long>(per_item, MIN_CHUNK);
+    long long rir_off = 0;
+    for (size_t k = 0; k < rh.size(); k++) {
+        const RoomHost& h = rh[k];
+        RoomDev R;
+        std::memset(&R, 0, sizeof(R));
+        for (int d = 0; d < 3; d++) {
+            R.L[d] = h.L[d];
+            R.nb[d] = h.nb[d];
+        }
+        for (int d = 0; d < 6; d++) R.beta[d] = h.beta[d];
+        R.fs_over_c = (double)opt.fs / (double)opt.c;
+        R.inv_nyz = 1.0 / ((double)h.nb[1] * (double)h.nb[2]);
+        R.inv_nz = 1.0 / (double)h.nb[2];
+        R.n_src = h.n_src;
+        R.n_rcv = h.n_rcv;
+        R.src_off = h.src_off;
+        R.rcv_off = h.rcv_off;
+        R.nISM = n_samples_rule(h.Tdiff, opt.fs);
+        R.nSamples = n_samples_rule(h.Tmax, opt.fs);
+        R.rir_off = (int)rir_off;
+        R.out_off = h.out_off;
+        R.room_id = h.room_id;
+        R.level = h.level;
+        const int N = h.nb[0] * h.nb[1] * h.nb[2];
+        const int M = h.n_src * h.n_rcv;
+        const int TT = std::min(R.nISM, tile_cap);
+        const int tiles = R.nISM > 0 ? (R.nISM + TT - 1) / TT : 0;
+        int C = (int)std::min<long long>((N + per_item - 1) / per_item, (N + MIN_CHUNK - 1) / MIN_CHUNK);
+        C = std::max(C, 1);
+        R.n_chunks = C;
+        if (C > 1) {
+            R.part_off = P.part_elems;
+            P.part_elems += (long long)M * C * R.nISM;
+        }
+        P.ntab_max = std::max(P.ntab_max, h.nb[0] + h.nb[1] + h.nb[2]);
+        P.tt_max = std::max(P.tt_max, TT);
+        const int csz = (N + C - 1) / C;
+        for (int m = 0; m < M; m++) {
+            P.rir_room.push_back((int)k);
+            for (int c = 0; c < C; c++)
+                for (int tl = 0; tl < tiles; tl++) {
+                    Item it;
+                    it.room = (int)k;
+                    it.rir = m;
+                    it.i0 = std::min(N, c * csz);
+                    it.i1 = std::min(N, (c + 1) * csz);
+                    it.t_lo = tl * TT;
+                    it.t_hi = std::min(R.nISM, (tl + 1) * TT);
+                    it.chunk = c;
+                    it.pad = 0;
+                    P.items.push_back(it);
+                }
+        }
+        rir_off += M;
+        P.rooms.push_back(R);
+    }
+    // longest items first, so that the tail of the launch is short
+    if (rh.size() > 1)
+        std::stable_sort(P.items.begin(), P.items.end(),
+                         [](const Item& x, const Item& y) { return (x.i1 - x.i0) > (y.i1 - y.i0); });
+    (void)total_rirs;
+    return P;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+// Host description of one simulation launch (two kernels).  The plan blob
+// [rooms | items | rir_room | jtab | lut] is uploaded to device memory once;
+// sim_enqueue() only launches kernels, so it can be captured in a hipGraph.
+struct SimLaunch {
+    Plan P;
+    TapTables T;
+    int prec = PREC_FP32;
+    int Tw = 0;
+    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_jt = 0, off_lut = 0, plan_bytes = 0;
+    std::vector<unsigned char> blob;
+    size_t lds = 0;
+    int acc_len = 0, ntab = 0;
+    SimOptions opt;
+};
+
+int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLaunch& S) {
+    GPURIR_CHECK(opt.precision == PREC_FP32 || opt.precision == PREC_LUT, "unknown precision mode");
+    GPURIR_CHECK(opt.fs > 0.0f && opt.c > 0.0f, "fs and c must be positive");
+    GPURIR_CHECK(!rooms.empty(), "no rooms");
+    for (const RoomHost& h : rooms) {
+        GPURIR_CHECK(h.nb[0] > 0 && h.nb[1] > 0 && h.nb[2] > 0, "nb_img must be positive in every dimension");
+        GPURIR_CHECK((long long)h.nb[0] * h.nb[1] * h.nb[2] < (1LL << 31), "too many images per RIR");
+        GPURIR_CHECK(h.nb[0] + h.nb[1] + h.nb[2] <= 4096, "too many images along the room axes");
+        GPURIR_CHECK(h.n_src > 0 && h.n_rcv > 0, "each room needs at least one source and one receiver");
+        GPURIR_CHECK(h.Tdiff <= h.Tmax, "Tmax must be equal or greater than Tdiff");
+        GPURIR_CHECK(h.Tdiff >= 0.0f, "Tdiff must be non-negative");
+    }
+    S.opt = opt;
+    S.Tw = tw_rule(opt.fs);
+    GPURIR_CHECK(S.Tw >= 2, "sampling frequency too low for the 8 ms window");
+    S.prec = opt.precision;
+    S.T = make_tap_tables(S.prec, S.Tw);
+    S.P = make_plan(rooms, opt);
+    const Plan& P = S.P;
+    const TapTables& T = S.T;
+    S.off_rooms = 0;
+    S.off_items = align_up(S.off_rooms + sizeof(RoomDev) * P.rooms.size(), 256);
+    S.off_rr = align_up(S.off_items + sizeof(Item) * P.items.size(), 256);
+    S.off_jt = align_up(S.off_rr + sizeof(int) * P.rir_room.size(), 256);
+    S.off_lut = align_up(S.off_jt + sizeof(float2) * T.jtab.size(), 256);
+    S.plan_bytes = align_up(S.off_lut + sizeof(float2) * (T.lut.size() + 1), 256);
+    S.blob.assign(S.plan_bytes, 0);
+    std::memcpy(S.blob.data() + S.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
+    if (!P.items.empty()) std::memcpy(S.blob.data() + S.off_items, P.items.data(), sizeof(Item) * P.items.size());
+    std::memcpy(S.blob.data() + S.off_rr, P.rir_room.data(), sizeof(int) * P.rir_room.size());
+    std::memcpy(S.blob.data() + S.off_jt, T.jtab.data(), sizeof(float2) * T.jtab.size());
+    if (!T.lut.empty()) std::memcpy(S.blob.data() + S.off_lut, T.lut.data(), sizeof(float2) * T.lut.size());
+    S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
+    S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.nJp - 1, 4);
+    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(float) * S.acc_len + (size_t)QCAP * 5 * 4 + 16 +
+            sizeof(float2) * T.nJp;
+    if (S.prec == PREC_LUT) S.lds += sizeof(float2) * (T.lut.size() + 1);
+    GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
+    return 0;
+}
+
+size_t sim_plan_bytes(const SimLaunch& S) { return S.plan_bytes; }
+size_t sim_partial_bytes(const SimLaunch& S) { return sizeof(float) * (size_t)S.P.part_elems; }
+long long sim_n_rirs(const SimLaunch& S) { return (long long)S.P.rir_room.size(); }
+
+static AccArgs acc_args(const SimLaunch& S, const void* d_plan, float* d_part, const float* d_pos_src,
+                        const float* d_pos_rcv, const float* d_orv_src, const float* d_orv_rcv, float* d_out) {
+    const unsigned char* dp = static_cast<const unsigned char*>(d_plan);
+    AccArgs a;
+    a.rooms = reinterpret_cast<const RoomDev*>(dp + S.off_rooms);
+    a.items = reinterpret_cast<const Item*>(dp + S.off_items);
+    a.pos_src = d_pos_src;
+    a.pos_rcv = d_pos_rcv;
+    a.orv_src = d_orv_src;
+    a.orv_rcv = d_orv_rcv;
+    a.jtab = reinterpret_cast<const float2*>(dp + S.off_jt);
+    a.lut = reinterpret_cast<const float2*>(dp + S.off_lut);
+    a.out = d_out;
+    a.part = d_part;
+    a.spkr = S.opt.spkr;
+    a.mic = S.opt.mic;
+    a.J0 = S.T.J0;
+    a.nJ = S.T.nJ;
+    a.nJp = S.T.nJp;
+    a.halfTw = S.T.halfTw;
+    a.inv_tw2 = 2.0f / (float)S.Tw;
+    a.lut_len = (int)S.T.lut.size();
+    a.ntab_max = S.ntab;
+    a.acc_len = S.acc_len;
+    return a;
+}
+
+int sim_count_taps(const SimLaunch& S, const void* d_plan, const float* d_pos_src, const float* d_pos_rcv,
+                   unsigned long long* d_counter, hipStream_t stream) {
+    if (S.P.items.empty()) return 0;
+    AccArgs a = acc_args(S, d_plan, nullptr, d_pos_src, d_pos_rcv, nullptr, nullptr, nullptr);
+    const size_t lds = (sizeof(double) + sizeof(float)) * S.ntab;
+    hipLaunchKernelGGL(count_taps_kernel, dim3((unsigned)S.P.items.size()), dim3(WG), lds, stream, a, d_counter);
+    GPURIR_HIP(hipGetLastError());
+    return 0;
+}
+
+int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* d_part, const float* d_pos_src,
+                      const float* d_pos_rcv, const float* d_orv_src, const float* d_orv_rcv, float* d_out,
+                      hipStream_t stream) {
+    const unsigned char* dp = static_cast<const unsigned char*>(d_plan);
+    const RoomDev* d_rooms = reinterpret_cast<const RoomDev*>(dp + S.off_rooms);
+    if ((stage & 1) && !S.P.items.empty()) {
+        AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
+        const void* fn = (S.prec == PREC_LUT) ? reinterpret_cast<const void*>(&ism_accumulate_kernel<PREC_LUT>)
+                                              : reinterpret_cast<const void*>(&ism_accumulate_kernel<PREC_FP32>);
+        if (S.lds > 64 * 1024)
+            GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds));
+        const dim3 grid((unsigned)S.P.items.size());
+        if (S.prec == PREC_LUT)
+            hipLaunchKernelGGL(ism_accumulate_kernel<PREC_LUT>, grid, dim3(WG), S.lds, stream, a);
+        else
+            hipLaunchKernelGGL(ism_accumulate_kernel<PREC_FP32>, grid, dim3(WG), S.lds, stream, a);
+        GPURIR_HIP(hipGetLastError());
+    }
+    if (!(stage & 2)) return 0;
+    FinArgs f;
+    f.rooms = d_rooms;
+    f.rir_room = reinterpret_cast<const int*>(dp + S.off_rr);
+    f.pos_src = d_pos_src;
+    f.pos_rcv = d_pos_rcv;
+    f.out = d_out;
+    f.part = d_part;
+    f.w_sz = 10e-3f * S.opt.fs;
+    f.fs = (double)S.opt.fs;
+    f.seed = S.opt.seed;
+    f.call_offset = S.opt.call_offset;
+    f.noise = S.opt.noise;
+    hipLaunchKernelGGL(ism_finalize_kernel, dim3((unsigned)S.P.rir_room.size()), dim3(WG), 0, stream, f);
+    GPURIR_HIP(hipGetLastError());
+    return 0;
+}
+
+int sim_enqueue(const SimLaunch& S, const void* d_plan, float* d_part, const float* d_pos_src, const float* d_pos_rcv,
+                const float* d_orv_src, const float* d_orv_rcv, float* d_out, hipStream_t stream) {
+    return sim_enqueue_stage(S, 3, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
+}
+
+SimLaunch* sim_new() { return new SimLaunch(); }
+void sim_delete(SimLaunch* s) { delete s; }
+const void* sim_blob(const SimLaunch& S) { return S.blob.data(); }
+void sim_set_stream_opts(SimLaunch& S, unsigned long long seed, unsigned call_offset) {
+    S.opt.seed = seed;
+    S.opt.call_offset = call_offset;
+}
+
+int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
+                      const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, float* d_out,
+                      hipStream_t stream) {
+    SimLaunch S;
+    if (int rc = sim_prepare(rooms, opt, S)) return rc;
+    void* dplan = nullptr;
+    if (int rc = workspace(WS_PLAN, S.plan_bytes, &dplan)) return rc;
+    if (int rc = upload_async(dplan, S.blob.data(), S.plan_bytes, stream)) return rc;
+    float* dpart = nullptr;
+    if (S.P.part_elems > 0) {
+        void* pv = nullptr;
+        if (int rc = workspace(WS_PARTIAL, sim_partial_bytes(S), &pv)) return rc;
+        dpart = static_cast<float*>(pv);
+    }
+    return sim_enqueue(S, dplan, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
+}
+
+int launch_image_list(const RoomHost& room, const float* d_pos_src, const float* d_pos_rcv, const float* d_orv_src,
+                      const float* d_orv_rcv, const SimOptions& opt, int* d_n0, float* d_frac, float* d_amp,
+                      hipStream_t stream) {
+    GPURIR_CHECK(room.nb[0] > 0 && room.nb[1] > 0 && room.nb[2] > 0, "nb_img must be positive in every dimension");
+    GPURIR_CHECK(room.nb[0] + room.nb[1] + room.nb[2] <= 4096, "too many images along the room axes");
+    std::vector<RoomHost> rh(1, room);
+    Plan P = make_plan(rh, opt);
+    void* dplan = nullptr;
+    if (int rc = workspace(WS_PLAN, sizeof(RoomDev), &dplan)) return rc;
+    if (int rc = upload_async(dplan, P.rooms.data(), sizeof(RoomDev), stream)) return rc;
+    const long long N = (long long)room.nb[0] * room.nb[1] * room.nb[2];
+    const int M = room.n_src * room.n_rcv;
+    GPURIR_CHECK(M <= 65535, "image listing supports at most 65535 (source, receiver) pairs per call");
+    const int ntab = (P.ntab_max + 3) / 4 * 4;
+    const size_t lds = (sizeof(double) + sizeof(float)) * ntab;
+    hipLaunchKernelGGL(image_list_kernel, dim3((unsigned)((N + WG - 1) / WG), (unsigned)M), dim3(WG), lds, stream,
+                       static_cast<const RoomDev*>(dplan), d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, opt.spkr,
+                       opt.mic, ntab, d_n0, d_frac, d_amp);
+    GPURIR_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace gpurir

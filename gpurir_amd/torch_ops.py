@@ -1,0 +1,228 @@
+"""Device-resident API over the C ABI: torch tensors in and out, work on the
+current HIP stream, nothing copied to the host.
+
+  simulate_rir       one room, explicit precision / noise seed / tail level
+  image_list         stage (a) per (RIR, image): n0, fraction, amplitude
+  simulate_batch     many rooms of uneven sizes in one launch
+  BatchPlan          the same, planned once and replayed as a hipGraph
+  trajectory         simulateTrajectory on device tensors (float64 output)
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PATTERNS = {"omni": 0, "homni": 1, "card": 2, "hypcard": 3, "subcard": 4, "bidir": 5}
+PRECISIONS = {"fp32": _lib.PREC_FP32, "lut": _lib.PREC_LUT}
+NOISES = {"logistic": _lib.NOISE_LOGISTIC, "gaussian": _lib.NOISE_GAUSSIAN}
+
+
+def _pat(p):
+    return PATTERNS[p] if isinstance(p, str) else int(p)
+
+
+def _dev_f32(x, dev):
+    if isinstance(x, torch.Tensor):
+        return x.detach().to(device=dev, dtype=torch.float32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float32), device=dev)
+
+
+def _stream(dev):
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def options(precision="fp32", noise="logistic", seed=1234, call_offset=0, room_index=0, level=1.0,
+            chunk_images=0, tile_samples=0):
+    o = _lib.Options()
+    o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    o.noise = NOISES[noise] if isinstance(noise, str) else int(noise)
+    o.seed = int(seed)
+    o.call_offset = int(call_offset)
+    o.room_index = int(room_index)
+    o.level = float(level)
+    o.chunk_images = int(chunk_images)
+    o.tile_samples = int(tile_samples)
+    return o
+
+
+def n_samples(T, fs):
+    return _lib.load().gpurir_n_samples(float(T), float(fs))
+
+
+def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, spkr_pattern="omni",
+                 mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0, precision="fp32", noise="logistic",
+                 seed=1234, call_offset=0, room_index=0, level=1.0, chunk_images=0, tile_samples=0, out=None,
+                 device=None):
+    """simulateRIR on device tensors; returns float32 [n_src, n_rcv, nSamples] on the GPU."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ps = _dev_f32(pos_src, dev).reshape(-1, 3)
+    pr = _dev_f32(pos_rcv, dev).reshape(-1, 3)
+    os_ = torch.zeros_like(ps) if orV_src is None else _dev_f32(orV_src, dev).reshape(-1, 3)
+    or_ = torch.zeros_like(pr) if orV_rcv is None else _dev_f32(orV_rcv, dev).reshape(-1, 3)
+    if Tdiff is None:
+        Tdiff = Tmax
+    room = np.ascontiguousarray(np.asarray(room_sz, dtype=np.float32).reshape(3))
+    b = np.ascontiguousarray(np.asarray(beta, dtype=np.float32).reshape(6))
+    nb = np.ascontiguousarray(np.asarray(nb_img, dtype=np.int32).reshape(3))
+    ns = n_samples(Tmax, fs)
+    shape = (ps.shape[0], pr.shape[0], ns)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=dev)
+    assert out.shape == shape and out.dtype == torch.float32 and out.is_contiguous()
+    o = options(precision, noise, seed, call_offset, room_index, level, chunk_images, tile_samples)
+    _lib.check(_lib.load().gpurir_simulate_rir_device(
+        _lib.fptr(room), _lib.fptr(b), ps.data_ptr(), ps.shape[0], pr.data_ptr(), pr.shape[0], os_.data_ptr(),
+        or_.data_ptr(), _pat(spkr_pattern), _pat(mic_pattern), _lib.iptr(nb), float(Tdiff), float(Tmax), float(fs),
+        float(c), ctypes.byref(o), out.data_ptr(), _stream(dev)))
+    return out
+
+
+def image_list(room_sz, beta, pos_src, pos_rcv, nb_img, fs, spkr_pattern="omni", mic_pattern="omni", orV_src=None,
+               orV_rcv=None, c=343.0, device=None):
+    """Stage (a): returns (n0 int32, frac float32, amp float32), each [n_src*n_rcv, N] on the GPU."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ps = _dev_f32(pos_src, dev).reshape(-1, 3)
+    pr = _dev_f32(pos_rcv, dev).reshape(-1, 3)
+    os_ = torch.zeros_like(ps) if orV_src is None else _dev_f32(orV_src, dev).reshape(-1, 3)
+    or_ = torch.zeros_like(pr) if orV_rcv is None else _dev_f32(orV_rcv, dev).reshape(-1, 3)
+    room = np.ascontiguousarray(np.asarray(room_sz, dtype=np.float32).reshape(3))
+    b = np.ascontiguousarray(np.asarray(beta, dtype=np.float32).reshape(6))
+    nb = np.ascontiguousarray(np.asarray(nb_img, dtype=np.int32).reshape(3))
+    M = ps.shape[0] * pr.shape[0]
+    N = int(np.prod(nb.astype(np.int64)))
+    n0 = torch.empty((M, N), dtype=torch.int32, device=dev)
+    fr = torch.empty((M, N), dtype=torch.float32, device=dev)
+    amp = torch.empty((M, N), dtype=torch.float32, device=dev)
+    _lib.check(_lib.load().gpurir_image_list_device(
+        _lib.fptr(room), _lib.fptr(b), ps.data_ptr(), ps.shape[0], pr.data_ptr(), pr.shape[0], os_.data_ptr(),
+        or_.data_ptr(), _pat(spkr_pattern), _pat(mic_pattern), _lib.iptr(nb), float(fs), float(c), n0.data_ptr(),
+        fr.data_ptr(), amp.data_ptr(), _stream(dev)))
+    return n0, fr, amp
+
+
+class _BatchArgs:
+    """Host arrays of a batch of rooms plus its device position buffers."""
+
+    def __init__(self, rooms, fs, dev):
+        R = len(rooms)
+        self.R = R
+        self.room_sz = np.ascontiguousarray([r["room_sz"] for r in rooms], dtype=np.float32).reshape(R, 3)
+        self.beta = np.ascontiguousarray([r["beta"] for r in rooms], dtype=np.float32).reshape(R, 6)
+        self.nb = np.ascontiguousarray([r["nb_img"] for r in rooms], dtype=np.int32).reshape(R, 3)
+        ps = [np.asarray(r["pos_src"], dtype=np.float32).reshape(-1, 3) for r in rooms]
+        pr = [np.asarray(r["pos_rcv"], dtype=np.float32).reshape(-1, 3) for r in rooms]
+        self.n_src = np.ascontiguousarray([p.shape[0] for p in ps], dtype=np.int32)
+        self.n_rcv = np.ascontiguousarray([p.shape[0] for p in pr], dtype=np.int32)
+        self.Tmax = np.ascontiguousarray([r["Tmax"] for r in rooms], dtype=np.float32)
+        self.Tdiff = np.ascontiguousarray([r.get("Tdiff", r["Tmax"]) if r.get("Tdiff") is not None else r["Tmax"]
+                                           for r in rooms], dtype=np.float32)
+        self.level = np.ascontiguousarray([r.get("level", 1.0) for r in rooms], dtype=np.float32)
+        os_ = [np.zeros_like(p) if r.get("orV_src") is None else np.asarray(r["orV_src"], np.float32).reshape(-1, 3)
+               for p, r in zip(ps, rooms)]
+        or_ = [np.zeros_like(p) if r.get("orV_rcv") is None else np.asarray(r["orV_rcv"], np.float32).reshape(-1, 3)
+               for p, r in zip(pr, rooms)]
+        self.d_ps = torch.as_tensor(np.concatenate(ps), device=dev)
+        self.d_pr = torch.as_tensor(np.concatenate(pr), device=dev)
+        self.d_os = torch.as_tensor(np.concatenate(os_), device=dev)
+        self.d_or = torch.as_tensor(np.concatenate(or_), device=dev)
+        lib = _lib.load()
+        self.ns = np.array([lib.gpurir_n_samples(float(t), float(fs)) for t in self.Tmax], dtype=np.int64)
+        sizes = self.n_src.astype(np.int64) * self.n_rcv.astype(np.int64) * self.ns
+        self.offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]), dtype=np.int64)
+        self.total = int(sizes.sum())
+
+    def split(self, out):
+        """Views [n_src, n_rcv, nSamples] of each room inside the flat output."""
+        res = []
+        for k in range(self.R):
+            o = int(self.offsets[k])
+            n = int(self.n_src[k]) * int(self.n_rcv[k]) * int(self.ns[k])
+            res.append(out[o:o + n].view(int(self.n_src[k]), int(self.n_rcv[k]), int(self.ns[k])))
+        return res
+
+    def call_args(self):
+        ll = ctypes.POINTER(ctypes.c_longlong)
+        return (self.R, _lib.fptr(self.room_sz), _lib.fptr(self.beta), _lib.iptr(self.nb), _lib.iptr(self.n_src),
+                _lib.iptr(self.n_rcv), _lib.fptr(self.Tdiff), _lib.fptr(self.Tmax), _lib.fptr(self.level),
+                self.d_ps.data_ptr(), self.d_pr.data_ptr(), self.d_os.data_ptr(), self.d_or.data_ptr())
+
+
+def simulate_batch(rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
+                   noise="logistic", seed=1234, call_offset=0, room_index=0, device=None):
+    """rooms: list of dicts with room_sz, beta, pos_src, pos_rcv, nb_img, Tmax [, Tdiff, orV_src, orV_rcv, level].
+
+    Returns (flat float32 output on the GPU, list of per-room [n_src, n_rcv, nSamples] views)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    B = _BatchArgs(rooms, fs, dev)
+    out = torch.empty(B.total, dtype=torch.float32, device=dev)
+    o = options(precision, noise, seed, call_offset, room_index)
+    _lib.check(_lib.load().gpurir_simulate_batch_device(
+        *B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),
+        out.data_ptr(), B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _stream(dev)))
+    return out, B.split(out)
+
+
+class BatchPlan:
+    """A batch of rooms planned once; run() replays the captured hipGraph."""
+
+    def __init__(self, rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
+                 noise="logistic", seed=1234, call_offset=0, room_index=0, device=None):
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.dev = dev
+        self.B = _BatchArgs(rooms, fs, dev)
+        self.out = torch.empty(self.B.total, dtype=torch.float32, device=dev)
+        self.rirs = self.B.split(self.out)
+        o = options(precision, noise, seed, call_offset, room_index)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().gpurir_batch_plan_create(
+            *self.B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),
+            self.out.data_ptr(), self.B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def n_rirs(self):
+        return int(_lib.load().gpurir_batch_plan_n_rirs(self._h))
+
+    def run(self):
+        _lib.check(_lib.load().gpurir_batch_plan_run(self._h, _stream(self.dev)))
+        return self.out
+
+    def count_taps(self):
+        """In-window taps (RIR, image, t) of the plan: its algorithmic work."""
+        n = ctypes.c_longlong()
+        _lib.check(_lib.load().gpurir_batch_plan_count_taps(self._h, ctypes.byref(n), _stream(self.dev)))
+        return int(n.value)
+
+    def profile(self):
+        """(accumulate ms, finalize ms) of one eager run, from HIP events on the current stream."""
+        a, f = ctypes.c_float(), ctypes.c_float()
+        _lib.check(_lib.load().gpurir_batch_plan_profile(self._h, _stream(self.dev), ctypes.byref(a), ctypes.byref(f)))
+        return float(a.value), float(f.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().gpurir_batch_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def trajectory(source_signal, rirs, timestamps=None, fs=None, device=None):
+    """simulateTrajectory on device tensors; returns float64 [n + len - 1, n_rcv] on the GPU."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    sig = _dev_f32(source_signal, dev).reshape(-1)
+    r = _dev_f32(rirs, dev)
+    n = sig.shape[0]
+    P, R, L = r.shape
+    if timestamps is None:
+        fs = n / P
+        timestamps = np.arange(P)
+    w_ini = np.ascontiguousarray(np.append((np.asarray(timestamps) * fs).astype(int), n), dtype=np.int32)
+    out = torch.empty((n + L - 1, R), dtype=torch.float64, device=dev)
+    _lib.check(_lib.load().gpurir_trajectory_device(sig.data_ptr(), n, r.data_ptr(), P, R, L, _lib.iptr(w_ini),
+                                                    out.data_ptr(), _stream(dev)))
+    return out

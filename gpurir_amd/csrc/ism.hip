@@ -55,7 +55,9 @@ struct AccArgs {
     float* out;
     float* part;
     int spkr, mic;
-    int J0, nJ, nJp;      // first tap offset, number of offsets, padded to a multiple of 8
+    int J0, nJ;           // tap offsets j = t - floor(tau) that can be in the window: [J0, J0 + nJ)
+    int nb_dn, nb_up;     // 8-tap blocks below (j <= 0) and above (j >= 1) the image's peak
+    int nJp;              // 8 * (nb_dn + nb_up): evaluated offsets j in [1 - 8 nb_dn, 8 nb_up]
     float halfTw;         // support |t - tau| < halfTw
     float inv_tw2;        // 2/Tw
     int lut_len;
@@ -178,11 +180,17 @@ __device__ inline void load_orient(const float* o, float out[4]) {
     out[3] = __builtin_amdgcn_rsqf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
 }
 
-// 8 consecutive taps of one image, exact windowed sinc.  jt (LDS) holds
-// (sigma_j cos(2 pi j/Tw), sigma_j sin(2 pi j/Tw)) of the block's offsets; xb is
-// j - f for the block's first offset, which is an even distance from J0.
-template <bool MASK>
-__device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float xb, float Bh, float BC, float BS,
+// 8 consecutive taps of one image.  The blocks tile the offsets outward from the
+// image's peak: "down" blocks hold j = -7-8k .. -8k and "up" blocks j = 1+8k .. 8+8k,
+// so that the first tap of every block has an odd j.  x = j - f is formed by one
+// rounding from an anchor that is itself one rounding from the exact value
+// (-8k - f for a down block, 1 + 8k - f for an up block): the relative error of
+// every x stays near 2^-23 however small |x| is.
+//   exact: jt (LDS) holds (sigma_j cos(2 pi j/Tw), sigma_j sin(2 pi j/Tw)),
+//          sigma_j = (-1)^(j+1); Bh = B/2 (sigma = +1 on the odd first tap).
+//   LUT:   lut (LDS) holds (T[i], T[i+1] - T[i]) on the 1/16-sample grid.
+template <bool MASK, bool DOWN>
+__device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float xa, float Bh, float BC, float BS,
                                             float halfTw) {
     const float4* j4 = reinterpret_cast<const float4*>(jt);
     const float4 q0 = j4[0], q1 = j4[1], q2 = j4[2], q3 = j4[3];
@@ -190,7 +198,7 @@ __device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float x
                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-        const float x = xb + (float)u;
+        const float x = DOWN ? xa - (float)(7 - u) : xa + (float)u;
         const float r = __builtin_amdgcn_rcpf(x);
         const float w = fmaf(cs[2 * u], BC, fmaf(cs[2 * u + 1], BS, (u & 1) ? -Bh : Bh));
         float v = w * r;
@@ -199,14 +207,12 @@ __device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float x
     }
 }
 
-// 8 consecutive taps, lookup table with linear interpolation (1/16 sample grid);
-// lut (LDS) holds (T[i], T[i+1] - T[i]).
-template <bool MASK>
-__device__ __forceinline__ void taps8_lut(float* ap, const float2* lut, float xb, float amp, float center,
+template <bool MASK, bool DOWN>
+__device__ __forceinline__ void taps8_lut(float* ap, const float2* lut, float xa, float amp, float center,
                                           float half_int, int lut_len) {
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-        const float x = xb + (float)u;
+        const float x = DOWN ? xa - (float)(7 - u) : xa + (float)u;
         const float uu = fmaf(x, 16.0f, center);
         int i = (int)uu;
         const float al = uu - floorf(uu);
@@ -253,8 +259,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     const int J1 = a.J0 + a.nJ - 1;
     const int lo_n0 = it.t_lo - J1;        // images with n0 in [lo_n0, hi_n0] touch the tile
     const int hi_n0 = it.t_hi - 1 - a.J0;
-    const int acc_shift = J1 - it.t_lo;    // acc index of the image's first tap = n0 + acc_shift
-    const float sigma0 = ((a.J0 & 1) != 0) ? 0.5f : -0.5f;  // 0.5 * (-1)^(J0+1)
+    const int acc_shift = J1 - it.t_lo;    // acc index of tap j = n0 + acc_shift + (j - (1 - 8 nb_dn))
 
     for (int r0 = it.i0; r0 < it.i1; r0 += QCAP) {
         if (tid == 0) *qcount = 0;
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
                         const float B = amp * sinpif(fc) * 0.31830988618379067f;
                         const float ph = fc * a.inv_tw2;
                         qf[q] = fc;
-                        qB[q] = B * sigma0;
+                        qB[q] = 0.5f * B;
                         qC[q] = B * 0.5f * cospif(ph);
                         qS[q] = B * 0.5f * sinpif(ph);
                     }
@@ -309,21 +314,42 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
             if (e < nq) {
                 float* ap = acc + qa[e];
                 const float f = qf[e];
-                const float xb = (float)a.J0 - f;
-                const int last = a.nJp - 8;
+                const int nd = a.nb_dn, nu = a.nb_up;
+                float* apu = ap + 8 * nd;  // tap j = 1
                 if (MODE == PREC_LUT) {
                     const float amp = qB[e];
                     const float cen = (float)(a.lut_len / 2);
-                    taps8_lut<true>(ap, lut, xb, amp, cen, a.halfTw, a.lut_len);
-                    for (int jb = 8; jb < last; jb += 8)
-                        taps8_lut<false>(ap + jb, lut, xb + (float)jb, amp, cen, a.halfTw, a.lut_len);
-                    if (last > 0) taps8_lut<true>(ap + last, lut, xb + (float)last, amp, cen, a.halfTw, a.lut_len);
+                    for (int k = 0; k < nd; k++) {
+                        const float xa = (float)(-8 * k) - f;
+                        if (k == nd - 1)
+                            taps8_lut<true, true>(apu - 8 * (k + 1), lut, xa, amp, cen, a.halfTw, a.lut_len);
+                        else
+                            taps8_lut<false, true>(apu - 8 * (k + 1), lut, xa, amp, cen, a.halfTw, a.lut_len);
+                    }
+                    for (int k = 0; k < nu; k++) {
+                        const float xa = (float)(1 + 8 * k) - f;
+                        if (k == nu - 1)
+                            taps8_lut<true, false>(apu + 8 * k, lut, xa, amp, cen, a.halfTw, a.lut_len);
+                        else
+                            taps8_lut<false, false>(apu + 8 * k, lut, xa, amp, cen, a.halfTw, a.lut_len);
+                    }
                 } else {
                     const float Bh = qB[e], BC = qC[e], BS = qS[e];
-                    taps8_exact<true>(ap, jt, xb, Bh, BC, BS, a.halfTw);
-                    for (int jb = 8; jb < last; jb += 8)
-                        taps8_exact<false>(ap + jb, jt + jb, xb + (float)jb, Bh, BC, BS, a.halfTw);
-                    if (last > 0) taps8_exact<true>(ap + last, jt + last, xb + (float)last, Bh, BC, BS, a.halfTw);
+                    const float2* jtu = jt + 8 * nd;
+                    for (int k = 0; k < nd; k++) {
+                        const float xa = (float)(-8 * k) - f;
+                        if (k == nd - 1)
+                            taps8_exact<true, true>(apu - 8 * (k + 1), jtu - 8 * (k + 1), xa, Bh, BC, BS, a.halfTw);
+                        else
+                            taps8_exact<false, true>(apu - 8 * (k + 1), jtu - 8 * (k + 1), xa, Bh, BC, BS, a.halfTw);
+                    }
+                    for (int k = 0; k < nu; k++) {
+                        const float xa = (float)(1 + 8 * k) - f;
+                        if (k == nu - 1)
+                            taps8_exact<true, false>(apu + 8 * k, jtu + 8 * k, xa, Bh, BC, BS, a.halfTw);
+                        else
+                            taps8_exact<false, false>(apu + 8 * k, jtu + 8 * k, xa, Bh, BC, BS, a.halfTw);
+                    }
                 }
             }
         }
@@ -332,7 +358,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     if (it.i0 >= it.i1) __syncthreads();
 
     // ---- flush the tile [t_lo, t_hi)
-    const float* src = acc + (J1 - a.J0);
+    const float* src = acc + (J1 - (1 - 8 * a.nb_dn));
     const int nt = it.t_hi - it.t_lo;
     float* dst;
     if (R.n_chunks == 1)
@@ -527,9 +553,11 @@ __global__ __launch_bounds__(WG) void count_taps_kernel(AccArgs a, unsigned long
 // ---------------------------------------------------------------------------
 
 struct TapTables {
-    int J0 = 0, nJ = 0, nJp = 0;
+    int J0 = 0, nJ = 0;        // offsets j that can be in the window
+    int nb_dn = 0, nb_up = 0;  // evaluated blocks: j in [1 - 8 nb_dn, 8 nb_up]
+    int nJp = 0;               // 8 (nb_dn + nb_up)
     float halfTw = 0.0f;
-    std::vector<float2> jtab;
+    std::vector<float2> jtab;  // indexed by j - (1 - 8 nb_dn)
     std::vector<float2> lut;
 };
 
@@ -555,18 +583,22 @@ TapTables make_tap_tables(int precision, int Tw) {
         }
         T.lut.resize(len);
         for (int i = 0; i < len; i++) T.lut[i] = make_float2((float)tab[i], (float)(tab[i + 1] - tab[i]));
-        T.nJp = (T.nJ + 7) / 8 * 8;
-        T.jtab.assign(T.nJp, make_float2(0.0f, 0.0f));
     } else {
         const double h = Tw / 2.0;
         T.halfTw = (float)h;
         T.J0 = (int)std::floor(-h) + 1;
         const int J1 = (int)std::ceil(h + 1.0) - 1;
         T.nJ = J1 - T.J0 + 1;
-        T.nJp = (T.nJ + 7) / 8 * 8;
-        T.jtab.resize(T.nJp);
+    }
+    const int J1 = T.J0 + T.nJ - 1;
+    T.nb_dn = (1 - T.J0 + 7) / 8;
+    T.nb_up = (J1 + 7) / 8;
+    T.nJp = 8 * (T.nb_dn + T.nb_up);
+    T.jtab.assign(T.nJp, make_float2(0.0f, 0.0f));
+    if (precision != PREC_LUT) {
+        const int Jp0 = 1 - 8 * T.nb_dn;
         for (int u = 0; u < T.nJp; u++) {
-            const int j = T.J0 + u;
+            const int j = Jp0 + u;
             const double sg = (j & 1) ? 1.0 : -1.0;  // (-1)^(j+1)
             const double th = 2.0 * M_PI * (double)j / (double)Tw;
             T.jtab[u] = make_float2((float)(sg * std::cos(th)), (float)(sg * std::sin(th)));
@@ -719,7 +751,8 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     std::memcpy(S.blob.data() + S.off_jt, T.jtab.data(), sizeof(float2) * T.jtab.size());
     if (!T.lut.empty()) std::memcpy(S.blob.data() + S.off_lut, T.lut.data(), sizeof(float2) * T.lut.size());
     S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
-    S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.nJp - 1, 4);
+    // acc index = n0 - t_lo + J1 + (j - Jp0), n0 <= t_hi - 1 - J0, j - Jp0 < nJp
+    S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.nJp, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(float) * S.acc_len + (size_t)QCAP * 5 * 4 + 16 +
             sizeof(float2) * T.nJp;
     if (S.prec == PREC_LUT) S.lds += sizeof(float2) * (T.lut.size() + 1);
@@ -750,6 +783,8 @@ static AccArgs acc_args(const SimLaunch& S, const void* d_plan, float* d_part, c
     a.J0 = S.T.J0;
     a.nJ = S.T.nJ;
     a.nJp = S.T.nJp;
+    a.nb_dn = S.T.nb_dn;
+    a.nb_up = S.T.nb_up;
     a.halfTw = S.T.halfTw;
     a.inv_tw2 = 2.0f / (float)S.Tw;
     a.lut_len = (int)S.T.lut.size();

@@ -47,6 +47,12 @@ def lib():
                                        ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
                                        dp, dp]
         L.orc_simulate_rir.restype = ctypes.c_int
+        L.orc_simulate_rir_d.argtypes = [dp, dp, dp, ctypes.c_int, dp, ctypes.c_int, dp, dp,
+                                         ctypes.c_int, ctypes.c_int, ip, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                         dp, dp, fp, fp, fp, fp, fp, fp]
+        L.orc_simulate_rir_d.restype = ctypes.c_int
         L.orc_images.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.c_int, fp, fp, ctypes.c_int,
                                  ctypes.c_int, ip, ctypes.c_float, ctypes.c_float,
                                  ctypes.POINTER(ctypes.c_int64), dp, dp, dp]
@@ -120,6 +126,42 @@ def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, 
                            _ptr(or_, ctypes.c_float), sp, mp, _ptr(nb, ctypes.c_int), float(Tdiff), float(Tmax),
                            float(fs), float(c), int(mode), kind, int(seed), int(room_index), int(call_offset),
                            float(level), _ptr(out, ctypes.c_double), _ptr(env, ctypes.c_double))
+    if return_env:
+        return out, env
+    return out
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def simulate_rir_f64(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, spkr_pattern="omni",
+                     mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0, mode=EXACT_SCATTER, noise="logistic",
+                     seed=1234, room_index=0, call_offset=0, level=1.0, return_env=False):
+    """simulate_rir with double-precision geometry (room, betas, positions, orientations)."""
+    room = _f64(room_sz).reshape(3)
+    b = _f64(beta).reshape(6)
+    ps = _f64(pos_src).reshape(-1, 3)
+    pr = _f64(pos_rcv).reshape(-1, 3)
+    os_ = np.zeros_like(ps) if orV_src is None else _f64(orV_src).reshape(-1, 3)
+    or_ = np.zeros_like(pr) if orV_rcv is None else _f64(orV_rcv).reshape(-1, 3)
+    nb = np.ascontiguousarray(np.asarray(nb_img, dtype=np.int32).reshape(3))
+    if Tdiff is None:
+        Tdiff = Tmax
+    assert mode != FP32_REF
+    ns = n_samples(Tmax, fs)
+    out = np.zeros((ps.shape[0], pr.shape[0], ns), dtype=np.float64)
+    env = np.zeros((ps.shape[0] * pr.shape[0], 3), dtype=np.float64)
+    sp = PATTERNS[spkr_pattern] if isinstance(spkr_pattern, str) else int(spkr_pattern)
+    mp = PATTERNS[mic_pattern] if isinstance(mic_pattern, str) else int(mic_pattern)
+    kind = {"logistic": 0, "gaussian": 1}[noise]
+    nf = ctypes.POINTER(ctypes.c_float)()
+    lib().orc_simulate_rir_d(_ptr(room, ctypes.c_double), _ptr(b, ctypes.c_double), _ptr(ps, ctypes.c_double),
+                             ps.shape[0], _ptr(pr, ctypes.c_double), pr.shape[0], _ptr(os_, ctypes.c_double),
+                             _ptr(or_, ctypes.c_double), sp, mp, _ptr(nb, ctypes.c_int), float(Tdiff), float(Tmax),
+                             float(fs), float(c), int(mode), kind, int(seed), int(room_index), int(call_offset),
+                             float(level), _ptr(out, ctypes.c_double), _ptr(env, ctypes.c_double), nf, nf, nf, nf,
+                             nf, nf)
     if return_env:
         return out, env
     return out

@@ -372,18 +372,23 @@ int orc_images(const float room[3], const float beta_f[6], const float* pos_src,
 /* rir_base: index of the first RIR of this room in the noise counter        */
 /* (0 for a single-room call).  level multiplies the tail (default 1).       */
 /* ------------------------------------------------------------------------ */
-int orc_simulate_rir(const float room[3], const float beta_f[6], const float* pos_src, int M_src,
-                     const float* pos_rcv, int M_rcv, const float* orv_src, const float* orv_rcv,
-                     int spkr, int mic, const int nb[3], float Tdiff, float Tmax, float fs, float c,
-                     int mode, int noise_kind, uint64_t seed, uint32_t room_index, uint32_t call_offset,
-                     double level, double* out, double* env_out /* [M][3]: A, alpha, tau_dp or NULL */) {
+/* Core with double-precision geometry (used directly by the gradient checks, which
+ * perturb positions and betas by less than a float ulp).  room/beta/positions/orientations
+ * are doubles; fs, c, Tdiff and Tmax keep the reference's float semantics. */
+int orc_simulate_rir_d(const double room[3], const double beta_d[6], const double* pos_src, int M_src,
+                       const double* pos_rcv, int M_rcv, const double* orv_src, const double* orv_rcv,
+                       int spkr, int mic, const int nb[3], float Tdiff, float Tmax, float fs, float c,
+                       int mode, int noise_kind, uint64_t seed, uint32_t room_index, uint32_t call_offset,
+                       double level, double* out, double* env_out,
+                       const float* f32_room, const float* f32_beta, const float* f32_src, const float* f32_rcv,
+                       const float* f32_osrc, const float* f32_orcv) {
     int nISM = orc_n_samples(Tdiff, fs);
     int nSamples = orc_n_samples(Tmax, fs);
     int nDiff = nSamples - nISM;
     int Tw = orc_tw(fs);
     double L[3] = {room[0], room[1], room[2]};
     double beta[6];
-    for (int i = 0; i < 6; i++) beta[i] = beta_f[i];
+    for (int i = 0; i < 6; i++) beta[i] = beta_d[i];
     double fs_over_c = (double)fs / (double)c;
     long N = (long)nb[0] * nb[1] * nb[2];
     int M = M_src * M_rcv;
@@ -401,8 +406,8 @@ int orc_simulate_rir(const float room[3], const float beta_f[6], const float* po
         for (int m = 0; m < M; m++) {
             int s = m / M_rcv, r = m % M_rcv;
             float tdp = 0.0f;
-            orc_fp32_ref_rir(room, beta_f, pos_src + 3 * s, pos_rcv + 3 * r, orv_src + 3 * s,
-                             orv_rcv + 3 * r, spkr, mic, nb, fs, c, nISM, part + (size_t)m * nch * nISM, &tdp);
+            orc_fp32_ref_rir(f32_room, f32_beta, f32_src + 3 * s, f32_rcv + 3 * r, f32_osrc + 3 * s,
+                             f32_orcv + 3 * r, spkr, mic, nb, fs, c, nISM, part + (size_t)m * nch * nISM, &tdp);
             (void)tdp;
             for (long k = 1; k < nch; k++) memset(part + ((size_t)m * nch + k) * nISM, 0, sizeof(double) * nISM);
         }
@@ -484,6 +489,27 @@ int orc_simulate_rir(const float room[3], const float beta_f[6], const float* po
     free(part);
     free(lut.tab);
     return 0;
+}
+
+int orc_simulate_rir(const float room[3], const float beta_f[6], const float* pos_src, int M_src,
+                     const float* pos_rcv, int M_rcv, const float* orv_src, const float* orv_rcv,
+                     int spkr, int mic, const int nb[3], float Tdiff, float Tmax, float fs, float c,
+                     int mode, int noise_kind, uint64_t seed, uint32_t room_index, uint32_t call_offset,
+                     double level, double* out, double* env_out /* [M][3]: A, alpha, tau_dp or NULL */) {
+    double L[3], b[6];
+    for (int i = 0; i < 3; i++) L[i] = room[i];
+    for (int i = 0; i < 6; i++) b[i] = beta_f[i];
+    double* ps = (double*)malloc(sizeof(double) * 3 * (size_t)M_src);
+    double* pr = (double*)malloc(sizeof(double) * 3 * (size_t)M_rcv);
+    double* os = (double*)malloc(sizeof(double) * 3 * (size_t)M_src);
+    double* orr = (double*)malloc(sizeof(double) * 3 * (size_t)M_rcv);
+    for (int i = 0; i < 3 * M_src; i++) { ps[i] = pos_src[i]; os[i] = orv_src[i]; }
+    for (int i = 0; i < 3 * M_rcv; i++) { pr[i] = pos_rcv[i]; orr[i] = orv_rcv[i]; }
+    int rc = orc_simulate_rir_d(L, b, ps, M_src, pr, M_rcv, os, orr, spkr, mic, nb, Tdiff, Tmax, fs, c, mode,
+                                noise_kind, seed, room_index, call_offset, level, out, env_out, room, beta_f,
+                                pos_src, pos_rcv, orv_src, orv_rcv);
+    free(ps); free(pr); free(os); free(orr);
+    return rc;
 }
 
 /* Table used by the LUT mode (for the tests of the table formula). */

@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rooms", type=int, default=4096)
-    ap.add_argument("--cpu-sample-rooms", type=int, default=6)
+    ap.add_argument("--cpu-sample-rooms", type=int, default=400)
     ap.add_argument("--skip-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

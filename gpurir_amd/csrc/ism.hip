@@ -102,8 +102,15 @@ __device__ inline float pattern_gain(float c, int p) {
     }
 }
 
-__device__ inline void lds_add(float* p, float v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// Taps are accumulated as 64-bit fixed point with integer LDS atomics
+// (ds_add_u64): on gfx950 these run at the rate of plain LDS stores, while
+// ds_add_f32 is an order of magnitude slower.  A tap already carries the
+// workgroup's power-of-two scale and is rounded to an int32; integer sums are
+// exact and independent of the order of the atomics.
+__device__ inline void lds_add(long long* p, float v) {
+    const long long q = (long long)__float2int_rn(v);
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p), (unsigned long long)q, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Per-axis tables of one RIR: offs[e] = 2 m L + (1-2q) src - rcv (FP64, exact for
@@ -190,7 +197,7 @@ __device__ inline void load_orient(const float* o, float out[4]) {
 //          sigma_j = (-1)^(j+1); Bh = B/2 (sigma = +1 on the odd first tap).
 //   LUT:   lut (LDS) holds (T[i], T[i+1] - T[i]) on the 1/16-sample grid.
 template <bool MASK, bool DOWN>
-__device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float xa, float Bh, float BC, float BS,
+__device__ __forceinline__ void taps8_exact(long long* ap, const float2* jt, float xa, float Bh, float BC, float BS,
                                             float halfTw) {
     const float4* j4 = reinterpret_cast<const float4*>(jt);
     const float4 q0 = j4[0], q1 = j4[1], q2 = j4[2], q3 = j4[3];
@@ -208,7 +215,7 @@ __device__ __forceinline__ void taps8_exact(float* ap, const float2* jt, float x
 }
 
 template <bool MASK, bool DOWN>
-__device__ __forceinline__ void taps8_lut(float* ap, const float2* lut, float xa, float amp, float center,
+__device__ __forceinline__ void taps8_lut(long long* ap, const float2* lut, float xa, float amp, float center,
                                           float half_int, int lut_len) {
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -232,22 +239,24 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     double* offs = reinterpret_cast<double*>(smem_raw);
-    float* att = reinterpret_cast<float*>(offs + a.ntab_max);
-    float* acc = att + a.ntab_max;
-    int* qa = reinterpret_cast<int*>(acc + a.acc_len);
+    long long* acc = reinterpret_cast<long long*>(offs + a.ntab_max);  // [acc_len] fixed point
+    float* att = reinterpret_cast<float*>(acc + a.acc_len);
+    int* qa = reinterpret_cast<int*>(att + a.ntab_max);
     float* qf = reinterpret_cast<float*>(qa + QCAP);
     float* qB = qf + QCAP;
     float* qC = qB + QCAP;
     float* qS = qC + QCAP;
-    int* qcount = reinterpret_cast<int*>(qS + QCAP);
+    int* qcount = reinterpret_cast<int*>(qS + QCAP);     // [0] queue length, [1..3] per-axis max |att| bits
     float2* jt = reinterpret_cast<float2*>(qcount + 4);   // [nJp] tap constants (exact mode)
     float2* lut = jt + a.nJp;                              // [lut_len + 1] (LUT mode)
 
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
     const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    if (tid < 4) qcount[tid] = 0;
+    __syncthreads();
     build_axis_tables(R, ps, pr, offs, att, tid, WG);
-    for (int e = tid; e < a.acc_len; e += WG) acc[e] = 0.0f;
+    for (int e = tid; e < a.acc_len; e += WG) acc[e] = 0;
     if (MODE == PREC_LUT)
         for (int e = tid; e < a.lut_len + 1; e += WG) lut[e] = (e < a.lut_len) ? a.lut[e] : make_float2(0.0f, 0.0f);
     else
@@ -255,6 +264,24 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
     if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
     if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
+
+    // fixed-point scale: |tap| <= max|att| / (4 pi d_direct) (no image is closer than the
+    // direct path; directivity gains and the windowed sinc are bounded by 1)
+    for (int e = tid; e < R.nb[0] + R.nb[1] + R.nb[2]; e += WG) {
+        const int d = (e < R.nb[0]) ? 0 : (e < R.nb[0] + R.nb[1] ? 1 : 2);
+        atomicMax(qcount + 1 + d, __float_as_int(fabsf(att[e])));
+    }
+    __syncthreads();
+    float scale, inv_scale;
+    {
+        const float dx = ps[0] - pr[0], dy = ps[1] - pr[1], dz = ps[2] - pr[2];
+        const float bound = __int_as_float(qcount[1]) * __int_as_float(qcount[2]) * __int_as_float(qcount[3]) *
+                            __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
+        int E = (bound > 0.0f && bound < 3.0e38f) ? 29 - ilogbf(bound) : 0;
+        E = min(max(E, -120), 120);
+        scale = ldexpf(1.0f, E);
+        inv_scale = ldexpf(1.0f, -E);
+    }
 
     const int J1 = a.J0 + a.nJ - 1;
     const int lo_n0 = it.t_lo - J1;        // images with n0 in [lo_n0, hi_n0] touch the tile
@@ -291,12 +318,12 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
                     qa[q] = n0 + acc_shift;
                     if (MODE == PREC_LUT) {
                         qf[q] = f;
-                        qB[q] = amp;
+                        qB[q] = amp * scale;
                     } else {
                         // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
                         // 2^-30 samples changes no tap by more than ~1e-9 relative.
                         const float fc = fmaxf(f, 0x1p-30f);
-                        const float B = amp * sinpif(fc) * 0.31830988618379067f;
+                        const float B = amp * scale * sinpif(fc) * 0.31830988618379067f;
                         const float ph = fc * a.inv_tw2;
                         qf[q] = fc;
                         qB[q] = 0.5f * B;
@@ -312,10 +339,10 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         for (int b = wave; b * 64 < nq; b += NWAVE) {
             const int e = b * 64 + lane;
             if (e < nq) {
-                float* ap = acc + qa[e];
+                long long* ap = acc + qa[e];
                 const float f = qf[e];
                 const int nd = a.nb_dn, nu = a.nb_up;
-                float* apu = ap + 8 * nd;  // tap j = 1
+                long long* apu = ap + 8 * nd;  // tap j = 1
                 if (MODE == PREC_LUT) {
                     const float amp = qB[e];
                     const float cen = (float)(a.lut_len / 2);
@@ -358,14 +385,14 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     if (it.i0 >= it.i1) __syncthreads();
 
     // ---- flush the tile [t_lo, t_hi)
-    const float* src = acc + (J1 - (1 - 8 * a.nb_dn));
+    const long long* src = acc + (J1 - (1 - 8 * a.nb_dn));
     const int nt = it.t_hi - it.t_lo;
     float* dst;
     if (R.n_chunks == 1)
         dst = a.out + R.out_off + (long long)it.rir * R.nSamples + it.t_lo;
     else
         dst = a.part + R.part_off + ((long long)it.rir * R.n_chunks + it.chunk) * R.nISM + it.t_lo;
-    for (int t = tid; t < nt; t += WG) dst[t] = src[t];
+    for (int t = tid; t < nt; t += WG) dst[t] = (float)((double)src[t] * (double)inv_scale);
 }
 
 __device__ inline void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1,
@@ -616,7 +643,7 @@ struct Plan {
     int tt_max = 0;
 };
 
-constexpr int TILE_DEFAULT = 8192;
+constexpr int TILE_DEFAULT = 4096;
 constexpr int MIN_CHUNK = 512;
 constexpr int TARGET_ITEMS = 2048;
 
@@ -753,7 +780,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
     // acc index = n0 - t_lo + J1 + (j - Jp0), n0 <= t_hi - 1 - J0, j - Jp0 < nJp
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.nJp, 4);
-    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(float) * S.acc_len + (size_t)QCAP * 5 * 4 + 16 +
+    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len + (size_t)QCAP * 5 * 4 + 16 +
             sizeof(float2) * T.nJp;
     if (S.prec == PREC_LUT) S.lds += sizeof(float2) * (T.lut.size() + 1);
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");

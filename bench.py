@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--rooms", type=int, default=4096)
     ap.add_argument("--cpu-sample-rooms", type=int, default=400)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--only-batch", action="store_true", help="time the batched workload only (for profiler runs)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (profiler traffic reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus must be 1)")
@@ -66,6 +69,15 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     fs = W.FS
+
+    if args.copy_mb > 0:
+        n = args.copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] reference copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
+        del src, dst
 
     # ---------------- batched: 4096 uneven rooms, one graph replay per step
     log(f"[bench] generating {args.rooms} uneven rooms")
@@ -106,6 +118,11 @@ def main():
                              Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
         errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
     log(f"[bench] batch rel-L2 vs FP64 (3 rooms): {max(errs):.3e}")
+
+    if args.only_batch:
+        print(json.dumps({"metric": "batched RIRs/s (profiler run)", "value": rirs_per_s, "unit": "RIRs/s",
+                          "ms_per_step": ms_step, "kernel_ms": ms_acc, "taps": taps}), flush=True)
+        return
 
     # ---------------- plain drop-in: one room, numpy in / numpy out, FP32 exact sinc
     c1 = W.plain_room()

@@ -50,19 +50,19 @@ struct AccArgs {
     const float* pos_rcv;
     const float* orv_src;
     const float* orv_rcv;
-    const float2* jtab;   // per tap offset j: (sigma_j cos(2 pi j/Tw), sigma_j sin(2 pi j/Tw)), sigma_j = (-1)^(j+1)
-    const float2* lut;    // LUT mode: (T[i], T[i+1]-T[i])
+    const float4* jtab;   // [n_pass*64] per lane of each pass: (jf, sigma_j cos(2 pi j/Tw), sigma_j sin(..), sigma_j)
+    const float2* lut;    // LUT mode, polyphase: [16][lut_row] (T[16k+ph], T[16k+ph+1]-T[16k+ph])
     float* out;
     float* part;
     int spkr, mic;
     int J0, nJ;           // tap offsets j = t - floor(tau) that can be in the window: [J0, J0 + nJ)
-    int nb_dn, nb_up;     // 8-tap blocks below (j <= 0) and above (j >= 1) the image's peak
-    int nJp;              // 8 * (nb_dn + nb_up): evaluated offsets j in [1 - 8 nb_dn, 8 nb_up]
+    int n_pass;           // passes of 64 lanes over j = J0 + 64 p + lane
+    unsigned mask_bits;   // bit p: pass p has lanes whose window membership depends on f (0: no test)
     float halfTw;         // support |t - tau| < halfTw
     float inv_tw2;        // 2/Tw
-    int lut_len;
+    int lut_center, lut_row, lut_k_lo;  // LUT: center index, row length, first k of a row
     int ntab_max;         // max Nx+Ny+Nz over the rooms of the launch
-    int acc_len;          // floats of the LDS accumulator
+    int acc_len;          // 64-bit words of the LDS accumulator
 };
 
 struct FinArgs {
@@ -113,27 +113,71 @@ __device__ inline void lds_add(long long* p, float v) {
                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Room constants in scalar form.  Kernels never keep a RoomDev copy: any runtime
+// index into such a copy (e.g. by axis) makes the compiler place it in scratch.
+struct RoomK {
+    double L0, L1, L2, fs_over_c, inv_nyz, inv_nz;
+    float b0, b1, b2, b3, b4, b5, level;
+    int Nx, Ny, Nz, n_rcv, src_off, rcv_off, nISM, nSamples, n_chunks, rir_off;
+    long long out_off, part_off;
+    unsigned room_id;
+};
+
+__device__ inline RoomK load_room(const RoomDev* __restrict__ p) {
+    RoomK r;
+    r.L0 = p->L[0];
+    r.L1 = p->L[1];
+    r.L2 = p->L[2];
+    r.fs_over_c = p->fs_over_c;
+    r.inv_nyz = p->inv_nyz;
+    r.inv_nz = p->inv_nz;
+    r.b0 = p->beta[0];
+    r.b1 = p->beta[1];
+    r.b2 = p->beta[2];
+    r.b3 = p->beta[3];
+    r.b4 = p->beta[4];
+    r.b5 = p->beta[5];
+    r.level = p->level;
+    r.Nx = p->nb[0];
+    r.Ny = p->nb[1];
+    r.Nz = p->nb[2];
+    r.n_rcv = p->n_rcv;
+    r.src_off = p->src_off;
+    r.rcv_off = p->rcv_off;
+    r.nISM = p->nISM;
+    r.nSamples = p->nSamples;
+    r.n_chunks = p->n_chunks;
+    r.rir_off = p->rir_off;
+    r.out_off = p->out_off;
+    r.part_off = p->part_off;
+    r.room_id = p->room_id;
+    return r;
+}
+
 // Per-axis tables of one RIR: offs[e] = 2 m L + (1-2q) src - rcv (FP64, exact for
 // float inputs) and att[e] = (-b_lo)^|m-q| (-b_hi)^|m| (src/gpuRIR_cuda.cu:336-340,353).
 // Axis 0 occupies [0, Nx), axis 1 [Nx, Nx+Ny), axis 2 the rest.
-__device__ inline void build_axis_tables(const RoomDev& R, const float* ps, const float* pr, double* offs, float* att,
+__device__ inline void axis_entry(int n, int N, double L, float sd, float rd, float blo, float bhi, double* offs,
+                                  float* att) {
+    const int k = n - N / 2;
+    const int m = (k + 1) >> 1;  // ceil(k / 2)
+    const int q = k & 1;         // |k % 2|
+    const double sgn = (double)(1 - 2 * q);
+    const double av = 2.0 * (double)m * L;
+    const double bv = sgn * (double)sd;
+    offs[n] = (av + bv) - (double)rd;
+    att[n] = (float)(ipow(-(double)blo, abs(m - q)) * ipow(-(double)bhi, abs(m)));
+}
+
+// One loop per axis: every field is read with a constant index (a runtime axis
+// index into the room constants would place them in scratch memory).
+__device__ inline void build_axis_tables(const RoomK& R, const float* ps, const float* pr, double* offs, float* att,
                                          int tid, int nthreads) {
-    const int Nx = R.nb[0], Ny = R.nb[1];
-    const int ntab = Nx + Ny + R.nb[2];
-    for (int e = tid; e < ntab; e += nthreads) {
-        const int d = (e < Nx) ? 0 : (e < Nx + Ny ? 1 : 2);
-        const int n = e - (d == 0 ? 0 : (d == 1 ? Nx : Nx + Ny));
-        const int N = R.nb[d];
-        const int k = n - N / 2;
-        const int m = (k + 1) >> 1;  // ceil(k / 2)
-        const int q = k & 1;         // |k % 2|
-        const double sgn = (double)(1 - 2 * q);
-        const double av = 2.0 * (double)m * R.L[d];
-        const double bv = sgn * (double)ps[d];
-        offs[e] = (av + bv) - (double)pr[d];
-        const double b0 = -(double)R.beta[2 * d], b1 = -(double)R.beta[2 * d + 1];
-        att[e] = (float)(ipow(b0, abs(m - q)) * ipow(b1, abs(m)));
-    }
+    for (int n = tid; n < R.Nx; n += nthreads) axis_entry(n, R.Nx, R.L0, ps[0], pr[0], R.b0, R.b1, offs, att);
+    for (int n = tid; n < R.Ny; n += nthreads)
+        axis_entry(n, R.Ny, R.L1, ps[1], pr[1], R.b2, R.b3, offs + R.Nx, att + R.Nx);
+    for (int n = tid; n < R.Nz; n += nthreads)
+        axis_entry(n, R.Nz, R.L2, ps[2], pr[2], R.b4, R.b5, offs + R.Nx + R.Ny, att + R.Nx + R.Ny);
 }
 
 struct ImageVal {
@@ -142,9 +186,9 @@ struct ImageVal {
 };
 
 // Stage (a) for image i of one RIR (src/gpuRIR_cuda.cu:336-369).
-__device__ inline ImageVal image_eval(const RoomDev& R, int i, const double* offs, const float* att, int spkr,
+__device__ inline ImageVal image_eval(const RoomK& R, int i, const double* offs, const float* att, int spkr,
                                       int mic, const float os[4], const float orr[4]) {
-    const int Nx = R.nb[0], Ny = R.nb[1], Nz = R.nb[2];
+    const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
     const int nyz = Ny * Nz;
     const int nx = (int)(((double)i + 0.5) * R.inv_nyz);
     const int rem = i - nx * nyz;
@@ -187,56 +231,24 @@ __device__ inline void load_orient(const float* o, float out[4]) {
     out[3] = __builtin_amdgcn_rsqf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
 }
 
-// 8 consecutive taps of one image.  The blocks tile the offsets outward from the
-// image's peak: "down" blocks hold j = -7-8k .. -8k and "up" blocks j = 1+8k .. 8+8k,
-// so that the first tap of every block has an odd j.  x = j - f is formed by one
-// rounding from an anchor that is itself one rounding from the exact value
-// (-8k - f for a down block, 1 + 8k - f for an up block): the relative error of
-// every x stays near 2^-23 however small |x| is.
-//   exact: jt (LDS) holds (sigma_j cos(2 pi j/Tw), sigma_j sin(2 pi j/Tw)),
-//          sigma_j = (-1)^(j+1); Bh = B/2 (sigma = +1 on the odd first tap).
-//   LUT:   lut (LDS) holds (T[i], T[i+1] - T[i]) on the 1/16-sample grid.
-template <bool MASK, bool DOWN>
-__device__ __forceinline__ void taps8_exact(long long* ap, const float2* jt, float xa, float Bh, float BC, float BS,
-                                            float halfTw) {
-    const float4* j4 = reinterpret_cast<const float4*>(jt);
-    const float4 q0 = j4[0], q1 = j4[1], q2 = j4[2], q3 = j4[3];
-    const float cs[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-        const float x = DOWN ? xa - (float)(7 - u) : xa + (float)u;
-        const float r = __builtin_amdgcn_rcpf(x);
-        const float w = fmaf(cs[2 * u], BC, fmaf(cs[2 * u + 1], BS, (u & 1) ? -Bh : Bh));
-        float v = w * r;
-        if (MASK) v = (fabsf(x) < halfTw) ? v : 0.0f;
-        lds_add(ap + u, v);
-    }
-}
-
-template <bool MASK, bool DOWN>
-__device__ __forceinline__ void taps8_lut(long long* ap, const float2* lut, float xa, float amp, float center,
-                                          float half_int, int lut_len) {
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-        const float x = DOWN ? xa - (float)(7 - u) : xa + (float)u;
-        const float uu = fmaf(x, 16.0f, center);
-        int i = (int)uu;
-        const float al = uu - floorf(uu);
-        if (MASK) i = min(max(i, 0), lut_len - 1);
-        const float2 t = lut[i];
-        float v = amp * fmaf(al, t.y, t.x);
-        if (MASK) v = (fabsf(x) < half_int) ? v : 0.0f;
-        lds_add(ap + u, v);
-    }
-}
-
-template <int MODE>
+// Stage (b) layout: a wave walks the queue one image at a time; lane l of pass p
+// owns tap offset j = J0 + 64 p + l, i.e. sample t = floor(tau) + j.  The 64 lanes of
+// an LDS atomic hit 64 consecutive words (no bank conflicts), x = j - f is a single
+// rounding of an exact integer minus the fraction, and the per-offset window
+// constants stay in registers for the whole kernel:
+//   sigma_j w_j(f) B = sigma_j B/2 + (sigma_j cos 2pi j/Tw)(B/2 cos 2pi f/Tw)
+//                                   + (sigma_j sin 2pi j/Tw)(B/2 sin 2pi f/Tw)
+//   tap = sigma_j w_j(f) B / x,   B = amp sin(pi f)/pi,   sigma_j = (-1)^(j+1).
+// LUT mode: u = 16 x + center = 16 j + center + floor(-16 f) + frac(-16 f), so all
+// lanes share the interpolation weight and read consecutive entries of one phase
+// row of the table.
+template <int MODE, int PMAX, bool MASK>
 __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const Item it = a.items[blockIdx.x];
-    const RoomDev R = a.rooms[it.room];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const RoomK R = load_room(a.rooms + it.room);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     double* offs = reinterpret_cast<double*>(smem_raw);
     long long* acc = reinterpret_cast<long long*>(offs + a.ntab_max);  // [acc_len] fixed point
@@ -247,8 +259,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     float* qC = qB + QCAP;
     float* qS = qC + QCAP;
     int* qcount = reinterpret_cast<int*>(qS + QCAP);     // [0] queue length, [1..3] per-axis max |att| bits
-    float2* jt = reinterpret_cast<float2*>(qcount + 4);   // [nJp] tap constants (exact mode)
-    float2* lut = jt + a.nJp;                              // [lut_len + 1] (LUT mode)
+    float2* lut = reinterpret_cast<float2*>(qcount + 4);  // LUT mode: [16 * lut_row] polyphase table
 
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
@@ -258,17 +269,25 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     build_axis_tables(R, ps, pr, offs, att, tid, WG);
     for (int e = tid; e < a.acc_len; e += WG) acc[e] = 0;
     if (MODE == PREC_LUT)
-        for (int e = tid; e < a.lut_len + 1; e += WG) lut[e] = (e < a.lut_len) ? a.lut[e] : make_float2(0.0f, 0.0f);
-    else
-        for (int e = tid; e < a.nJp; e += WG) jt[e] = a.jtab[e];
+        for (int e = tid; e < 16 * a.lut_row; e += WG) lut[e] = a.lut[e];
+    // per-lane constants of every pass
+    float jf[PMAX], cj[PMAX], sj[PMAX], hj[PMAX];
+#pragma unroll
+    for (int p = 0; p < PMAX; p++) {
+        const float4 c = (p < a.n_pass) ? a.jtab[64 * p + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        jf[p] = c.x;
+        cj[p] = c.y;
+        sj[p] = c.z;
+        hj[p] = c.w;
+    }
     float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
     if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
     if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
 
     // fixed-point scale: |tap| <= max|att| / (4 pi d_direct) (no image is closer than the
     // direct path; directivity gains and the windowed sinc are bounded by 1)
-    for (int e = tid; e < R.nb[0] + R.nb[1] + R.nb[2]; e += WG) {
-        const int d = (e < R.nb[0]) ? 0 : (e < R.nb[0] + R.nb[1] ? 1 : 2);
+    for (int e = tid; e < R.Nx + R.Ny + R.Nz; e += WG) {
+        const int d = (e < R.Nx) ? 0 : (e < R.Nx + R.Ny ? 1 : 2);
         atomicMax(qcount + 1 + d, __float_as_int(fabsf(att[e])));
     }
     __syncthreads();
@@ -286,7 +305,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     const int J1 = a.J0 + a.nJ - 1;
     const int lo_n0 = it.t_lo - J1;        // images with n0 in [lo_n0, hi_n0] touch the tile
     const int hi_n0 = it.t_hi - 1 - a.J0;
-    const int acc_shift = J1 - it.t_lo;    // acc index of tap j = n0 + acc_shift + (j - (1 - 8 nb_dn))
+    const int acc_shift = J1 - it.t_lo;    // acc index of tap j = n0 + acc_shift + (j - J0)
 
     for (int r0 = it.i0; r0 < it.i1; r0 += QCAP) {
         if (tid == 0) *qcount = 0;
@@ -334,50 +353,55 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
             }
         }
         __syncthreads();
-        // ---- phase B: every live image scatters its taps into the LDS tile
-        const int nq = *qcount;
-        for (int b = wave; b * 64 < nq; b += NWAVE) {
-            const int e = b * 64 + lane;
-            if (e < nq) {
-                long long* ap = acc + qa[e];
-                const float f = qf[e];
-                const int nd = a.nb_dn, nu = a.nb_up;
-                long long* apu = ap + 8 * nd;  // tap j = 1
+        // ---- phase B: every live image scatters its taps into the LDS tile.  The queue
+        // entry of the wave's next image is read before this image's atomics are issued,
+        // so the in-order LDS pipe never makes those reads wait behind the atomics.
+        const int nq = __builtin_amdgcn_readfirstlane(*qcount);
+        if (wave < nq) {
+            int c_a = qa[wave];
+            float c_f = qf[wave], c_B = qB[wave], c_C = qC[wave], c_S = qS[wave];
+            for (int e = wave; e < nq; e += NWAVE) {
+                const int en = (e + NWAVE < nq) ? e + NWAVE : e;
+                const int n_a = qa[en];
+                const float n_f = qf[en], n_B = qB[en], n_C = qC[en], n_S = qS[en];
+                long long* ap = acc + c_a + lane;
+                const float f = c_f;
                 if (MODE == PREC_LUT) {
-                    const float amp = qB[e];
-                    const float cen = (float)(a.lut_len / 2);
-                    for (int k = 0; k < nd; k++) {
-                        const float xa = (float)(-8 * k) - f;
-                        if (k == nd - 1)
-                            taps8_lut<true, true>(apu - 8 * (k + 1), lut, xa, amp, cen, a.halfTw, a.lut_len);
-                        else
-                            taps8_lut<false, true>(apu - 8 * (k + 1), lut, xa, amp, cen, a.halfTw, a.lut_len);
-                    }
-                    for (int k = 0; k < nu; k++) {
-                        const float xa = (float)(1 + 8 * k) - f;
-                        if (k == nu - 1)
-                            taps8_lut<true, false>(apu + 8 * k, lut, xa, amp, cen, a.halfTw, a.lut_len);
-                        else
-                            taps8_lut<false, false>(apu + 8 * k, lut, xa, amp, cen, a.halfTw, a.lut_len);
+                    const float amp = c_B;
+                    const float m16 = -16.0f * f;
+                    const float k0 = floorf(m16);
+                    const float al = m16 - k0;
+                    const int i0 = a.lut_center + (int)k0;  // >= 0
+                    const int ph = i0 & 15;
+                    const float2* row = lut + ph * a.lut_row + ((i0 >> 4) - a.lut_k_lo) + a.J0 + lane;
+#pragma unroll
+                    for (int p = 0; p < PMAX; p++) {
+                        if (p < a.n_pass) {
+                            const float2 t = row[64 * p];
+                            float v = amp * fmaf(al, t.y, t.x);
+                            if (MASK) v = (fabsf(jf[p] - f) < a.halfTw) ? v : 0.0f;
+                            lds_add(ap + 64 * p, v);
+                        }
                     }
                 } else {
-                    const float Bh = qB[e], BC = qC[e], BS = qS[e];
-                    const float2* jtu = jt + 8 * nd;
-                    for (int k = 0; k < nd; k++) {
-                        const float xa = (float)(-8 * k) - f;
-                        if (k == nd - 1)
-                            taps8_exact<true, true>(apu - 8 * (k + 1), jtu - 8 * (k + 1), xa, Bh, BC, BS, a.halfTw);
-                        else
-                            taps8_exact<false, true>(apu - 8 * (k + 1), jtu - 8 * (k + 1), xa, Bh, BC, BS, a.halfTw);
-                    }
-                    for (int k = 0; k < nu; k++) {
-                        const float xa = (float)(1 + 8 * k) - f;
-                        if (k == nu - 1)
-                            taps8_exact<true, false>(apu + 8 * k, jtu + 8 * k, xa, Bh, BC, BS, a.halfTw);
-                        else
-                            taps8_exact<false, false>(apu + 8 * k, jtu + 8 * k, xa, Bh, BC, BS, a.halfTw);
+                    const float Bh = c_B, BC = c_C, BS = c_S;
+#pragma unroll
+                    for (int p = 0; p < PMAX; p++) {
+                        if (p < a.n_pass) {
+                            const float x = jf[p] - f;
+                            const float r = __builtin_amdgcn_rcpf(x);
+                            const float w = fmaf(cj[p], BC, fmaf(sj[p], BS, hj[p] * Bh));
+                            float v = w * r;
+                            if (MASK) v = (fabsf(x) < a.halfTw) ? v : 0.0f;
+                            lds_add(ap + 64 * p, v);
+                        }
                     }
                 }
+                c_a = n_a;
+                c_f = n_f;
+                c_B = n_B;
+                c_C = n_C;
+                c_S = n_S;
             }
         }
         __syncthreads();
@@ -385,7 +409,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     if (it.i0 >= it.i1) __syncthreads();
 
     // ---- flush the tile [t_lo, t_hi)
-    const long long* src = acc + (J1 - (1 - 8 * a.nb_dn));
+    const long long* src = acc + (J1 - a.J0);
     const int nt = it.t_hi - it.t_lo;
     float* dst;
     if (R.n_chunks == 1)
@@ -431,7 +455,7 @@ __device__ inline double block_sum(double v, double* red) {
 __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     __shared__ double red[NWAVE];
     const int g = blockIdx.x;
-    const RoomDev R = a.rooms[a.rir_room[g]];
+    const RoomK R = load_room(a.rooms + a.rir_room[g]);
     const int m = g - R.rir_off;
     const int s = m / R.n_rcv, r = m - s * R.n_rcv;
     const int tid = threadIdx.x;
@@ -468,9 +492,8 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     const double P = block_sum(pw, red) / (double)(nISM - w_start);
 
     // Sabine T60 (src/gpuRIR_cuda.cu:123-130) and the envelope parameters
-    const double* L = R.L;
-    double b[6];
-    for (int k = 0; k < 6; k++) b[k] = R.beta[k];
+    const double L[3] = {R.L0, R.L1, R.L2};
+    const double b[6] = {R.b0, R.b1, R.b2, R.b3, R.b4, R.b5};
     const double Sa = ((1.0 - b[0] * b[0]) + (1.0 - b[1] * b[1])) * L[1] * L[2] +
                       ((1.0 - b[2] * b[2]) + (1.0 - b[3] * b[3])) * L[0] * L[2] +
                       ((1.0 - b[4] * b[4]) + (1.0 - b[5] * b[5])) * L[0] * L[1];
@@ -518,7 +541,7 @@ __global__ __launch_bounds__(WG) void image_list_kernel(const RoomDev* rooms, co
                                                         const float* orv_rcv, int spkr, int mic, int ntab_max,
                                                         int* n0_out, float* f_out, float* amp_out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const RoomDev R = rooms[0];
+    const RoomK R = load_room(rooms);
     const int m = blockIdx.y;
     const int s = m / R.n_rcv, r = m - s * R.n_rcv;
     double* offs = reinterpret_cast<double*>(smem_raw);
@@ -530,7 +553,7 @@ __global__ __launch_bounds__(WG) void image_list_kernel(const RoomDev* rooms, co
     if (spkr != 0) load_orient(orv_src + 3 * s, os);
     if (mic != 0) load_orient(orv_rcv + 3 * r, orr);
     __syncthreads();
-    const long long N = (long long)R.nb[0] * R.nb[1] * R.nb[2];
+    const long long N = (long long)R.Nx * R.Ny * R.Nz;
     const int i = blockIdx.x * WG + threadIdx.x;
     if (i < N) {
         const ImageVal iv = image_eval(R, i, offs, att, spkr, mic, os, orr);
@@ -547,7 +570,7 @@ __global__ __launch_bounds__(WG) void count_taps_kernel(AccArgs a, unsigned long
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ unsigned long long red[NWAVE];
     const Item it = a.items[blockIdx.x];
-    const RoomDev R = a.rooms[it.room];
+    const RoomK R = load_room(a.rooms + it.room);
     double* offs = reinterpret_cast<double*>(smem_raw);
     float* att = reinterpret_cast<float*>(offs + a.ntab_max);
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
@@ -581,23 +604,54 @@ __global__ __launch_bounds__(WG) void count_taps_kernel(AccArgs a, unsigned long
 
 struct TapTables {
     int J0 = 0, nJ = 0;        // offsets j that can be in the window
-    int nb_dn = 0, nb_up = 0;  // evaluated blocks: j in [1 - 8 nb_dn, 8 nb_up]
-    int nJp = 0;               // 8 (nb_dn + nb_up)
+    int n_pass = 0;            // ceil(nJ / 64)
+    unsigned mask_bits = 0;    // passes that need the |x| < halfTw test
     float halfTw = 0.0f;
-    std::vector<float2> jtab;  // indexed by j - (1 - 8 nb_dn)
-    std::vector<float2> lut;
+    std::vector<float4> jtab;  // [n_pass * 64]: (jf, sigma cos, sigma sin, sigma)
+    int lut_center = 0, lut_row = 0, lut_k_lo = 0;
+    std::vector<float2> lut;   // polyphase [16][lut_row]
 };
 
 // Tap offsets j = t - floor(tau) that can lie inside the window, and their
 // per-offset constants.  Exact mode: |j - f| < Tw/2 for some f in [0,1).  LUT
-// mode: |j - f| < Tw/2 (integer division), src/gpuRIR_cuda.cu:637.
+// mode: |j - f| < Tw/2 (integer division), src/gpuRIR_cuda.cu:637, with the table
+// of src/gpuRIR_cuda.cu:578-590 (1/16-sample grid, odd length, zero outside).
 TapTables make_tap_tables(int precision, int Tw) {
     TapTables T;
-    if (precision == PREC_LUT) {
+    const bool lut = precision == PREC_LUT;
+    if (lut) {
         const int half = Tw / 2;
         T.J0 = -half + 1;
         T.nJ = 2 * half;
         T.halfTw = (float)half;
+    } else {
+        const double h = Tw / 2.0;
+        T.halfTw = (float)h;
+        T.J0 = (int)std::floor(-h) + 1;
+        const int J1 = (int)std::ceil(h + 1.0) - 1;
+        T.nJ = J1 - T.J0 + 1;
+    }
+    const int J1 = T.J0 + T.nJ - 1;
+    T.n_pass = (T.nJ + 63) / 64;
+    // a pass needs the window test if it holds padding lanes (j > J1) or, for an odd
+    // window (half-integer Tw/2, or integer Tw/2 in LUT mode with odd Tw), the end
+    // offsets J0, J1 whose membership depends on f.  For an even window the end
+    // offsets are inside for every f, except j = J1 at f = 0 whose tap is exactly 0.
+    const bool f_dependent_ends = (Tw % 2) != 0;
+    for (int p = 0; p < T.n_pass; p++) {
+        const int jlo = T.J0 + 64 * p, jhi = jlo + 63;
+        const bool pad = jhi > J1;
+        const bool ends = f_dependent_ends && (jlo <= T.J0 || jhi >= J1);
+        if (pad || ends) T.mask_bits |= 1u << p;
+    }
+    T.jtab.resize((size_t)T.n_pass * 64);
+    for (int u = 0; u < T.n_pass * 64; u++) {
+        const int j = T.J0 + u;
+        const double sg = (j & 1) ? 1.0 : -1.0;  // (-1)^(j+1)
+        const double th = 2.0 * M_PI * (double)j / (double)Tw;
+        T.jtab[u] = make_float4((float)j, (float)(sg * std::cos(th)), (float)(sg * std::sin(th)), (float)sg);
+    }
+    if (lut) {
         int len = Tw * 16;
         len += (len % 2) ? 0 : 1;
         const int center = len / 2;
@@ -608,28 +662,21 @@ TapTables make_tap_tables(int precision, int Tw) {
             const double h = 0.5 * (1.0 + std::cos(2.0 * M_PI * x / Tw));
             tab[center + i] = tab[center - i] = h * sn;
         }
-        T.lut.resize(len);
-        for (int i = 0; i < len; i++) T.lut[i] = make_float2((float)tab[i], (float)(tab[i + 1] - tab[i]));
-    } else {
-        const double h = Tw / 2.0;
-        T.halfTw = (float)h;
-        T.J0 = (int)std::floor(-h) + 1;
-        const int J1 = (int)std::ceil(h + 1.0) - 1;
-        T.nJ = J1 - T.J0 + 1;
-    }
-    const int J1 = T.J0 + T.nJ - 1;
-    T.nb_dn = (1 - T.J0 + 7) / 8;
-    T.nb_up = (J1 + 7) / 8;
-    T.nJp = 8 * (T.nb_dn + T.nb_up);
-    T.jtab.assign(T.nJp, make_float2(0.0f, 0.0f));
-    if (precision != PREC_LUT) {
-        const int Jp0 = 1 - 8 * T.nb_dn;
-        for (int u = 0; u < T.nJp; u++) {
-            const int j = Jp0 + u;
-            const double sg = (j & 1) ? 1.0 : -1.0;  // (-1)^(j+1)
-            const double th = 2.0 * M_PI * (double)j / (double)Tw;
-            T.jtab[u] = make_float2((float)(sg * std::cos(th)), (float)(sg * std::sin(th)));
-        }
+        // table index of lane j: i = 16 j + center + k0, k0 = floor(-16 f) in [-16, 0];
+        // row k = floor(i / 16) spans [k_lo, k_hi] over all lanes of all passes
+        auto fdiv = [](int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); };
+        const int i_min = 16 * T.J0 + center - 16;
+        const int i_max = 16 * (T.J0 + 64 * T.n_pass - 1) + center;
+        T.lut_center = center;
+        T.lut_k_lo = fdiv(i_min, 16);
+        T.lut_row = fdiv(i_max, 16) - T.lut_k_lo + 2;
+        T.lut.assign((size_t)16 * T.lut_row, make_float2(0.0f, 0.0f));
+        auto val = [&](int i) { return (i >= 0 && i < len) ? tab[i] : 0.0; };  // zero border
+        for (int ph = 0; ph < 16; ph++)
+            for (int k = 0; k < T.lut_row; k++) {
+                const int i = 16 * (T.lut_k_lo + k) + ph;
+                T.lut[(size_t)ph * T.lut_row + k] = make_float2((float)val(i), (float)(val(i + 1) - val(i)));
+            }
     }
     return T;
 }
@@ -769,20 +816,20 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.off_items = align_up(S.off_rooms + sizeof(RoomDev) * P.rooms.size(), 256);
     S.off_rr = align_up(S.off_items + sizeof(Item) * P.items.size(), 256);
     S.off_jt = align_up(S.off_rr + sizeof(int) * P.rir_room.size(), 256);
-    S.off_lut = align_up(S.off_jt + sizeof(float2) * T.jtab.size(), 256);
+    S.off_lut = align_up(S.off_jt + sizeof(float4) * T.jtab.size(), 256);
     S.plan_bytes = align_up(S.off_lut + sizeof(float2) * (T.lut.size() + 1), 256);
     S.blob.assign(S.plan_bytes, 0);
     std::memcpy(S.blob.data() + S.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(S.blob.data() + S.off_items, P.items.data(), sizeof(Item) * P.items.size());
     std::memcpy(S.blob.data() + S.off_rr, P.rir_room.data(), sizeof(int) * P.rir_room.size());
-    std::memcpy(S.blob.data() + S.off_jt, T.jtab.data(), sizeof(float2) * T.jtab.size());
+    std::memcpy(S.blob.data() + S.off_jt, T.jtab.data(), sizeof(float4) * T.jtab.size());
     if (!T.lut.empty()) std::memcpy(S.blob.data() + S.off_lut, T.lut.data(), sizeof(float2) * T.lut.size());
     S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
-    // acc index = n0 - t_lo + J1 + (j - Jp0), n0 <= t_hi - 1 - J0, j - Jp0 < nJp
-    S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.nJp, 4);
-    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len + (size_t)QCAP * 5 * 4 + 16 +
-            sizeof(float2) * T.nJp;
-    if (S.prec == PREC_LUT) S.lds += sizeof(float2) * (T.lut.size() + 1);
+    // acc index = n0 - t_lo + J1 + (j - J0), n0 <= t_hi - 1 - J0, j - J0 < 64 n_pass
+    S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
+    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len + (size_t)QCAP * 5 * 4 +
+            16 + sizeof(float2) * T.lut.size();
+    GPURIR_CHECK(T.n_pass <= 12, "sampling frequency too high: at most 768 window taps are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
 }
@@ -801,7 +848,7 @@ static AccArgs acc_args(const SimLaunch& S, const void* d_plan, float* d_part, c
     a.pos_rcv = d_pos_rcv;
     a.orv_src = d_orv_src;
     a.orv_rcv = d_orv_rcv;
-    a.jtab = reinterpret_cast<const float2*>(dp + S.off_jt);
+    a.jtab = reinterpret_cast<const float4*>(dp + S.off_jt);
     a.lut = reinterpret_cast<const float2*>(dp + S.off_lut);
     a.out = d_out;
     a.part = d_part;
@@ -809,12 +856,13 @@ static AccArgs acc_args(const SimLaunch& S, const void* d_plan, float* d_part, c
     a.mic = S.opt.mic;
     a.J0 = S.T.J0;
     a.nJ = S.T.nJ;
-    a.nJp = S.T.nJp;
-    a.nb_dn = S.T.nb_dn;
-    a.nb_up = S.T.nb_up;
+    a.n_pass = S.T.n_pass;
+    a.mask_bits = S.T.mask_bits;
+    a.lut_center = S.T.lut_center;
+    a.lut_row = S.T.lut_row;
+    a.lut_k_lo = S.T.lut_k_lo;
     a.halfTw = S.T.halfTw;
     a.inv_tw2 = 2.0f / (float)S.Tw;
-    a.lut_len = (int)S.T.lut.size();
     a.ntab_max = S.ntab;
     a.acc_len = S.acc_len;
     return a;
@@ -837,16 +885,24 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     const RoomDev* d_rooms = reinterpret_cast<const RoomDev*>(dp + S.off_rooms);
     if ((stage & 1) && !S.P.items.empty()) {
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
-        const void* fn = (S.prec == PREC_LUT) ? reinterpret_cast<const void*>(&ism_accumulate_kernel<PREC_LUT>)
-                                              : reinterpret_cast<const void*>(&ism_accumulate_kernel<PREC_FP32>);
+        const dim3 grid((unsigned)S.P.items.size());
+        // register-resident window constants: the smallest instantiation that holds n_pass passes
+        const int pm = S.T.n_pass <= 2 ? 2 : (S.T.n_pass <= 6 ? 6 : 12);
+        const void* fn = nullptr;
+        const bool mk = S.T.mask_bits != 0;
+#define GPURIR_ACC_FN(M, P) \
+    (mk ? reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, true>) \
+        : reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, false>))
+        if (S.prec == PREC_LUT)
+            fn = pm == 2 ? GPURIR_ACC_FN(PREC_LUT, 2) : (pm == 6 ? GPURIR_ACC_FN(PREC_LUT, 6) : GPURIR_ACC_FN(PREC_LUT, 12));
+        else
+            fn = pm == 2 ? GPURIR_ACC_FN(PREC_FP32, 2)
+                         : (pm == 6 ? GPURIR_ACC_FN(PREC_FP32, 6) : GPURIR_ACC_FN(PREC_FP32, 12));
+#undef GPURIR_ACC_FN
         if (S.lds > 64 * 1024)
             GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds));
-        const dim3 grid((unsigned)S.P.items.size());
-        if (S.prec == PREC_LUT)
-            hipLaunchKernelGGL(ism_accumulate_kernel<PREC_LUT>, grid, dim3(WG), S.lds, stream, a);
-        else
-            hipLaunchKernelGGL(ism_accumulate_kernel<PREC_FP32>, grid, dim3(WG), S.lds, stream, a);
-        GPURIR_HIP(hipGetLastError());
+        void* args[] = {&a};
+        GPURIR_HIP(hipLaunchKernel(fn, grid, dim3(WG), args, S.lds, stream));
     }
     if (!(stage & 2)) return 0;
     FinArgs f;

@@ -286,6 +286,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
 
     // fixed-point scale: |tap| <= max|att| / (4 pi d_direct) (no image is closer than the
     // direct path; directivity gains and the windowed sinc are bounded by 1)
+    __syncthreads();  // table entries are written by other threads than the ones reading them here
     for (int e = tid; e < R.Nx + R.Ny + R.Nz; e += WG) {
         const int d = (e < R.Nx) ? 0 : (e < R.Nx + R.Ny ? 1 : 2);
         atomicMax(qcount + 1 + d, __float_as_int(fabsf(att[e])));

@@ -242,6 +242,42 @@ __device__ inline void load_orient(const float* o, float out[4]) {
 // LUT mode: u = 16 x + center = 16 j + center + floor(-16 f) + frac(-16 f), so all
 // lanes share the interpolation weight and read consecutive entries of one phase
 // row of the table.
+// All taps of one queued image; PMAX is the exact number of 64-lane passes (entry c = (acc index bits, f, B/2 | amp, B/2 cos), s = B/2 sin).
+template <int MODE, int PMAX, bool MASK>
+__device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, const float2* lut, int lane, float4 c,
+                                           float s, const float* jf, const float* cj, const float* sj,
+                                           const float* hj) {
+    long long* ap = acc + __float_as_int(c.x) + lane;
+    const float f = c.y;
+    if (MODE == PREC_LUT) {
+        const float amp = c.z;
+        const float m16 = -16.0f * f;
+        const float k0 = floorf(m16);
+        const float al = m16 - k0;
+        const int i0 = a.lut_center + (int)k0;  // >= 0
+        const int ph = i0 & 15;
+        const float2* row = lut + ph * a.lut_row + ((i0 >> 4) - a.lut_k_lo) + a.J0 + lane;
+#pragma unroll
+        for (int p = 0; p < PMAX; p++) {
+            const float2 t = row[64 * p];
+            float v = amp * fmaf(al, t.y, t.x);
+            if (MASK) v = (fabsf(jf[p] - f) < a.halfTw) ? v : 0.0f;
+            lds_add(ap + 64 * p, v);
+        }
+    } else {
+        const float Bh = c.z, BC = c.w, BS = s;
+#pragma unroll
+        for (int p = 0; p < PMAX; p++) {
+            const float x = jf[p] - f;
+            const float r = __builtin_amdgcn_rcpf(x);
+            const float w = fmaf(cj[p], BC, fmaf(sj[p], BS, hj[p] * Bh));
+            float v = w * r;
+            if (MASK) v = (fabsf(x) < a.halfTw) ? v : 0.0f;
+            lds_add(ap + 64 * p, v);
+        }
+    }
+}
+
 template <int MODE, int PMAX, bool MASK>
 __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -253,11 +289,9 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     double* offs = reinterpret_cast<double*>(smem_raw);
     long long* acc = reinterpret_cast<long long*>(offs + a.ntab_max);  // [acc_len] fixed point
     float* att = reinterpret_cast<float*>(acc + a.acc_len);
-    int* qa = reinterpret_cast<int*>(att + a.ntab_max);
-    float* qf = reinterpret_cast<float*>(qa + QCAP);
-    float* qB = qf + QCAP;
-    float* qC = qB + QCAP;
-    float* qS = qC + QCAP;
+    // queue of live images: q4[e] = (acc index bits, f, B/2 or amp, B/2 cos), qS[e] = B/2 sin
+    float4* q4 = reinterpret_cast<float4*>(att + a.ntab_max);
+    float* qS = reinterpret_cast<float*>(q4 + QCAP);
     int* qcount = reinterpret_cast<int*>(qS + QCAP);     // [0] queue length, [1..3] per-axis max |att| bits
     float2* lut = reinterpret_cast<float2*>(qcount + 4);  // LUT mode: [16 * lut_row] polyphase table
 
@@ -274,7 +308,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     float jf[PMAX], cj[PMAX], sj[PMAX], hj[PMAX];
 #pragma unroll
     for (int p = 0; p < PMAX; p++) {
-        const float4 c = (p < a.n_pass) ? a.jtab[64 * p + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 c = a.jtab[64 * p + lane];  // the table holds exactly PMAX passes
         jf[p] = c.x;
         cj[p] = c.y;
         sj[p] = c.z;
@@ -335,74 +369,43 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
                 base = __shfl(base, 0);
                 if (live) {
                     const int q = base + pre;
-                    qa[q] = n0 + acc_shift;
                     if (MODE == PREC_LUT) {
-                        qf[q] = f;
-                        qB[q] = amp * scale;
+                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), f, amp * scale, 0.0f);
                     } else {
                         // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
                         // 2^-30 samples changes no tap by more than ~1e-9 relative.
                         const float fc = fmaxf(f, 0x1p-30f);
                         const float B = amp * scale * sinpif(fc) * 0.31830988618379067f;
                         const float ph = fc * a.inv_tw2;
-                        qf[q] = fc;
-                        qB[q] = 0.5f * B;
-                        qC[q] = B * 0.5f * cospif(ph);
+                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), fc, 0.5f * B, B * 0.5f * cospif(ph));
                         qS[q] = B * 0.5f * sinpif(ph);
                     }
                 }
             }
         }
         __syncthreads();
-        // ---- phase B: every live image scatters its taps into the LDS tile.  The queue
-        // entry of the wave's next image is read before this image's atomics are issued,
+        // ---- phase B: every live image scatters its taps into the LDS tile.  A wave
+        // takes two images per iteration (more independent work between LDS waits) and
+        // reads the queue entries of its next pair before issuing this pair's atomics,
         // so the in-order LDS pipe never makes those reads wait behind the atomics.
         const int nq = __builtin_amdgcn_readfirstlane(*qcount);
         if (wave < nq) {
-            int c_a = qa[wave];
-            float c_f = qf[wave], c_B = qB[wave], c_C = qC[wave], c_S = qS[wave];
-            for (int e = wave; e < nq; e += NWAVE) {
-                const int en = (e + NWAVE < nq) ? e + NWAVE : e;
-                const int n_a = qa[en];
-                const float n_f = qf[en], n_B = qB[en], n_C = qC[en], n_S = qS[en];
-                long long* ap = acc + c_a + lane;
-                const float f = c_f;
-                if (MODE == PREC_LUT) {
-                    const float amp = c_B;
-                    const float m16 = -16.0f * f;
-                    const float k0 = floorf(m16);
-                    const float al = m16 - k0;
-                    const int i0 = a.lut_center + (int)k0;  // >= 0
-                    const int ph = i0 & 15;
-                    const float2* row = lut + ph * a.lut_row + ((i0 >> 4) - a.lut_k_lo) + a.J0 + lane;
-#pragma unroll
-                    for (int p = 0; p < PMAX; p++) {
-                        if (p < a.n_pass) {
-                            const float2 t = row[64 * p];
-                            float v = amp * fmaf(al, t.y, t.x);
-                            if (MASK) v = (fabsf(jf[p] - f) < a.halfTw) ? v : 0.0f;
-                            lds_add(ap + 64 * p, v);
-                        }
-                    }
-                } else {
-                    const float Bh = c_B, BC = c_C, BS = c_S;
-#pragma unroll
-                    for (int p = 0; p < PMAX; p++) {
-                        if (p < a.n_pass) {
-                            const float x = jf[p] - f;
-                            const float r = __builtin_amdgcn_rcpf(x);
-                            const float w = fmaf(cj[p], BC, fmaf(sj[p], BS, hj[p] * Bh));
-                            float v = w * r;
-                            if (MASK) v = (fabsf(x) < a.halfTw) ? v : 0.0f;
-                            lds_add(ap + 64 * p, v);
-                        }
-                    }
-                }
-                c_a = n_a;
-                c_f = n_f;
-                c_B = n_B;
-                c_C = n_C;
-                c_S = n_S;
+            float4 cA = q4[wave];
+            float sA = qS[wave];
+            const int eb0 = min(wave + NWAVE, nq - 1);
+            float4 cB = q4[eb0];
+            float sB = qS[eb0];
+            for (int e = wave; e < nq; e += 2 * NWAVE) {
+                const bool hasB = e + NWAVE < nq;
+                const int na = min(e + 2 * NWAVE, nq - 1), nb = min(e + 3 * NWAVE, nq - 1);
+                const float4 nA = q4[na], nB = q4[nb];
+                const float nsA = qS[na], nsB = qS[nb];
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, cA, sA, jf, cj, sj, hj);
+                if (hasB) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, cB, sB, jf, cj, sj, hj);
+                cA = nA;
+                cB = nB;
+                sA = nsA;
+                sB = nsB;
             }
         }
         __syncthreads();
@@ -633,7 +636,10 @@ TapTables make_tap_tables(int precision, int Tw) {
         T.nJ = J1 - T.J0 + 1;
     }
     const int J1 = T.J0 + T.nJ - 1;
-    T.n_pass = (T.nJ + 63) / 64;
+    // passes are instantiated for 1, 2, 3, 4, 6 and 12; other counts are padded with
+    // masked lanes up to the next one
+    const int need = (T.nJ + 63) / 64;
+    T.n_pass = need <= 4 ? need : (need <= 6 ? 6 : 12);
     // a pass needs the window test if it holds padding lanes (j > J1) or, for an odd
     // window (half-integer Tw/2, or integer Tw/2 in LUT mode with odd Tw), the end
     // offsets J0, J1 whose membership depends on f.  For an even window the end
@@ -887,18 +893,26 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     if ((stage & 1) && !S.P.items.empty()) {
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
         const dim3 grid((unsigned)S.P.items.size());
-        // register-resident window constants: the smallest instantiation that holds n_pass passes
-        const int pm = S.T.n_pass <= 2 ? 2 : (S.T.n_pass <= 6 ? 6 : 12);
         const void* fn = nullptr;
         const bool mk = S.T.mask_bits != 0;
 #define GPURIR_ACC_FN(M, P) \
     (mk ? reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, true>) \
         : reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, false>))
-        if (S.prec == PREC_LUT)
-            fn = pm == 2 ? GPURIR_ACC_FN(PREC_LUT, 2) : (pm == 6 ? GPURIR_ACC_FN(PREC_LUT, 6) : GPURIR_ACC_FN(PREC_LUT, 12));
-        else
-            fn = pm == 2 ? GPURIR_ACC_FN(PREC_FP32, 2)
-                         : (pm == 6 ? GPURIR_ACC_FN(PREC_FP32, 6) : GPURIR_ACC_FN(PREC_FP32, 12));
+#define GPURIR_ACC_PICK(M)                                         \
+    switch (S.T.n_pass) {                                          \
+        case 1: fn = GPURIR_ACC_FN(M, 1); break;                   \
+        case 2: fn = GPURIR_ACC_FN(M, 2); break;                   \
+        case 3: fn = GPURIR_ACC_FN(M, 3); break;                   \
+        case 4: fn = GPURIR_ACC_FN(M, 4); break;                   \
+        case 6: fn = GPURIR_ACC_FN(M, 6); break;                   \
+        default: fn = GPURIR_ACC_FN(M, 12); break;                 \
+    }
+        if (S.prec == PREC_LUT) {
+            GPURIR_ACC_PICK(PREC_LUT)
+        } else {
+            GPURIR_ACC_PICK(PREC_FP32)
+        }
+#undef GPURIR_ACC_PICK
 #undef GPURIR_ACC_FN
         if (S.lds > 64 * 1024)
             GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds));

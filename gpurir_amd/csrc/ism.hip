@@ -41,7 +41,7 @@ namespace {
 
 constexpr int WG = 256;
 constexpr int NWAVE = WG / 64;
-constexpr int QCAP = 1024;
+constexpr int QCAP = 512;
 
 struct AccArgs {
     const RoomDev* rooms;
@@ -50,7 +50,7 @@ struct AccArgs {
     const float* pos_rcv;
     const float* orv_src;
     const float* orv_rcv;
-    const float4* jtab;   // [n_pass*64] per lane of each pass: (jf, sigma_j cos(2 pi j/Tw), sigma_j sin(..), sigma_j)
+    const float4* jtab;   // [n_pass*64] per lane of each pass: (sigma_j j, cos(2 pi j/Tw), sin(2 pi j/Tw), sigma_j)
     const float2* lut;    // LUT mode, polyphase: [16][lut_row] (T[16k+ph], T[16k+ph+1]-T[16k+ph])
     float* out;
     float* part;
@@ -108,9 +108,39 @@ __device__ inline float pattern_gain(float c, int p) {
 // workgroup's power-of-two scale and is rounded to an int32; integer sums are
 // exact and independent of the order of the atomics.
 __device__ inline void lds_add(long long* p, float v) {
-    const long long q = (long long)__float2int_rn(v);
-    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p), (unsigned long long)q, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    int q;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(v));  // floor(v + 0.5) in one instruction
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p), (unsigned long long)(long long)q,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// sin(pi f) for f in [0, 1]: odd Taylor polynomial of sin(pi g), g = min(f, 1 - f) <= 1/2,
+// through g^13 (truncation below 2e-9 relative).
+__device__ inline float sinpi01(float f) {
+    constexpr double P = 3.14159265358979323846;
+    constexpr float c1 = (float)P, c3 = (float)(-P * P * P / 6.0), c5 = (float)(P * P * P * P * P / 120.0),
+                    c7 = (float)(-P * P * P * P * P * P * P / 5040.0),
+                    c9 = (float)(P * P * P * P * P * P * P * P * P / 362880.0),
+                    c11 = (float)(-P * P * P * P * P * P * P * P * P * P * P / 39916800.0),
+                    c13 = (float)(P * P * P * P * P * P * P * P * P * P * P * P * P / 6227020800.0);
+    const float g = fminf(f, 1.0f - f);
+    const float u = g * g;
+    float q = fmaf(u, c13, c11);
+    q = fmaf(u, q, c9);
+    q = fmaf(u, q, c7);
+    q = fmaf(u, q, c5);
+    q = fmaf(u, q, c3);
+    q = fmaf(u, q, c1);
+    return g * q;
+}
+
+// cos(pi h), sin(pi h) for 0 <= h <= 1/16 (Taylor through the 7th power; error < 1e-10).
+__device__ inline void sincospi_small(float h, float* sn, float* cs) {
+    constexpr double P = 3.14159265358979323846;
+    const float a = (float)P * h;
+    const float u = a * a;
+    *cs = fmaf(u, fmaf(u, fmaf(u, -1.0f / 720.0f, 1.0f / 24.0f), -0.5f), 1.0f);
+    *sn = a * fmaf(u, fmaf(u, fmaf(u, -1.0f / 5040.0f, 1.0f / 120.0f), -1.0f / 6.0f), 1.0f);
 }
 
 // Room constants in scalar form.  Kernels never keep a RoomDev copy: any runtime
@@ -236,9 +266,9 @@ __device__ inline void load_orient(const float* o, float out[4]) {
 // an LDS atomic hit 64 consecutive words (no bank conflicts), x = j - f is a single
 // rounding of an exact integer minus the fraction, and the per-offset window
 // constants stay in registers for the whole kernel:
-//   sigma_j w_j(f) B = sigma_j B/2 + (sigma_j cos 2pi j/Tw)(B/2 cos 2pi f/Tw)
-//                                   + (sigma_j sin 2pi j/Tw)(B/2 sin 2pi f/Tw)
-//   tap = sigma_j w_j(f) B / x,   B = amp sin(pi f)/pi,   sigma_j = (-1)^(j+1).
+//   w_j(f) B = B/2 + (cos 2pi j/Tw)(B/2 cos 2pi f/Tw) + (sin 2pi j/Tw)(B/2 sin 2pi f/Tw)
+//   tap = sigma_j w_j(f) B / (j - f) = w_j(f) B / x',   x' = sigma_j (j - f) (one FMA),
+//   B = amp sin(pi f)/pi,   sigma_j = (-1)^(j+1).
 // LUT mode: u = 16 x + center = 16 j + center + floor(-16 f) + frac(-16 f), so all
 // lanes share the interpolation weight and read consecutive entries of one phase
 // row of the table.
@@ -265,14 +295,15 @@ __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, con
             lds_add(ap + 64 * p, v);
         }
     } else {
-        const float Bh = c.z, BC = c.w, BS = s;
+        // sigma_j folded into the offset: x' = sigma_j (j - f), tap = B w_j / x'
+        const float H = c.z, BC = c.w, BS = s;
 #pragma unroll
         for (int p = 0; p < PMAX; p++) {
-            const float x = jf[p] - f;
-            const float r = __builtin_amdgcn_rcpf(x);
-            const float w = fmaf(cj[p], BC, fmaf(sj[p], BS, hj[p] * Bh));
+            const float xs = fmaf(-hj[p], f, jf[p]);
+            const float r = __builtin_amdgcn_rcpf(xs);
+            const float w = fmaf(cj[p], BC, fmaf(sj[p], BS, H));
             float v = w * r;
-            if (MASK) v = (fabsf(x) < a.halfTw) ? v : 0.0f;
+            if (MASK) v = (fabsf(xs) < a.halfTw) ? v : 0.0f;
             lds_add(ap + 64 * p, v);
         }
     }
@@ -375,10 +406,17 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
                         // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
                         // 2^-30 samples changes no tap by more than ~1e-9 relative.
                         const float fc = fmaxf(f, 0x1p-30f);
-                        const float B = amp * scale * sinpif(fc) * 0.31830988618379067f;
-                        const float ph = fc * a.inv_tw2;
-                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), fc, 0.5f * B, B * 0.5f * cospif(ph));
-                        qS[q] = B * 0.5f * sinpif(ph);
+                        const float B = amp * scale * sinpi01(fc) * 0.31830988618379067f;
+                        const float ph = fc * a.inv_tw2;  // phi / pi = 2 f / Tw
+                        float sph, cph;
+                        if (a.inv_tw2 <= 0.0625f) {
+                            sincospi_small(ph, &sph, &cph);
+                        } else {
+                            sph = sinpif(ph);
+                            cph = cospif(ph);
+                        }
+                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), fc, 0.5f * B, 0.5f * B * cph);
+                        qS[q] = 0.5f * B * sph;
                     }
                 }
             }
@@ -611,7 +649,7 @@ struct TapTables {
     int n_pass = 0;            // ceil(nJ / 64)
     unsigned mask_bits = 0;    // passes that need the |x| < halfTw test
     float halfTw = 0.0f;
-    std::vector<float4> jtab;  // [n_pass * 64]: (jf, sigma cos, sigma sin, sigma)
+    std::vector<float4> jtab;  // [n_pass * 64]: (sigma j, cos, sin, sigma)
     int lut_center = 0, lut_row = 0, lut_k_lo = 0;
     std::vector<float2> lut;   // polyphase [16][lut_row]
 };
@@ -656,7 +694,7 @@ TapTables make_tap_tables(int precision, int Tw) {
         const int j = T.J0 + u;
         const double sg = (j & 1) ? 1.0 : -1.0;  // (-1)^(j+1)
         const double th = 2.0 * M_PI * (double)j / (double)Tw;
-        T.jtab[u] = make_float4((float)j, (float)(sg * std::cos(th)), (float)(sg * std::sin(th)), (float)sg);
+        T.jtab[u] = make_float4((float)(sg * j), (float)std::cos(th), (float)std::sin(th), (float)sg);
     }
     if (lut) {
         int len = Tw * 16;
@@ -697,13 +735,35 @@ struct Plan {
     int tt_max = 0;
 };
 
-constexpr int TILE_DEFAULT = 4096;
 constexpr int MIN_CHUNK = 512;
+constexpr int MAX_CHUNK = 65536;      // images per work item at most (splits the largest rooms)
 constexpr int TARGET_ITEMS = 2048;
+constexpr size_t LDS_PER_WG = 53 * 1024;  // three workgroups per CU (160 KiB of LDS)
+constexpr int MIN_TILE = 512;
 
-Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt) {
+// LDS bytes of one accumulation workgroup for a tile of tt samples
+size_t acc_lds_bytes(int tt, int ntab, const TapTables& T) {
+    const size_t nt = (size_t)(ntab + 3) / 4 * 4;
+    const size_t acc = ((size_t)tt + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64 + 3) / 4 * 4;
+    return 12 * nt + 8 * acc + (size_t)QCAP * 20 + 16 + sizeof(float2) * T.lut.size();
+}
+
+Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt) {
     Plan P;
-    const int tile_cap = opt.tile_samples > 0 ? opt.tile_samples : TILE_DEFAULT;
+    int ntab = 0, nism_max = 0;
+    for (const RoomHost& h : rh) {
+        ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
+        nism_max = std::max(nism_max, n_samples_rule(h.Tdiff, opt.fs));
+    }
+    // default tile: the whole early part if a workgroup then still fits three per CU
+    int tile_cap = opt.tile_samples;
+    if (tile_cap <= 0) {
+        tile_cap = std::max(nism_max, 1);
+        while (tile_cap > MIN_TILE && acc_lds_bytes(tile_cap, ntab, Tt) > LDS_PER_WG) {
+            const int ntiles = (nism_max + tile_cap - 1) / tile_cap + 1;
+            tile_cap = std::max(MIN_TILE, (nism_max + ntiles - 1) / ntiles);
+        }
+    }
     double total = 0.0;
     long long total_rirs = 0;
     for (const RoomHost& h : rh) {
@@ -714,8 +774,8 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt) {
         total_rirs += (long long)h.n_src * h.n_rcv;
     }
     long long per_item = (long long)std::ceil(total / (double)TARGET_ITEMS);
-    if (opt.chunk_images > 0) per_item = opt.chunk_images;
-    per_item = std::max<long long>(per_item, MIN_CHUNK);
+    per_item = std::min<long long>(std::max<long long>(per_item, MIN_CHUNK), MAX_CHUNK);
+    if (opt.chunk_images > 0) per_item = std::max(opt.chunk_images, 1);
     long long rir_off = 0;
     for (size_t k = 0; k < rh.size(); k++) {
         const RoomHost& h = rh[k];
@@ -743,7 +803,7 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt) {
         const int M = h.n_src * h.n_rcv;
         const int TT = std::min(R.nISM, tile_cap);
         const int tiles = R.nISM > 0 ? (R.nISM + TT - 1) / TT : 0;
-        int C = (int)std::min<long long>((N + per_item - 1) / per_item, (N + MIN_CHUNK - 1) / MIN_CHUNK);
+        int C = (int)((N + per_item - 1) / per_item);
         C = std::max(C, 1);
         R.n_chunks = C;
         if (C > 1) {
@@ -816,7 +876,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     GPURIR_CHECK(S.Tw >= 2, "sampling frequency too low for the 8 ms window");
     S.prec = opt.precision;
     S.T = make_tap_tables(S.prec, S.Tw);
-    S.P = make_plan(rooms, opt);
+    S.P = make_plan(rooms, opt, S.T);
     const Plan& P = S.P;
     const TapTables& T = S.T;
     S.off_rooms = 0;
@@ -973,7 +1033,7 @@ int launch_image_list(const RoomHost& room, const float* d_pos_src, const float*
     GPURIR_CHECK(room.nb[0] > 0 && room.nb[1] > 0 && room.nb[2] > 0, "nb_img must be positive in every dimension");
     GPURIR_CHECK(room.nb[0] + room.nb[1] + room.nb[2] <= 4096, "too many images along the room axes");
     std::vector<RoomHost> rh(1, room);
-    Plan P = make_plan(rh, opt);
+    Plan P = make_plan(rh, opt, make_tap_tables(PREC_FP32, std::max(tw_rule(opt.fs), 2)));
     void* dplan = nullptr;
     if (int rc = workspace(WS_PLAN, sizeof(RoomDev), &dplan)) return rc;
     if (int rc = upload_async(dplan, P.rooms.data(), sizeof(RoomDev), stream)) return rc;

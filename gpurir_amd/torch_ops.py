@@ -150,14 +150,16 @@ class _BatchArgs:
 
 
 def simulate_batch(rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
-                   noise="logistic", seed=1234, call_offset=0, room_index=0, device=None):
+                   noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
+                   device=None):
     """rooms: list of dicts with room_sz, beta, pos_src, pos_rcv, nb_img, Tmax [, Tdiff, orV_src, orV_rcv, level].
 
     Returns (flat float32 output on the GPU, list of per-room [n_src, n_rcv, nSamples] views)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     B = _BatchArgs(rooms, fs, dev)
     out = torch.empty(B.total, dtype=torch.float32, device=dev)
-    o = options(precision, noise, seed, call_offset, room_index)
+    o = options(precision, noise, seed, call_offset, room_index, chunk_images=chunk_images,
+                tile_samples=tile_samples)
     _lib.check(_lib.load().gpurir_simulate_batch_device(
         *B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),
         out.data_ptr(), B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _stream(dev)))
@@ -168,13 +170,15 @@ class BatchPlan:
     """A batch of rooms planned once; run() replays the captured hipGraph."""
 
     def __init__(self, rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
-                 noise="logistic", seed=1234, call_offset=0, room_index=0, device=None):
+                 noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
+                 device=None):
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dev = dev
         self.B = _BatchArgs(rooms, fs, dev)
         self.out = torch.empty(self.B.total, dtype=torch.float32, device=dev)
         self.rirs = self.B.split(self.out)
-        o = options(precision, noise, seed, call_offset, room_index)
+        o = options(precision, noise, seed, call_offset, room_index, chunk_images=chunk_images,
+                    tile_samples=tile_samples)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().gpurir_batch_plan_create(
             *self.B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),

@@ -42,6 +42,7 @@ namespace {
 constexpr int WG = 256;
 constexpr int NWAVE = WG / 64;
 constexpr int QCAP = 512;
+constexpr int QPAD = 16;  // queue entries read ahead past the live part (4 per wave)
 
 struct AccArgs {
     const RoomDev* rooms;
@@ -71,7 +72,6 @@ struct FinArgs {
     const float* pos_src;
     const float* pos_rcv;
     float* out;
-    const float* part;
     float w_sz;           // float(10e-3f * fs), src/gpuRIR_cuda.cu:421
     double fs;
     unsigned long long seed;
@@ -322,8 +322,8 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     float* att = reinterpret_cast<float*>(acc + a.acc_len);
     // queue of live images: q4[e] = (acc index bits, f, B/2 or amp, B/2 cos), qS[e] = B/2 sin
     float4* q4 = reinterpret_cast<float4*>(att + a.ntab_max);
-    float* qS = reinterpret_cast<float*>(q4 + QCAP);
-    int* qcount = reinterpret_cast<int*>(qS + QCAP);     // [0] queue length, [1..3] per-axis max |att| bits
+    float* qS = reinterpret_cast<float*>(q4 + QCAP + QPAD);
+    int* qcount = reinterpret_cast<int*>(qS + QCAP + QPAD);     // [0] queue length, [1..3] per-axis max |att| bits
     float2* lut = reinterpret_cast<float2*>(qcount + 4);  // LUT mode: [16 * lut_row] polyphase table
 
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
@@ -423,27 +423,37 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         }
         __syncthreads();
         // ---- phase B: every live image scatters its taps into the LDS tile.  A wave
-        // takes two images per iteration (more independent work between LDS waits) and
+        // takes two images per step (more independent work between LDS waits) and
         // reads the queue entries of its next pair before issuing this pair's atomics,
         // so the in-order LDS pipe never makes those reads wait behind the atomics.
+        // The two register sets X and Y alternate (no copies), and the queue is read
+        // through per-lane pointers with immediate offsets; reads past the live part
+        // stay inside the queue's padding and their values are never used.
         const int nq = __builtin_amdgcn_readfirstlane(*qcount);
         if (wave < nq) {
-            float4 cA = q4[wave];
-            float sA = qS[wave];
-            const int eb0 = min(wave + NWAVE, nq - 1);
-            float4 cB = q4[eb0];
-            float sB = qS[eb0];
-            for (int e = wave; e < nq; e += 2 * NWAVE) {
-                const bool hasB = e + NWAVE < nq;
-                const int na = min(e + 2 * NWAVE, nq - 1), nb = min(e + 3 * NWAVE, nq - 1);
-                const float4 nA = q4[na], nB = q4[nb];
-                const float nsA = qS[na], nsB = qS[nb];
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, cA, sA, jf, cj, sj, hj);
-                if (hasB) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, cB, sB, jf, cj, sj, hj);
-                cA = nA;
-                cB = nB;
-                sA = nsA;
-                sB = nsB;
+            const float4* pq = q4 + (tid >> 6);
+            const float* pS = qS + (tid >> 6);
+            float4 xA = pq[0], xB = pq[NWAVE];
+            float sxA = pS[0], sxB = pS[NWAVE];
+            for (int e = wave;;) {
+                const float4 yA = pq[2 * NWAVE], yB = pq[3 * NWAVE];
+                const float syA = pS[2 * NWAVE], syB = pS[3 * NWAVE];
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xA, sxA, jf, cj, sj, hj);
+                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xB, sxB, jf, cj, sj, hj);
+                e += 2 * NWAVE;
+                if (e >= nq) break;
+                pq += 2 * NWAVE;
+                pS += 2 * NWAVE;
+                xA = pq[2 * NWAVE];
+                xB = pq[3 * NWAVE];
+                sxA = pS[2 * NWAVE];
+                sxB = pS[3 * NWAVE];
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yA, syA, jf, cj, sj, hj);
+                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yB, syB, jf, cj, sj, hj);
+                e += 2 * NWAVE;
+                if (e >= nq) break;
+                pq += 2 * NWAVE;
+                pS += 2 * NWAVE;
             }
         }
         __syncthreads();
@@ -494,6 +504,38 @@ __device__ inline double block_sum(double v, double* red) {
 
 // One workgroup per RIR: fixed-order sum of the partial rows (if any), the
 // envelope of src/gpuRIR_cuda.cu:415-443 and the tail of :445-461.
+// Sum of the partial rows of RIRs whose images were split into several chunks.
+// One workgroup covers 64 samples of one RIR: thread (g, l) adds chunks g, g+4,
+// g+8, ... of sample l with eight loads in flight, and the four group sums are
+// combined in a fixed order, so every run gives the same bits.
+__global__ __launch_bounds__(WG) void partial_sum_kernel(const RoomDev* rooms, const int* rir_room,
+                                                         const int2* blocks, const float* part, float* out) {
+    __shared__ float red[4][64];
+    const int2 b = blocks[blockIdx.x];
+    const RoomDev* Rp = rooms + rir_room[b.x];
+    const int C = Rp->n_chunks, nISM = Rp->nISM, nS = Rp->nSamples;
+    const int m = b.x - Rp->rir_off;
+    const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int t = b.y + l;
+    float acc = 0.0f;
+    if (t < nISM) {
+        const float* pp = part + Rp->part_off + (long long)m * C * nISM + t;
+        int c = g;
+        for (; c + 28 < C; c += 32) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = pp[(long long)(c + 4 * u) * nISM];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc += v[u];
+        }
+        for (; c < C; c += 4) acc += pp[(long long)c * nISM];
+    }
+    red[g][l] = acc;
+    __syncthreads();
+    if (g == 0 && t < nISM)
+        out[Rp->out_off + (long long)m * nS + t] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+}
+
 __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     __shared__ double red[NWAVE];
     const int g = blockIdx.x;
@@ -515,20 +557,12 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     const double tau_dp = sqrt(d2) * R.fs_over_c;
     const int w_start = (int)ceil(fmax((double)nISM - (double)a.w_sz, tau_dp));
 
+    // the early part is complete here: single-chunk items wrote it directly and
+    // partial_sum_kernel added up the partial rows of multi-chunk RIRs
     double pw = 0.0;
-    if (R.n_chunks > 1) {
-        const float* pp = a.part + R.part_off + (long long)m * R.n_chunks * nISM;
-        for (int t = tid; t < nISM; t += WG) {
-            float acc = 0.0f;
-            for (int c = 0; c < R.n_chunks; c++) acc += pp[(long long)c * nISM + t];
-            o[t] = acc;
-            if (t >= w_start) pw += (double)acc * (double)acc;
-        }
-    } else {
-        for (int t = w_start + tid; t < nISM; t += WG) {
-            const double x = o[t];
-            pw += x * x;
-        }
+    for (int t = w_start + tid; t < nISM; t += WG) {
+        const double x = o[t];
+        pw += x * x;
     }
     if (nDiff <= 0) return;
     const double P = block_sum(pw, red) / (double)(nISM - w_start);
@@ -730,6 +764,7 @@ struct Plan {
     std::vector<RoomDev> rooms;
     std::vector<Item> items;
     std::vector<int> rir_room;
+    std::vector<int2> sum_blocks;  // (global RIR, first sample) of 64-sample partial-sum blocks
     long long part_elems = 0;
     int ntab_max = 0;
     int tt_max = 0;
@@ -745,7 +780,7 @@ constexpr int MIN_TILE = 512;
 size_t acc_lds_bytes(int tt, int ntab, const TapTables& T) {
     const size_t nt = (size_t)(ntab + 3) / 4 * 4;
     const size_t acc = ((size_t)tt + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64 + 3) / 4 * 4;
-    return 12 * nt + 8 * acc + (size_t)QCAP * 20 + 16 + sizeof(float2) * T.lut.size();
+    return 12 * nt + 8 * acc + (size_t)(QCAP + QPAD) * 20 + 16 + sizeof(float2) * T.lut.size();
 }
 
 Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt) {
@@ -814,6 +849,8 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const Tap
         P.tt_max = std::max(P.tt_max, TT);
         const int csz = (N + C - 1) / C;
         for (int m = 0; m < M; m++) {
+            if (C > 1)
+                for (int t0 = 0; t0 < R.nISM; t0 += 64) P.sum_blocks.push_back(make_int2((int)(rir_off + m), t0));
             P.rir_room.push_back((int)k);
             for (int c = 0; c < C; c++)
                 for (int tl = 0; tl < tiles; tl++) {
@@ -852,7 +889,7 @@ struct SimLaunch {
     TapTables T;
     int prec = PREC_FP32;
     int Tw = 0;
-    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_jt = 0, off_lut = 0, plan_bytes = 0;
+    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_jt = 0, off_lut = 0, off_sb = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
     size_t lds = 0;
     int acc_len = 0, ntab = 0;
@@ -884,18 +921,21 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.off_rr = align_up(S.off_items + sizeof(Item) * P.items.size(), 256);
     S.off_jt = align_up(S.off_rr + sizeof(int) * P.rir_room.size(), 256);
     S.off_lut = align_up(S.off_jt + sizeof(float4) * T.jtab.size(), 256);
-    S.plan_bytes = align_up(S.off_lut + sizeof(float2) * (T.lut.size() + 1), 256);
+    S.off_sb = align_up(S.off_lut + sizeof(float2) * (T.lut.size() + 1), 256);
+    S.plan_bytes = align_up(S.off_sb + sizeof(int2) * (P.sum_blocks.size() + 1), 256);
     S.blob.assign(S.plan_bytes, 0);
     std::memcpy(S.blob.data() + S.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(S.blob.data() + S.off_items, P.items.data(), sizeof(Item) * P.items.size());
     std::memcpy(S.blob.data() + S.off_rr, P.rir_room.data(), sizeof(int) * P.rir_room.size());
     std::memcpy(S.blob.data() + S.off_jt, T.jtab.data(), sizeof(float4) * T.jtab.size());
     if (!T.lut.empty()) std::memcpy(S.blob.data() + S.off_lut, T.lut.data(), sizeof(float2) * T.lut.size());
+    if (!P.sum_blocks.empty())
+        std::memcpy(S.blob.data() + S.off_sb, P.sum_blocks.data(), sizeof(int2) * P.sum_blocks.size());
     S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
     // acc index = n0 - t_lo + J1 + (j - J0), n0 <= t_hi - 1 - J0, j - J0 < 64 n_pass
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
-    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len + (size_t)QCAP * 5 * 4 +
-            16 + sizeof(float2) * T.lut.size();
+    S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
+            (size_t)(QCAP + QPAD) * 5 * 4 + 16 + sizeof(float2) * T.lut.size();
     GPURIR_CHECK(T.n_pass <= 12, "sampling frequency too high: at most 768 window taps are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -980,13 +1020,18 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         GPURIR_HIP(hipLaunchKernel(fn, grid, dim3(WG), args, S.lds, stream));
     }
     if (!(stage & 2)) return 0;
+    if (!S.P.sum_blocks.empty()) {
+        hipLaunchKernelGGL(partial_sum_kernel, dim3((unsigned)S.P.sum_blocks.size()), dim3(WG), 0, stream, d_rooms,
+                           reinterpret_cast<const int*>(dp + S.off_rr), reinterpret_cast<const int2*>(dp + S.off_sb),
+                           d_part, d_out);
+        GPURIR_HIP(hipGetLastError());
+    }
     FinArgs f;
     f.rooms = d_rooms;
     f.rir_room = reinterpret_cast<const int*>(dp + S.off_rr);
     f.pos_src = d_pos_src;
     f.pos_rcv = d_pos_rcv;
     f.out = d_out;
-    f.part = d_part;
     f.w_sz = 10e-3f * S.opt.fs;
     f.fs = (double)S.opt.fs;
     f.seed = S.opt.seed;

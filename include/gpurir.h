@@ -58,13 +58,13 @@ int gpurir_version(void);
 const char* gpurir_last_error(void);
 
 /* Sample count rule nSamples = ceil(f32(T) * f32(fs)) rounded up to even:
- * src/gpuRIR_cuda.cu:831-835, src/python_bind.cpp:119-120. */
+ * src/gpuRIR_cuda.cu:831-835, src/python_bind.cpp:82-83. */
 int gpurir_n_samples(float T, float fs);
 
 /* Toggles of the drop-in state, mutually exclusive as in the reference; return the new
  * state of the flag.  Replace gpuRIR_cuda::activate_mixed_precision / activate_lut
  * (src/gpuRIR_cuda.cu:1050-1073) bound as activate_mixed_precision_bind /
- * activate_lut_bind (src/python_bind.cpp:158-164).  Mixed precision is not provided
+ * activate_lut_bind (src/python_bind.cpp:121-128,140-142).  Mixed precision is not provided
  * by this library: activating it prints the reference's conflict message (if the
  * table was active), disables the table and reports 0. */
 int gpurir_activate_mixed_precision(int activate);
@@ -75,7 +75,7 @@ int gpurir_activate_lut(int activate);
  * every plain gpurir_simulate_rir call uses call_offset = the number of previous calls. */
 int gpurir_set_seed(unsigned long long seed, unsigned int next_call_offset);
 
-/* simulateRIR: replaces gpuRIR_bind::simulateRIR_bind (src/python_bind.cpp:66-125)
+/* simulateRIR: replaces gpuRIR_bind::simulateRIR_bind (src/python_bind.cpp:29-88)
  * -> gpuRIR_cuda::cuda_simulateRIR (src/gpuRIR_cuda.cu:766-931).  Host arrays, C order:
  * pos_src/orV_src [M_src][3], pos_rcv/orV_rcv [M_rcv][3], out [M_src][M_rcv][nSamples]
  * with nSamples = gpurir_n_samples(Tmax, fs).  Uses the drop-in precision and noise state. */
@@ -136,7 +136,7 @@ int gpurir_batch_plan_count_taps(gpurir_plan_t plan, long long* taps, void* stre
  * counterpart (measurement support). */
 int gpurir_batch_plan_profile(gpurir_plan_t plan, void* stream, float* ms_accumulate, float* ms_finalize);
 
-/* gpu_conv: replaces gpuRIR_bind::gpu_conv (src/python_bind.cpp:127-156) ->
+/* gpu_conv: replaces gpuRIR_bind::gpu_conv (src/python_bind.cpp:90-119) ->
  * gpuRIR_cuda::cuda_convolutions (src/gpuRIR_cuda.cu:933-1023).  Host arrays:
  * segments [M_src][seg_len], rirs [M_src][M_rcv][rir_len],
  * out [M_src][M_rcv][seg_len + rir_len - 1] (linear convolutions). */

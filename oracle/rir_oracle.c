@@ -49,7 +49,7 @@
 static const double ORC_PI = 3.14159265358979323846;
 
 /* ------------------------------------------------------------------------ */
-/* Sample-count rules: gpuRIR_cuda.cu:831-835 and python_bind.cpp:119-120.  */
+/* Sample-count rules: gpuRIR_cuda.cu:831-835 and python_bind.cpp:82-83.  */
 /* Tmax and Fs arrive as C floats; their product is a float.                 */
 /* ------------------------------------------------------------------------ */
 int orc_n_samples(float T, float fs) {

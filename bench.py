@@ -124,6 +124,35 @@ def main():
                           "ms_per_step": ms_step, "kernel_ms": ms_acc, "taps": taps}), flush=True)
         return
 
+    # ---------------- gradients: 256 rooms, MSE against targets simulated from perturbed
+    # betas and source positions; one step = forward (graph) + dL/dout + adjoint kernels
+    c4 = W.gradient_rooms(256, seed=0)
+    target = T.BatchPlan(W.gradient_targets(c4, seed=1), fs, seed=1234).run().clone()
+    gplan = T.BatchPlan(c4, fs, seed=1234)
+
+    def grad_step():
+        y = gplan.run()
+        return gplan.adjoint((2.0 / y.numel()) * (y - target))
+
+    for _ in range(max(args.warmup, 1)):
+        grads = grad_step()
+    torch.cuda.synchronize()
+    n_grad = max(args.steps // 2, 3)
+    t0 = time.perf_counter()
+    for _ in range(n_grad):
+        grads = grad_step()
+    torch.cuda.synchronize()
+    ms_grad = (time.perf_counter() - t0) * 1e3 / n_grad
+    t0 = time.perf_counter()
+    for _ in range(n_grad):
+        gplan.run()
+    torch.cuda.synchronize()
+    ms_fwd4 = (time.perf_counter() - t0) * 1e3 / n_grad
+    grads_per_s = len(c4) / (ms_grad / 1e3)
+    gfinite = all(bool(torch.isfinite(gk).all()) for gk in grads)
+    log(f"[bench] gradients: {ms_grad:.3f} ms per fwd+bwd step of {len(c4)} rooms ({grads_per_s:.4e} rooms/s), "
+        f"forward alone {ms_fwd4:.3f} ms, finite={gfinite}")
+
     # ---------------- plain drop-in: one room, numpy in / numpy out, FP32 exact sinc
     c1 = W.plain_room()
     gpuRIR.activateLUT(False)
@@ -228,6 +257,10 @@ def main():
         "mid_room_ms": ms_mid,
         "finalize_ms": ms_fin,
         "trajectory": {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj},
+        "gradients": {"workload": "256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 src x 4 rcv, Tmax 0.5 s, "
+                                  "Tdiff at 20 dB), MSE to targets from perturbed betas and sources; "
+                                  "d/d(6 betas, source xyz, receiver xyz, level)",
+                      "grads_per_s": grads_per_s, "ms_per_fwd_bwd": ms_grad, "ms_fwd": ms_fwd4},
     }
     print(json.dumps(line), flush=True)
 

@@ -62,6 +62,16 @@ _SIG = {
     "gpurir_batch_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "gpurir_batch_plan_count_taps": (ctypes.c_int, [ctypes.c_void_p, c_ll_p, ctypes.c_void_p]),
     "gpurir_batch_plan_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_float_p, c_float_p]),
+    "gpurir_simulate_batch_adjoint_device": (ctypes.c_int, [ctypes.c_int, c_float_p, c_float_p, c_int_p, c_int_p,
+                                                            c_int_p, c_float_p, c_float_p, c_float_p,
+                                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                            ctypes.c_float, ctypes.c_float, ctypes.POINTER(Options),
+                                                            ctypes.c_void_p, ctypes.c_void_p, c_ll_p,
+                                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                            ctypes.c_void_p, ctypes.c_void_p]),
+    "gpurir_batch_plan_adjoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gpurir_conv": (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, ctypes.c_int, ctypes.c_int,
                                    c_float_p]),
     "gpurir_trajectory_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,

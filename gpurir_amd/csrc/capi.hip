@@ -172,6 +172,12 @@ struct gpurir_plan {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // adjoint, prepared on first use
+    std::vector<RoomHost> rooms;
+    SimOptions so{};
+    AdjLaunch* adj = nullptr;
+    void* d_adj_plan = nullptr;
+    void* d_adj_ws = nullptr;
 };
 
 extern "C" {
@@ -389,6 +395,8 @@ int gpurir_batch_plan_create(int n_rooms, const float* room_sz, const float* bet
     p->orv_rcv = d_orV_rcv;
     p->out = d_out;
     p->out_elems = elems;
+    p->rooms = rooms;
+    p->so = so;
     *plan = p;
     return 0;
 }
@@ -457,8 +465,65 @@ int gpurir_batch_plan_profile(gpurir_plan_t p, void* stream, float* ms_accumulat
 long long gpurir_batch_plan_out_elems(gpurir_plan_t p) { return p ? p->out_elems : -1; }
 long long gpurir_batch_plan_n_rirs(gpurir_plan_t p) { return p ? sim_n_rirs(*p->sim) : -1; }
 
+int gpurir_batch_plan_adjoint(gpurir_plan_t p, const float* d_grad_out, float* d_grad_beta, float* d_grad_src,
+                              float* d_grad_rcv, float* d_grad_level, void* stream) {
+    GPURIR_CHECK(p != nullptr && d_grad_out != nullptr, "null argument");
+    if (!p->adj) {
+        AdjLaunch* a = adj_new();
+        if (int rc = adj_prepare(p->rooms, p->so, *a)) {
+            adj_delete(a);
+            return rc;
+        }
+        void *dp = nullptr, *dw = nullptr;
+        if (hipMalloc(&dp, adj_plan_bytes(*a)) != hipSuccess || hipMalloc(&dw, adj_ws_bytes(*a)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (dp) (void)hipFree(dp);
+            adj_delete(a);
+            set_error("out of device memory for the adjoint plan");
+            return -3;
+        }
+        if (hipMemcpy(dp, adj_blob(*a), adj_plan_bytes(*a), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(dp);
+            (void)hipFree(dw);
+            adj_delete(a);
+            set_error("adjoint plan upload failed");
+            return -2;
+        }
+        p->adj = a;
+        p->d_adj_plan = dp;
+        p->d_adj_ws = dw;
+    }
+    return adj_enqueue(*p->adj, p->d_adj_plan, p->d_adj_ws, p->pos_src, p->pos_rcv, p->orv_src, p->orv_rcv, p->out,
+                       d_grad_out, d_grad_beta, d_grad_src, d_grad_rcv, d_grad_level, static_cast<hipStream_t>(stream));
+}
+
+int gpurir_simulate_batch_adjoint_device(int n_rooms, const float* room_sz, const float* beta, const int* nb_img,
+                                         const int* n_src, const int* n_rcv, const float* Tdiff, const float* Tmax,
+                                         const float* level, const float* d_pos_src, const float* d_pos_rcv,
+                                         const float* d_orV_src, const float* d_orV_rcv, int spkr_pattern,
+                                         int mic_pattern, float fs, float c, const gpurir_options_t* opt,
+                                         const float* d_out, const float* d_grad_out, const long long* out_offsets,
+                                         float* d_grad_beta, float* d_grad_src, float* d_grad_rcv,
+                                         float* d_grad_level, void* stream) {
+    GPURIR_CHECK(spkr_pattern >= 0 && spkr_pattern <= 5 && mic_pattern >= 0 && mic_pattern <= 5, "invalid polar pattern");
+    GPURIR_CHECK(d_pos_src && d_pos_rcv && d_out && d_grad_out, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || d_orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || d_orV_rcv, "the mics are not omni but their orientation is undefined");
+    std::vector<RoomHost> rooms;
+    long long elems = 0;
+    if (int rc = batch_rooms(n_rooms, room_sz, beta, nb_img, n_src, n_rcv, Tdiff, Tmax, level, fs, opt, out_offsets,
+                             rooms, &elems))
+        return rc;
+    SimOptions so = sim_options(opt, spkr_pattern, mic_pattern, fs, c);
+    return launch_adjoint(rooms, d_pos_src, d_pos_rcv, d_orV_src, d_orV_rcv, so, d_out, d_grad_out, d_grad_beta,
+                          d_grad_src, d_grad_rcv, d_grad_level, static_cast<hipStream_t>(stream));
+}
+
 int gpurir_batch_plan_destroy(gpurir_plan_t p) {
     if (!p) return 0;
+    if (p->d_adj_plan) (void)hipFree(p->d_adj_plan);
+    if (p->d_adj_ws) (void)hipFree(p->d_adj_ws);
+    if (p->adj) adj_delete(p->adj);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);

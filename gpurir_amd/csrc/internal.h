@@ -34,7 +34,7 @@ const char* last_error();
 // ---- device workspace: one grow-only buffer per (device, slot).
 //      Slots keep concurrent users (e.g. an ISM call's partials and its
 //      staging copies) apart.
-enum WsSlot { WS_PLAN = 0, WS_PARTIAL, WS_STAGE, WS_CONV, WS_GRAD, WS_COUNT };
+enum WsSlot { WS_PLAN = 0, WS_PARTIAL, WS_STAGE, WS_CONV, WS_GRAD, WS_ADJ_PLAN, WS_COUNT };
 int workspace(int slot, size_t bytes, void** ptr);
 
 // Host-to-device copy through a pinned staging buffer; the host source may be
@@ -115,6 +115,22 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
                       hipStream_t stream);
 int sim_count_taps(const SimLaunch& S, const void* d_plan, const float* d_pos_src, const float* d_pos_rcv,
                    unsigned long long* d_counter, hipStream_t stream);
+// Adjoint of a batch: gradients of L with respect to beta [n_rooms][6], source and
+// receiver positions (rows as the position arrays) and level [n_rooms], given the
+// forward output d_y and dL/dout d_gy.  Any gradient pointer may be null.
+struct AdjLaunch;
+AdjLaunch* adj_new();
+void adj_delete(AdjLaunch* a);
+int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, AdjLaunch& A);
+size_t adj_plan_bytes(const AdjLaunch& A);
+const void* adj_blob(const AdjLaunch& A);
+size_t adj_ws_bytes(const AdjLaunch& A);
+int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float* d_pos_src, const float* d_pos_rcv,
+                const float* d_orv_src, const float* d_orv_rcv, const float* d_y, const float* d_gy, float* g_beta,
+                float* g_src, float* g_rcv, float* g_level, hipStream_t stream);
+int launch_adjoint(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
+                   const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, const float* d_y,
+                   const float* d_gy, float* g_beta, float* g_src, float* g_rcv, float* g_level, hipStream_t stream);
 int launch_image_list(const RoomHost& room, const float* d_pos_src, const float* d_pos_rcv, const float* d_orv_src,
                       const float* d_orv_rcv, const SimOptions& opt, int* d_n0, float* d_frac, float* d_amp,
                       hipStream_t stream);

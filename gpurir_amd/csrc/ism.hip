@@ -492,9 +492,27 @@ __device__ inline void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsi
     out[3] = c3;
 }
 
+// Noise value u (0..3) of one Philox block.  Logistic (the reference's
+// 0.551329 log(u / (1 - u + 1e-6)), src/gpuRIR_cuda.cu:454-455) with
+// u = (x+1)/2^32 and 1-u = (2^32-1-x)/2^32 as exact integers, or Box-Muller.
+__device__ inline float tail_noise(const unsigned w[4], int u, int noise) {
+    if (noise == 0) {
+        const float num = (float)w[u] + 1.0f;
+        const float den = (float)(0xFFFFFFFFu - w[u]) + 4294.967296f;
+        return 0.551329f * logf(num / den);
+    }
+    const int pr2 = u >> 1;
+    const float lnua = log1pf(-(float)(0xFFFFFFFFu - w[2 * pr2]) * 0x1p-32f);
+    const float ub = (float)w[2 * pr2 + 1] * 0x1p-32f;
+    const float rad = sqrtf(-2.0f * lnua);
+    return (u & 1) ? rad * sinpif(2.0f * ub) : rad * cospif(2.0f * ub);
+}
+
+// Sum over the workgroup (every thread gets it); red may be reused right after.
 __device__ inline double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();  // earlier readers of red are done
     if (lane == 0) red[wave] = v;
     __syncthreads();
     double t = 0.0;
@@ -502,8 +520,6 @@ __device__ inline double block_sum(double v, double* red) {
     return t;
 }
 
-// One workgroup per RIR: fixed-order sum of the partial rows (if any), the
-// envelope of src/gpuRIR_cuda.cu:415-443 and the tail of :445-461.
 // Sum of the partial rows of RIRs whose images were split into several chunks.
 // One workgroup covers 64 samples of one RIR: thread (g, l) adds chunks g, g+4,
 // g+8, ... of sample l with eight loads in flight, and the four group sums are
@@ -536,6 +552,7 @@ __global__ __launch_bounds__(WG) void partial_sum_kernel(const RoomDev* rooms, c
         out[Rp->out_off + (long long)m * nS + t] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
 }
 
+// One workgroup per RIR: the envelope of src/gpuRIR_cuda.cu:415-443 and the tail of :445-461.
 __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     __shared__ double red[NWAVE];
     const int g = blockIdx.x;
@@ -591,19 +608,7 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
         for (int u = 0; u < 4; u++) {
             const int sd = 4 * blk + u;
             if (sd < nDiff) {
-                float z;
-                if (a.noise == 0) {
-                    // u = (x+1)/2^32, 1-u = (2^32-1-x)/2^32 (exact integers), log(u/(1-u+1e-6))
-                    const float num = (float)w[u] + 1.0f;
-                    const float den = (float)(0xFFFFFFFFu - w[u]) + 4294.967296f;
-                    z = 0.551329f * logf(num / den);
-                } else {
-                    const int pr2 = u >> 1;
-                    const float lnua = log1pf(-(float)(0xFFFFFFFFu - w[2 * pr2]) * 0x1p-32f);
-                    const float ub = (float)w[2 * pr2 + 1] * 0x1p-32f;
-                    const float rad = sqrtf(-2.0f * lnua);
-                    z = (u & 1) ? rad * sinpif(2.0f * ub) : rad * cospif(2.0f * ub);
-                }
+                const float z = tail_noise(w, u, a.noise);
                 const float env = expf(fmaf((float)sd, ha, base));
                 tail[sd] = gain * env * z;
             }
@@ -671,6 +676,488 @@ __global__ __launch_bounds__(WG) void count_taps_kernel(AccArgs a, unsigned long
         unsigned long long t = 0;
         for (int w = 0; w < NWAVE; w++) t += red[w];
         atomicAdd(total, t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint (gradients of a loss L(out) with respect to beta, positions, level)
+// ---------------------------------------------------------------------------
+//
+// out[t] = sum_i amp_i h(t - tau_i) for t < nISM, and the tail
+// out[nISM + s] = level sqrt(A) exp(alpha (nISM + s - tau_dp) / 2) z_s with
+// A = P exp(-alpha (w_c - tau_dp)), P = mean(out[w_start:nISM]^2) (src/gpuRIR_cuda.cu:
+// 415-461).  Given g = dL/dout:
+//   * tail_adjoint_kernel (one workgroup per RIR) regenerates z_s, and from
+//     S1 = sum g_s sqrt(A) E_s z_s and S2 = sum g_s tail_s (nISM + s - tau_dp)/2
+//     forms dL/dlevel = S1, dL/dA, dL/dalpha, dL/dtau_dp; dL/dP adds c_w out[t]
+//     (c_w = dL/dP 2 / n_w) to the gradient of the early part on the envelope window;
+//   * ism_adjoint_kernel (one workgroup per work item) needs, per image, only
+//     G0 = sum_t g'_t h(t - tau) and G1 = sum_t g'_t h'(t - tau): dL/damp = G0 and
+//     dL/dtau = -amp G1.  The windowed sinc and its derivative are recomputed tap by
+//     tap (nothing per image is stored by the forward pass) and the chain rule
+//     through amp(beta, v), tau(v), v = 2mL + sigma src - rcv gives per-item sums;
+//   * grad_rir_kernel and grad_param_kernel add the per-item and per-RIR terms in a
+//     fixed order, so the gradients are identical from run to run.
+// The integer envelope window start w_start is held fixed (piecewise constant).
+
+struct AdjArgs {
+    const RoomDev* rooms;
+    const Item* items;
+    const float* pos_src;
+    const float* pos_rcv;
+    const float* orv_src;
+    const float* orv_rcv;
+    const float4* ktab;   // [nJ]: (k, sigma_k cos 2pi k/Tw, sigma_k sin 2pi k/Tw, sigma_k / 2), k = J0 + u
+    const float* y;       // forward output
+    const float* gy;      // dL/dout, same layout
+    const float* rir_cw;  // per RIR: c_w
+    const int* rir_ws;    // per RIR: w_start
+    double* item_part;    // [n_items][12]: dbeta[6], dsrc[3], drcv[3]
+    int spkr, mic;
+    int J0, nJ;
+    float halfTw, inv_tw2, pi_tw;  // Tw/2, 2/Tw, pi/Tw
+    int ntab_max, gp_len;
+};
+
+constexpr int AQ = 2 * WG;  // adjoint image queue (ring of two rounds)
+
+// Adjoint axis tables: offsets as the forward pass, F = (-b_lo)^e0 (-b_hi)^e1 and its
+// derivatives dF/db_lo = -e0 (-b_lo)^(e0-1) (-b_hi)^e1, dF/db_hi likewise.
+__device__ inline void axis_entry_adj(int n, int N, double L, float sd, float rd, float blo, float bhi, double* offs,
+                                      float* F, float* Flo, float* Fhi) {
+    const int k = n - N / 2;
+    const int m = (k + 1) >> 1;
+    const int q = k & 1;
+    const double sgn = (double)(1 - 2 * q);
+    offs[n] = (2.0 * (double)m * L + sgn * (double)sd) - (double)rd;
+    const int e0 = abs(m - q), e1 = abs(m);
+    const double a = -(double)blo, b = -(double)bhi;
+    const double pa = ipow(a, e0), pb = ipow(b, e1);
+    F[n] = (float)(pa * pb);
+    Flo[n] = e0 > 0 ? (float)(-(double)e0 * ipow(a, e0 - 1) * pb) : 0.0f;
+    Fhi[n] = e1 > 0 ? (float)(-(double)e1 * pa * ipow(b, e1 - 1)) : 0.0f;
+}
+
+// Polar pattern gain and its derivative with respect to cos(theta) (src/gpuRIR_cuda.cu:139-146).
+__device__ inline void pattern_gain_d(double c, int p, double* D, double* Dp) {
+    switch (p) {
+        case 0: *D = 1.0; *Dp = 0.0; break;
+        case 1: *D = c > 0.0 ? 1.0 : 0.0; *Dp = 0.0; break;
+        case 2: *D = 0.5 + 0.5 * c; *Dp = 0.5; break;
+        case 3: *D = 0.25 + 0.75 * c; *Dp = 0.75; break;
+        case 4: *D = 0.75 + 0.25 * c; *Dp = 0.25; break;
+        default: *D = c; *Dp = 1.0; break;
+    }
+}
+
+// One tap of G0, G1 at offset k (uniform constants kc = ktab[k - J0]):
+//   x = k - f, sin(pi x)/pi = sigma_k S, cos(pi x) = -sigma_k C (S = sin(pi f)/pi, C = cos(pi f)),
+//   u = S/x, v = (C + u)/x:  sinc = sigma_k u, sinc' = -sigma_k v,
+//   sigma_k w = sigma_k/2 + (sigma_k ck) hc + (sigma_k sk) hs,  sigma_k w' = (sigma_k ck) ps - (sigma_k sk) pc,
+//   g h = g (sigma w) u,  g h' = g ((sigma w') u - (sigma w) v).
+template <bool MASK>
+__device__ __forceinline__ void adj_tap(float4 kc, float g, float f, float S, float C, float hc, float hs, float pc,
+                                        float ps, float halfTw, float& G0, float& G1) {
+    const float x = kc.x - f;
+    if (MASK) g = fabsf(x) < halfTw ? g : 0.0f;
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float u = S * r;
+    const float v = (C + u) * r;
+    const float ws = fmaf(kc.y, hc, fmaf(kc.z, hs, kc.w));
+    const float wps = fmaf(kc.y, ps, -kc.z * pc);
+    G0 = fmaf(g * ws, u, G0);
+    G1 = fmaf(g, fmaf(wps, u, -ws * v), G1);
+}
+
+// Taps k = 0 and k = 1, where |x| can be small: series for |x| < 1/16,
+// sinc = 1 - (pi x)^2/6 + (pi x)^4/120, sinc' = pi^2 x (-1/3 + (pi x)^2/30).
+template <bool MASK>
+__device__ __forceinline__ void adj_tap_near(float4 kc, float g, float f, float S, float C, float hc, float hs,
+                                             float pc, float ps, float halfTw, float& G0, float& G1) {
+    const float x = kc.x - f;
+    if (MASK) g = fabsf(x) < halfTw ? g : 0.0f;
+    const float sg = 2.0f * kc.w;  // sigma_k
+    float u, v;
+    if (fabsf(x) < 0.0625f) {
+        const float px2 = 9.8696044010893586f * x * x;
+        const float sinc = fmaf(px2, fmaf(px2, 1.0f / 120.0f, -1.0f / 6.0f), 1.0f);
+        const float sincp = 9.8696044010893586f * x * fmaf(px2, 1.0f / 30.0f, -1.0f / 3.0f);
+        u = sg * sinc;
+        v = -sg * sincp;
+    } else {
+        const float r = 1.0f / x;
+        u = S * r;
+        v = (C + u) * r;
+    }
+    const float ws = fmaf(kc.y, hc, fmaf(kc.z, hs, kc.w));
+    const float wps = fmaf(kc.y, ps, -kc.z * pc);
+    G0 = fmaf(g * ws, u, G0);
+    G1 = fmaf(g, fmaf(wps, u, -ws * v), G1);
+}
+
+struct AdjImg {
+    double ox, oy, oz;  // unit source / receiver orientation (receiver: or*, source: os*)
+};
+
+template <bool MASK>
+__global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NWAVE][12];
+    __shared__ int wcnt[NWAVE];
+    const int bid = blockIdx.x;
+    const Item it = a.items[bid];
+    const RoomK R = load_room(a.rooms + it.room);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ntab = a.ntab_max;
+    double* offs = reinterpret_cast<double*>(smem_raw);
+    float* F = reinterpret_cast<float*>(offs + ntab);
+    float* Flo = F + ntab;
+    float* Fhi = Flo + ntab;
+    float* gp = Fhi + ntab;                                  // [gp_len] g' around the tile
+    int* qi = reinterpret_cast<int*>(gp + a.gp_len);         // [AQ] image index
+    int* qn = qi + AQ;                                       // [AQ] g' index of tap k = J0
+    float* qf = reinterpret_cast<float*>(qn + AQ);           // [AQ] fraction
+
+    const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
+    const int g_rir = R.rir_off + it.rir;
+    const float* ps = a.pos_src + 3 * (R.src_off + s);
+    const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    for (int n = tid; n < R.Nx; n += WG)
+        axis_entry_adj(n, R.Nx, R.L0, ps[0], pr[0], R.b0, R.b1, offs, F, Flo, Fhi);
+    for (int n = tid; n < R.Ny; n += WG)
+        axis_entry_adj(n, R.Ny, R.L1, ps[1], pr[1], R.b2, R.b3, offs + R.Nx, F + R.Nx, Flo + R.Nx, Fhi + R.Nx);
+    for (int n = tid; n < R.Nz; n += WG) {
+        const int o = R.Nx + R.Ny;
+        axis_entry_adj(n, R.Nz, R.L2, ps[2], pr[2], R.b4, R.b5, offs + o, F + o, Flo + o, Fhi + o);
+    }
+    // g'(t) on [tb, tb + gp_len): dL/dout plus the envelope-window term, zero outside the tile
+    const int J1 = a.J0 + a.nJ - 1;
+    const int tb = it.t_lo - J1 + a.J0;
+    {
+        const long long row = R.out_off + (long long)it.rir * R.nSamples;
+        const float cw = a.rir_cw[g_rir];
+        const int ws = a.rir_ws[g_rir];
+        const int lo = max(it.t_lo, 0), hi = min(it.t_hi, R.nISM);
+        for (int u = tid; u < a.gp_len; u += WG) {
+            const int t = tb + u;
+            float g = 0.0f;
+            if (t >= lo && t < hi) {
+                g = a.gy[row + t];
+                if (t >= ws) g = fmaf(cw, a.y[row + t], g);
+            }
+            gp[u] = g;
+        }
+    }
+    // unit orientations (sigma applied per image for the source)
+    double orx = 0, ory = 0, orz = 0, osx = 0, osy = 0, osz = 0;
+    if (a.mic != 0) {
+        const float* o = a.orv_rcv + 3 * (R.rcv_off + r);
+        const double n = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+        orx = o[0] / n;
+        ory = o[1] / n;
+        orz = o[2] / n;
+    }
+    if (a.spkr != 0) {
+        const float* o = a.orv_src + 3 * (R.src_off + s);
+        const double n = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+        osx = o[0] / n;
+        osy = o[1] / n;
+        osz = o[2] / n;
+    }
+    __syncthreads();
+
+    const int lo_n0 = it.t_lo - J1, hi_n0 = it.t_hi - 1 - a.J0;
+    double acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = 0.0;
+    const int u0 = -a.J0;  // ktab index of k = 0 (J0 <= 0 < 1 <= J1)
+
+    auto process = [&](int e) {
+        const int i = qi[e], nrel = qn[e];
+        const float f = qf[e];
+        const float S = sinpif(f) * 0.31830988618379067f, C = cospif(f);
+        const float ph = f * a.inv_tw2;
+        const float cph = cospif(ph), sph = sinpif(ph);
+        const float hc = 0.5f * cph, hs = 0.5f * sph, pc = a.pi_tw * cph, pss = a.pi_tw * sph;
+        const float* gk = gp + nrel;
+        float G0 = 0.0f, G1 = 0.0f;
+#pragma unroll 4
+        for (int u = 0; u < u0; u++)
+            adj_tap<MASK>(a.ktab[u], gk[u], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+        adj_tap_near<MASK>(a.ktab[u0], gk[u0], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+        adj_tap_near<MASK>(a.ktab[u0 + 1], gk[u0 + 1], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+#pragma unroll 4
+        for (int u = u0 + 2; u < a.nJ; u++)
+            adj_tap<MASK>(a.ktab[u], gk[u], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+
+        // chain rule through amp(beta, v) and tau(v) in FP64
+        const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+        const int nx = (int)(((double)i + 0.5) * R.inv_nyz);
+        const int rem = i - nx * (Ny * Nz);
+        const int ny = (int)(((double)rem + 0.5) * R.inv_nz);
+        const int nz = rem - ny * Nz;
+        const int ey = Nx + ny, ez = Nx + Ny + nz;
+        const double vx = offs[nx], vy = offs[ey], vz = offs[ez];
+        const double d = sqrt(vx * vx + vy * vy + vz * vz);
+        const double id = 1.0 / d;
+        const double ux = vx * id, uy = vy * id, uz = vz * id;
+        const double Fx = F[nx], Fy = F[ey], Fz = F[ez];
+        const double att = Fx * Fy * Fz;
+        const double a0 = id / 12.566370614359172;
+        const double sx = ((nx - Nx / 2) & 1) ? -1.0 : 1.0;
+        const double sy = ((ny - Ny / 2) & 1) ? -1.0 : 1.0;
+        const double sz = ((nz - Nz / 2) & 1) ? -1.0 : 1.0;
+        double Dr = 1.0, Drp = 0.0, Ds = 1.0, Dsp = 0.0, cr = 0.0, cs = 0.0;
+        if (a.mic != 0) {
+            cr = ux * orx + uy * ory + uz * orz;
+            pattern_gain_d(cr, a.mic, &Dr, &Drp);
+        }
+        const double qx = sx * osx, qy = sy * osy, qz = sz * osz;  // mirrored source orientation
+        if (a.spkr != 0) {
+            cs = -(ux * qx + uy * qy + uz * qz);
+            pattern_gain_d(cs, a.spkr, &Ds, &Dsp);
+        }
+        const double geo = Ds * Dr * a0;
+        const double amp = att * geo;
+        const double g0 = (double)G0, g1 = (double)G1;
+        const double gbeta = g0 * geo;
+        acc[0] += gbeta * (double)Flo[nx] * Fy * Fz;
+        acc[1] += gbeta * (double)Fhi[nx] * Fy * Fz;
+        acc[2] += gbeta * Fx * (double)Flo[ey] * Fz;
+        acc[3] += gbeta * Fx * (double)Fhi[ey] * Fz;
+        acc[4] += gbeta * Fx * Fy * (double)Flo[ez];
+        acc[5] += gbeta * Fx * Fy * (double)Fhi[ez];
+        // d amp / d v = att a0 (-Ds Dr u/d + Ds' Dr dcs/dv + Ds Dr' dcr/dv),
+        // dcr/dv = (o_r - cr u)/d, dcs/dv = (-o_s' - cs u)/d;  d tau / d v = (fs/c) u
+        const double kr = att * a0 * Ds * Drp * id, ks = att * a0 * Dsp * Dr * id;
+        const double kd = -att * a0 * Ds * Dr * id;
+        const double ku = g0 * (kd - kr * cr - ks * cs) + (-amp * g1) * R.fs_over_c;
+        const double gvx = ku * ux + g0 * (kr * orx - ks * qx);
+        const double gvy = ku * uy + g0 * (kr * ory - ks * qy);
+        const double gvz = ku * uz + g0 * (kr * orz - ks * qz);
+        acc[6] += sx * gvx;
+        acc[7] += sy * gvy;
+        acc[8] += sz * gvz;
+        acc[9] -= gvx;
+        acc[10] -= gvy;
+        acc[11] -= gvz;
+    };
+
+    // image rounds of WG: phase A lists the images whose taps reach the tile into a
+    // ring queue in a fixed order (wave by wave), phase B processes WG entries at a time
+    int head = 0, n = 0;  // ring head and live entries (uniform)
+    for (int r0 = it.i0; r0 < it.i1; r0 += WG) {
+        const int i = r0 + tid;
+        bool live = false;
+        int n0 = 0;
+        float f = 0.0f;
+        if (i < it.i1) {
+            const int Nyz = R.Ny * R.Nz;
+            const int nx = (int)(((double)i + 0.5) * R.inv_nyz);
+            const int rem = i - nx * Nyz;
+            const int ny = (int)(((double)rem + 0.5) * R.inv_nz);
+            const int nz = rem - ny * R.Nz;
+            const double vx = offs[nx], vy = offs[R.Nx + ny], vz = offs[R.Nx + R.Ny + nz];
+            double d2 = vx * vx;
+            d2 = d2 + vy * vy;
+            d2 = d2 + vz * vz;
+            const double tau = sqrt(d2) * R.fs_over_c;
+            const double fl = floor(tau);
+            n0 = (int)fl;
+            f = (float)(tau - fl);
+            live = n0 >= lo_n0 && n0 <= hi_n0;
+        }
+        const unsigned long long mask = __ballot(live);
+        if (lane == 0) wcnt[wave] = __popcll(mask);
+        __syncthreads();
+        int base = head + n, total = 0;
+        for (int w = 0; w < NWAVE; w++) {
+            const int c = wcnt[w];
+            if (w < wave) base += c;
+            total += c;
+        }
+        if (live) {
+            const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+            const int q = (base + pre) & (AQ - 1);
+            qi[q] = i;
+            qn[q] = n0 + a.J0 - tb;
+            qf[q] = f;
+        }
+        n += total;
+        __syncthreads();
+        const bool last = r0 + WG >= it.i1;
+        while (n >= WG || (last && n > 0)) {
+            const int take = min(n, WG);
+            if (tid < take) process((head + tid) & (AQ - 1));
+            head = (head + take) & (AQ - 1);
+            n -= take;
+            __syncthreads();  // slots of this pass are rewritten by the next round
+        }
+    }
+
+    // fixed-order reduction: lanes, then waves
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (tid < 12) {
+        double t = 0.0;
+        for (int w = 0; w < NWAVE; w++) t += red[w][tid];
+        a.item_part[(long long)bid * 12 + tid] = t;
+    }
+}
+
+struct TailAdjArgs {
+    const RoomDev* rooms;
+    const int* rir_room;
+    const float* pos_src;
+    const float* pos_rcv;
+    const float* y;
+    const float* gy;
+    float* rir_cw;
+    int* rir_ws;
+    double* rir_tail;  // [n_rirs][13]: dbeta[6], dsrc[3], drcv[3], dlevel
+    float w_sz;
+    double fs;
+    unsigned long long seed;
+    unsigned call_offset;
+    int noise;
+};
+
+__global__ __launch_bounds__(WG) void tail_adjoint_kernel(TailAdjArgs a) {
+    __shared__ double red[NWAVE];
+    const int g = blockIdx.x;
+    const RoomK R = load_room(a.rooms + a.rir_room[g]);
+    const int m = g - R.rir_off;
+    const int s = m / R.n_rcv, r = m - s * R.n_rcv;
+    const int tid = threadIdx.x;
+    const int nISM = R.nISM, nS = R.nSamples, nDiff = nS - nISM;
+    const long long row = R.out_off + (long long)m * nS;
+    const float* y = a.y + row;
+    const float* gy = a.gy + row;
+    double* outp = a.rir_tail + 13LL * g;
+
+    const float* ps = a.pos_src + 3 * (R.src_off + s);
+    const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    double v[3];
+    for (int d = 0; d < 3; d++) v[d] = (0.0 + (double)ps[d]) - (double)pr[d];
+    double d2 = v[0] * v[0];
+    d2 = d2 + v[1] * v[1];
+    d2 = d2 + v[2] * v[2];
+    const double d0 = sqrt(d2);
+    const double tau_dp = d0 * R.fs_over_c;
+    const int w_start = (int)ceil(fmax((double)nISM - (double)a.w_sz, tau_dp));
+    if (nDiff <= 0) {
+        if (tid < 13) outp[tid] = 0.0;
+        if (tid == 0) {
+            a.rir_cw[g] = 0.0f;
+            a.rir_ws[g] = nISM;
+        }
+        return;
+    }
+    double pw = 0.0;
+    for (int t = w_start + tid; t < nISM; t += WG) {
+        const double x = y[t];
+        pw += x * x;
+    }
+    const double nw = (double)(nISM - w_start);
+    const double P = block_sum(pw, red) / nw;
+    const double L0 = R.L0, L1 = R.L1, L2 = R.L2;
+    const double b[6] = {R.b0, R.b1, R.b2, R.b3, R.b4, R.b5};
+    const double Sw[6] = {L1 * L2, L1 * L2, L0 * L2, L0 * L2, L0 * L1, L0 * L1};
+    const double Sa = ((1.0 - b[0] * b[0]) + (1.0 - b[1] * b[1])) * L1 * L2 +
+                      ((1.0 - b[2] * b[2]) + (1.0 - b[3] * b[3])) * L0 * L2 +
+                      ((1.0 - b[4] * b[4]) + (1.0 - b[5] * b[5])) * L0 * L1;
+    const double V = L0 * L1 * L2;
+    const double T60 = 0.161 * V / Sa;
+    const double alpha = -13.8155 / (T60 * a.fs);
+    const double w_center = (double)w_start + (double)(nISM - w_start) / 2.0;
+    const double A = P / exp(alpha * (w_center - tau_dp));
+    const double sA = sqrt(A);
+    const double level = (double)R.level;
+
+    const unsigned k0 = (unsigned)a.seed, k1 = (unsigned)(a.seed >> 32);
+    double s1 = 0.0, s2 = 0.0;
+    const int nblk = (nDiff + 3) >> 2;
+    for (int blk = tid; blk < nblk; blk += WG) {
+        unsigned w[4];
+        philox4x32_10((unsigned)blk, (unsigned)m, R.room_id, a.call_offset, k0, k1, w);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int sd = 4 * blk + u;
+            if (sd < nDiff) {
+                const double z = (double)tail_noise(w, u, a.noise);
+                const double ts = (double)(nISM + sd) - tau_dp;
+                const double q = sA * exp(0.5 * alpha * ts) * z;  // tail / level
+                const double gq = (double)gy[nISM + sd] * q;
+                s1 += gq;
+                s2 += gq * ts;
+            }
+        }
+    }
+    s1 = block_sum(s1, red);
+    s2 = 0.5 * level * block_sum(s2, red);
+    const double gA = A > 0.0 ? level * s1 / (2.0 * A) : 0.0;
+    const double galpha = s2 - gA * A * (w_center - tau_dp);
+    const double gtau = -0.5 * alpha * level * s1 + gA * A * alpha;
+    if (tid == 0) {
+        a.rir_cw[g] = P > 0.0 ? (float)(gA * (A / P) * 2.0 / nw) : 0.0f;
+        a.rir_ws[g] = w_start;
+        // alpha = -13.8155 Sa / (0.161 V fs): d alpha / d b_k = 13.8155 * 2 b_k S_k / (0.161 V fs)
+        const double ka = 13.8155 * 2.0 / (0.161 * V * a.fs);
+        for (int k = 0; k < 6; k++) outp[k] = galpha * ka * b[k] * Sw[k];
+        // tau_dp = |src - rcv| fs/c
+        for (int d = 0; d < 3; d++) {
+            const double gs = gtau * R.fs_over_c * v[d] / d0;
+            outp[6 + d] = gs;
+            outp[9 + d] = -gs;
+        }
+        outp[12] = s1;
+    }
+}
+
+// Per RIR: its items' partial sums (items of one RIR are contiguous) plus the tail terms.
+__global__ __launch_bounds__(WG) void grad_rir_kernel(const double* item_part, const int* rir_item_begin,
+                                                      const double* rir_tail, double* rir_grad, int n_rirs) {
+    const int g = blockIdx.x * WG + threadIdx.x;
+    if (g >= n_rirs) return;
+    double t[13];
+    for (int k = 0; k < 13; k++) t[k] = 0.0;
+    for (int e = rir_item_begin[g]; e < rir_item_begin[g + 1]; e++)
+        for (int k = 0; k < 12; k++) t[k] += item_part[(long long)e * 12 + k];
+    for (int k = 0; k < 13; k++) rir_grad[(long long)g * 13 + k] = t[k] + rir_tail[(long long)g * 13 + k];
+}
+
+// Per room: d beta[6], d level, and the rows of its sources and receivers.
+__global__ __launch_bounds__(WG) void grad_param_kernel(const RoomDev* rooms, const double* rir_grad, float* g_beta,
+                                                        float* g_src, float* g_rcv, float* g_level) {
+    const RoomDev* R = rooms + blockIdx.x;
+    const int ns = R->n_src, nr = R->n_rcv, ro = R->rir_off;
+    const int n_out = 7 + 3 * ns + 3 * nr;
+    for (int o = threadIdx.x; o < n_out; o += WG) {
+        double t = 0.0;
+        if (o < 7) {
+            const int k = o < 6 ? o : 12;
+            for (int m = 0; m < ns * nr; m++) t += rir_grad[(long long)(ro + m) * 13 + k];
+            if (o < 6) {
+                if (g_beta) g_beta[(long long)blockIdx.x * 6 + o] = (float)t;
+            } else if (g_level) {
+                g_level[blockIdx.x] = (float)t;
+            }
+        } else if (o < 7 + 3 * ns) {
+            const int sidx = (o - 7) / 3, d = (o - 7) % 3;
+            for (int rr = 0; rr < nr; rr++) t += rir_grad[(long long)(ro + sidx * nr + rr) * 13 + 6 + d];
+            if (g_src) g_src[(long long)(R->src_off + sidx) * 3 + d] = (float)t;
+        } else {
+            const int ridx = (o - 7 - 3 * ns) / 3, d = (o - 7 - 3 * ns) % 3;
+            for (int ss = 0; ss < ns; ss++) t += rir_grad[(long long)(ro + ss * nr + ridx) * 13 + 9 + d];
+            if (g_rcv) g_rcv[(long long)(R->rcv_off + ridx) * 3 + d] = (float)t;
+        }
     }
 }
 
@@ -783,7 +1270,7 @@ size_t acc_lds_bytes(int tt, int ntab, const TapTables& T) {
     return 12 * nt + 8 * acc + (size_t)(QCAP + QPAD) * 20 + 16 + sizeof(float2) * T.lut.size();
 }
 
-Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt) {
+Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt, bool sort_items) {
     Plan P;
     int ntab = 0, nism_max = 0;
     for (const RoomHost& h : rh) {
@@ -870,7 +1357,7 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const Tap
         P.rooms.push_back(R);
     }
     // longest items first, so that the tail of the launch is short
-    if (rh.size() > 1)
+    if (sort_items && rh.size() > 1)
         std::stable_sort(P.items.begin(), P.items.end(),
                          [](const Item& x, const Item& y) { return (x.i1 - x.i0) > (y.i1 - y.i0); });
     (void)total_rirs;
@@ -913,7 +1400,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     GPURIR_CHECK(S.Tw >= 2, "sampling frequency too low for the 8 ms window");
     S.prec = opt.precision;
     S.T = make_tap_tables(S.prec, S.Tw);
-    S.P = make_plan(rooms, opt, S.T);
+    S.P = make_plan(rooms, opt, S.T, true);
     const Plan& P = S.P;
     const TapTables& T = S.T;
     S.off_rooms = 0;
@@ -1078,7 +1565,7 @@ int launch_image_list(const RoomHost& room, const float* d_pos_src, const float*
     GPURIR_CHECK(room.nb[0] > 0 && room.nb[1] > 0 && room.nb[2] > 0, "nb_img must be positive in every dimension");
     GPURIR_CHECK(room.nb[0] + room.nb[1] + room.nb[2] <= 4096, "too many images along the room axes");
     std::vector<RoomHost> rh(1, room);
-    Plan P = make_plan(rh, opt, make_tap_tables(PREC_FP32, std::max(tw_rule(opt.fs), 2)));
+    Plan P = make_plan(rh, opt, make_tap_tables(PREC_FP32, std::max(tw_rule(opt.fs), 2)), false);
     void* dplan = nullptr;
     if (int rc = workspace(WS_PLAN, sizeof(RoomDev), &dplan)) return rc;
     if (int rc = upload_async(dplan, P.rooms.data(), sizeof(RoomDev), stream)) return rc;
@@ -1094,4 +1581,188 @@ int launch_image_list(const RoomHost& room, const float* d_pos_src, const float*
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// adjoint launch
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr int ADJ_TILE = 8192;  // ISM samples of g' per adjoint workgroup at most
+}
+
+// Host description of one adjoint launch.  Items are those of the forward plan
+// for the same rooms, left in (room, RIR, chunk, tile) order so that the items of
+// one RIR are contiguous; the blob is [rooms | items | rir_room | ktab | rir_item_begin].
+struct AdjLaunch {
+    Plan P;
+    TapTables T;
+    std::vector<float4> ktab;
+    std::vector<int> rir_item_begin;
+    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, plan_bytes = 0;
+    std::vector<unsigned char> blob;
+    size_t lds = 0;
+    int ntab = 0, gp_len = 0, Tw = 0;
+    bool mask = false;
+    SimOptions opt;
+};
+
+AdjLaunch* adj_new() { return new AdjLaunch(); }
+void adj_delete(AdjLaunch* a) { delete a; }
+size_t adj_plan_bytes(const AdjLaunch& A) { return A.plan_bytes; }
+const void* adj_blob(const AdjLaunch& A) { return A.blob.data(); }
+
+// workspace: item partials [n_items][12] | rir_tail [n_rirs][13] | rir_grad [n_rirs][13] | cw | ws
+size_t adj_ws_bytes(const AdjLaunch& A) {
+    const size_t ni = A.P.items.size(), nr = A.P.rir_room.size();
+    return align_up(sizeof(double) * 12 * ni, 256) + 2 * align_up(sizeof(double) * 13 * nr, 256) +
+           align_up(sizeof(float) * nr, 256) + align_up(sizeof(int) * nr, 256);
+}
+
+int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, AdjLaunch& A) {
+    GPURIR_CHECK(opt_in.fs > 0.0f && opt_in.c > 0.0f, "fs and c must be positive");
+    GPURIR_CHECK(!rooms.empty(), "no rooms");
+    int nism_max = 0;
+    for (const RoomHost& h : rooms) {
+        GPURIR_CHECK(h.nb[0] > 0 && h.nb[1] > 0 && h.nb[2] > 0, "nb_img must be positive in every dimension");
+        GPURIR_CHECK((long long)h.nb[0] * h.nb[1] * h.nb[2] < (1LL << 31), "too many images per RIR");
+        GPURIR_CHECK(h.nb[0] + h.nb[1] + h.nb[2] <= 4096, "too many images along the room axes");
+        GPURIR_CHECK(h.n_src > 0 && h.n_rcv > 0, "each room needs at least one source and one receiver");
+        GPURIR_CHECK(h.Tdiff <= h.Tmax, "Tmax must be equal or greater than Tdiff");
+        GPURIR_CHECK(h.Tdiff >= 0.0f, "Tdiff must be non-negative");
+        nism_max = std::max(nism_max, n_samples_rule(h.Tdiff, opt_in.fs));
+    }
+    SimOptions opt = opt_in;
+    if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
+    A.opt = opt;
+    A.Tw = tw_rule(opt.fs);
+    GPURIR_CHECK(A.Tw >= 2, "sampling frequency too low for the 8 ms window");
+    A.T = make_tap_tables(PREC_FP32, A.Tw);
+    A.mask = (A.Tw % 2) != 0;
+    A.P = make_plan(rooms, opt, A.T, false);
+    const Plan& P = A.P;
+    const int J1 = A.T.J0 + A.T.nJ - 1;
+    GPURIR_CHECK(A.T.J0 <= 0 && J1 >= 1, "window too short for the adjoint");
+    A.ktab.resize(A.T.nJ);
+    for (int u = 0; u < A.T.nJ; u++) {
+        const int k = A.T.J0 + u;
+        const double sg = (k & 1) ? 1.0 : -1.0;  // (-1)^(k+1)
+        const double th = 2.0 * M_PI * (double)k / (double)A.Tw;
+        A.ktab[u] = make_float4((float)k, (float)(sg * std::cos(th)), (float)(sg * std::sin(th)), (float)(0.5 * sg));
+    }
+    const int n_rirs = (int)P.rir_room.size();
+    A.rir_item_begin.assign(n_rirs + 1, 0);
+    {
+        // items are in (room, rir, chunk, tile) order; global RIR = rir_off + rir
+        std::vector<int> cnt(n_rirs, 0);
+        for (const Item& it : P.items) cnt[P.rooms[it.room].rir_off + it.rir]++;
+        for (int g = 0; g < n_rirs; g++) A.rir_item_begin[g + 1] = A.rir_item_begin[g] + cnt[g];
+    }
+    A.off_rooms = 0;
+    A.off_items = align_up(sizeof(RoomDev) * P.rooms.size(), 256);
+    A.off_rr = align_up(A.off_items + sizeof(Item) * P.items.size(), 256);
+    A.off_kt = align_up(A.off_rr + sizeof(int) * P.rir_room.size(), 256);
+    A.off_rib = align_up(A.off_kt + sizeof(float4) * A.ktab.size(), 256);
+    A.plan_bytes = align_up(A.off_rib + sizeof(int) * A.rir_item_begin.size(), 256);
+    A.blob.assign(A.plan_bytes, 0);
+    std::memcpy(A.blob.data() + A.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
+    if (!P.items.empty()) std::memcpy(A.blob.data() + A.off_items, P.items.data(), sizeof(Item) * P.items.size());
+    std::memcpy(A.blob.data() + A.off_rr, P.rir_room.data(), sizeof(int) * P.rir_room.size());
+    std::memcpy(A.blob.data() + A.off_kt, A.ktab.data(), sizeof(float4) * A.ktab.size());
+    std::memcpy(A.blob.data() + A.off_rib, A.rir_item_begin.data(), sizeof(int) * A.rir_item_begin.size());
+    A.ntab = (P.ntab_max + 3) / 4 * 4;
+    A.gp_len = (int)align_up((size_t)P.tt_max + 2 * (size_t)(A.T.nJ - 1) + 4, 4);
+    A.lds = (sizeof(double) + 3 * sizeof(float)) * A.ntab + sizeof(float) * A.gp_len + (size_t)AQ * 12;
+    GPURIR_CHECK(A.lds <= 150 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
+    return 0;
+}
+
+int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float* d_pos_src, const float* d_pos_rcv,
+                const float* d_orv_src, const float* d_orv_rcv, const float* d_y, const float* d_gy, float* g_beta,
+                float* g_src, float* g_rcv, float* g_level, hipStream_t stream) {
+    const unsigned char* dp = static_cast<const unsigned char*>(d_plan);
+    const Plan& P = A.P;
+    const int n_rirs = (int)P.rir_room.size();
+    unsigned char* w = static_cast<unsigned char*>(d_ws);
+    double* item_part = reinterpret_cast<double*>(w);
+    w += align_up(sizeof(double) * 12 * P.items.size(), 256);
+    double* rir_tail = reinterpret_cast<double*>(w);
+    w += align_up(sizeof(double) * 13 * n_rirs, 256);
+    double* rir_grad = reinterpret_cast<double*>(w);
+    w += align_up(sizeof(double) * 13 * n_rirs, 256);
+    float* rir_cw = reinterpret_cast<float*>(w);
+    w += align_up(sizeof(float) * n_rirs, 256);
+    int* rir_ws = reinterpret_cast<int*>(w);
+    const RoomDev* d_rooms = reinterpret_cast<const RoomDev*>(dp + A.off_rooms);
+    const int* d_rr = reinterpret_cast<const int*>(dp + A.off_rr);
+
+    TailAdjArgs t;
+    t.rooms = d_rooms;
+    t.rir_room = d_rr;
+    t.pos_src = d_pos_src;
+    t.pos_rcv = d_pos_rcv;
+    t.y = d_y;
+    t.gy = d_gy;
+    t.rir_cw = rir_cw;
+    t.rir_ws = rir_ws;
+    t.rir_tail = rir_tail;
+    t.w_sz = 10e-3f * A.opt.fs;
+    t.fs = (double)A.opt.fs;
+    t.seed = A.opt.seed;
+    t.call_offset = A.opt.call_offset;
+    t.noise = A.opt.noise;
+    hipLaunchKernelGGL(tail_adjoint_kernel, dim3((unsigned)n_rirs), dim3(WG), 0, stream, t);
+    GPURIR_HIP(hipGetLastError());
+
+    if (!P.items.empty()) {
+        AdjArgs a;
+        a.rooms = d_rooms;
+        a.items = reinterpret_cast<const Item*>(dp + A.off_items);
+        a.pos_src = d_pos_src;
+        a.pos_rcv = d_pos_rcv;
+        a.orv_src = d_orv_src;
+        a.orv_rcv = d_orv_rcv;
+        a.ktab = reinterpret_cast<const float4*>(dp + A.off_kt);
+        a.y = d_y;
+        a.gy = d_gy;
+        a.rir_cw = rir_cw;
+        a.rir_ws = rir_ws;
+        a.item_part = item_part;
+        a.spkr = A.opt.spkr;
+        a.mic = A.opt.mic;
+        a.J0 = A.T.J0;
+        a.nJ = A.T.nJ;
+        a.halfTw = A.T.halfTw;
+        a.inv_tw2 = 2.0f / (float)A.Tw;
+        a.pi_tw = (float)(M_PI / (double)A.Tw);
+        a.ntab_max = A.ntab;
+        a.gp_len = A.gp_len;
+        const void* fn = A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
+                                : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);
+        if (A.lds > 64 * 1024)
+            GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)A.lds));
+        void* args[] = {&a};
+        GPURIR_HIP(hipLaunchKernel(fn, dim3((unsigned)P.items.size()), dim3(WG), args, A.lds, stream));
+    }
+    hipLaunchKernelGGL(grad_rir_kernel, dim3((unsigned)((n_rirs + WG - 1) / WG)), dim3(WG), 0, stream, item_part,
+                       reinterpret_cast<const int*>(dp + A.off_rib), rir_tail, rir_grad, n_rirs);
+    GPURIR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(grad_param_kernel, dim3((unsigned)P.rooms.size()), dim3(WG), 0, stream, d_rooms, rir_grad,
+                       g_beta, g_src, g_rcv, g_level);
+    GPURIR_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_adjoint(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
+                   const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, const float* d_y,
+                   const float* d_gy, float* g_beta, float* g_src, float* g_rcv, float* g_level, hipStream_t stream) {
+    AdjLaunch A;
+    if (int rc = adj_prepare(rooms, opt, A)) return rc;
+    void* dplan = nullptr;
+    if (int rc = workspace(WS_ADJ_PLAN, A.plan_bytes, &dplan)) return rc;
+    if (int rc = upload_async(dplan, A.blob.data(), A.plan_bytes, stream)) return rc;
+    void* dws = nullptr;
+    if (int rc = workspace(WS_GRAD, adj_ws_bytes(A), &dws)) return rc;
+    return adj_enqueue(A, dplan, dws, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_y, d_gy, g_beta, g_src, g_rcv,
+                       g_level, stream);
+}
 }  // namespace gpurir

@@ -5,6 +5,9 @@ current HIP stream, nothing copied to the host.
   image_list         stage (a) per (RIR, image): n0, fraction, amplitude
   simulate_batch     many rooms of uneven sizes in one launch
   BatchPlan          the same, planned once and replayed as a hipGraph
+  simulate_batch_diff / simulate_rir_diff
+                     differentiable versions (torch.autograd) with respect to beta,
+                     source and receiver positions and the diffuse-tail level
   trajectory         simulateTrajectory on device tensors (float64 output)
 """
 import ctypes
@@ -205,6 +208,19 @@ class BatchPlan:
         _lib.check(_lib.load().gpurir_batch_plan_profile(self._h, _stream(self.dev), ctypes.byref(a), ctypes.byref(f)))
         return float(a.value), float(f.value)
 
+    def adjoint(self, grad_out):
+        """Gradients (beta [R,6], pos_src [sum n_src,3], pos_rcv [sum n_rcv,3], level [R]) of
+        L given grad_out = dL/d out (flat, the layout of self.out) and the last run()'s output."""
+        g = grad_out.detach().to(device=self.dev, dtype=torch.float32).contiguous()
+        assert g.numel() == self.out.numel()
+        gb = torch.empty((self.B.R, 6), dtype=torch.float32, device=self.dev)
+        gs = torch.empty_like(self.B.d_ps)
+        gr = torch.empty_like(self.B.d_pr)
+        gl = torch.empty(self.B.R, dtype=torch.float32, device=self.dev)
+        _lib.check(_lib.load().gpurir_batch_plan_adjoint(self._h, g.data_ptr(), gb.data_ptr(), gs.data_ptr(),
+                                                         gr.data_ptr(), gl.data_ptr(), _stream(self.dev)))
+        return gb, gs, gr, gl
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -213,6 +229,95 @@ class BatchPlan:
             except Exception:
                 pass
             self._h = None
+
+
+class _BatchFn(torch.autograd.Function):
+    """out = simulate(beta, pos_src, pos_rcv, level) for a fixed batch description."""
+
+    @staticmethod
+    def forward(ctx, beta, pos_src, pos_rcv, level, cfg):
+        dev = cfg["dev"]
+        B = cfg["B"]
+        B.beta = np.ascontiguousarray(beta.detach().to("cpu", torch.float32).numpy().reshape(B.R, 6))
+        B.level = np.ascontiguousarray(level.detach().to("cpu", torch.float32).numpy().reshape(B.R))
+        B.d_ps = pos_src.detach().to(device=dev, dtype=torch.float32).contiguous()
+        B.d_pr = pos_rcv.detach().to(device=dev, dtype=torch.float32).contiguous()
+        out = torch.empty(B.total, dtype=torch.float32, device=dev)
+        o = cfg["opt"]
+        _lib.check(_lib.load().gpurir_simulate_batch_device(
+            *B.call_args(), cfg["spkr"], cfg["mic"], cfg["fs"], cfg["c"], ctypes.byref(o), out.data_ptr(),
+            B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _stream(dev)))
+        ctx.cfg = cfg
+        ctx.args = (B.beta.copy(), B.level.copy(), B.d_ps, B.d_pr)
+        ctx.in_meta = [(t.dtype, t.device) for t in (beta, pos_src, pos_rcv, level)]
+        ctx.save_for_backward(out)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (out,) = ctx.saved_tensors
+        cfg = ctx.cfg
+        dev, B = cfg["dev"], cfg["B"]
+        B.beta, B.level, B.d_ps, B.d_pr = ctx.args
+        g = grad_out.detach().to(device=dev, dtype=torch.float32).contiguous()
+        gb = torch.empty((B.R, 6), dtype=torch.float32, device=dev)
+        gs = torch.empty_like(B.d_ps)
+        gr = torch.empty_like(B.d_pr)
+        gl = torch.empty(B.R, dtype=torch.float32, device=dev)
+        o = cfg["opt"]
+        _lib.check(_lib.load().gpurir_simulate_batch_adjoint_device(
+            *B.call_args(), cfg["spkr"], cfg["mic"], cfg["fs"], cfg["c"], ctypes.byref(o), out.data_ptr(),
+            g.data_ptr(), B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), gb.data_ptr(), gs.data_ptr(),
+            gr.data_ptr(), gl.data_ptr(), _stream(dev)))
+        res = [t.to(device=dv, dtype=dt) for t, (dt, dv) in zip((gb, gs, gr, gl), ctx.in_meta)]
+        return res[0], res[1], res[2], res[3], None
+
+
+def simulate_batch_diff(rooms, beta, pos_src, pos_rcv, level, fs, spkr_pattern="omni", mic_pattern="omni",
+                        c=343.0, noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0,
+                        tile_samples=0, device=None):
+    """Differentiable batch simulation.
+
+    rooms: list of dicts with room_sz, nb_img, Tmax [, Tdiff, orV_src, orV_rcv] and the number of
+    sources / receivers given by pos_src / pos_rcv rows (keys "n_src", "n_rcv" or the shapes of
+    "pos_src" / "pos_rcv").  beta [R,6], pos_src [sum n_src,3], pos_rcv [sum n_rcv,3], level [R] are
+    tensors (they may require grad).  The forward pass is the exact FP32 simulation; the backward
+    pass is the hand-written adjoint.  Returns (flat output, per-room views)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    R = len(rooms)
+    desc = []
+    for r in rooms:
+        ns = int(r["n_src"]) if "n_src" in r else int(np.asarray(r["pos_src"]).reshape(-1, 3).shape[0])
+        nr = int(r["n_rcv"]) if "n_rcv" in r else int(np.asarray(r["pos_rcv"]).reshape(-1, 3).shape[0])
+        d = dict(r)
+        d["beta"] = np.zeros(6, np.float32)
+        d["pos_src"] = np.zeros((ns, 3), np.float32)
+        d["pos_rcv"] = np.zeros((nr, 3), np.float32)
+        desc.append(d)
+    B = _BatchArgs(desc, fs, dev)
+    assert tuple(beta.shape) == (R, 6) and tuple(level.shape) == (R,)
+    assert tuple(pos_src.shape) == (int(B.n_src.sum()), 3) and tuple(pos_rcv.shape) == (int(B.n_rcv.sum()), 3)
+    cfg = {"dev": dev, "B": B, "spkr": _pat(spkr_pattern), "mic": _pat(mic_pattern), "fs": float(fs),
+           "c": float(c), "opt": options("fp32", noise, seed, call_offset, room_index, chunk_images=chunk_images,
+                                         tile_samples=tile_samples)}
+    out = _BatchFn.apply(beta, pos_src, pos_rcv, level, cfg)
+    return out, B.split(out)
+
+
+def simulate_rir_diff(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, level=None,
+                      spkr_pattern="omni", mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0,
+                      noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
+                      device=None):
+    """Differentiable simulateRIR of one room: beta [6], pos_src [S,3], pos_rcv [R,3] and level
+    (scalar tensor or None = 1) may require grad.  Returns [S, R, nSamples] float32 on the GPU."""
+    if level is None:
+        level = torch.ones((), dtype=torch.float32)
+    room = dict(room_sz=room_sz, nb_img=nb_img, Tmax=Tmax, Tdiff=Tdiff, orV_src=orV_src, orV_rcv=orV_rcv,
+                n_src=pos_src.shape[0], n_rcv=pos_rcv.shape[0])
+    out, views = simulate_batch_diff([room], beta.reshape(1, 6), pos_src, pos_rcv, level.reshape(1), fs,
+                                     spkr_pattern, mic_pattern, c, noise, seed, call_offset, room_index,
+                                     chunk_images, tile_samples, device)
+    return views[0]
 
 
 def trajectory(source_signal, rirs, timestamps=None, fs=None, device=None):

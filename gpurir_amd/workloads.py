@@ -102,6 +102,21 @@ def gradient_rooms(n_rooms=256, seed=0):
     return rooms
 
 
+def gradient_targets(rooms, seed=1):
+    """Targets of the inverse problem: each room's betas scaled by 1 + U(-0.2, 0.2) (clipped
+    to [0, 1]) and its source moved by N(0, 0.1^2) m per axis (kept 0.1 m inside the walls)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in rooms:
+        t = dict(r)
+        t["beta"] = np.clip(r["beta"] * (1.0 + rng.uniform(-0.2, 0.2, 6)), 0.0, 1.0).astype(np.float32)
+        L = np.asarray(r["room_sz"], np.float64)
+        ps = np.asarray(r["pos_src"], np.float64) + rng.normal(0.0, 0.1, np.shape(r["pos_src"]))
+        t["pos_src"] = np.clip(ps, 0.1, L - 0.1).astype(np.float32)
+        out.append(t)
+    return out
+
+
 def trajectory(seed=0, n_pts=128, n_signal=160000):
     rng = np.random.default_rng(seed)
     L = [8.0, 6.0, 3.5]

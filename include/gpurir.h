@@ -136,6 +136,25 @@ int gpurir_batch_plan_count_taps(gpurir_plan_t plan, long long* taps, void* stre
  * counterpart (measurement support). */
 int gpurir_batch_plan_profile(gpurir_plan_t plan, void* stream, float* ms_accumulate, float* ms_finalize);
 
+/* Adjoint of gpurir_simulate_batch_device (no reference counterpart: gpuRIR is not
+ * differentiable).  Given the forward output d_out (same arguments, options and noise
+ * stream) and d_grad_out = dL/d out (same layout), writes dL/d beta [n_rooms][6],
+ * dL/d pos_src and dL/d pos_rcv (rows as the position arrays) and dL/d level
+ * [n_rooms]; any gradient pointer may be NULL.  The windowed sinc and its derivative
+ * are evaluated exactly (FP32 taps, FP64 chain rule) whatever opt->precision is; the
+ * integer envelope window start is held fixed.  Deterministic. */
+int gpurir_simulate_batch_adjoint_device(int n_rooms, const float* room_sz, const float* beta, const int* nb_img,
+                                         const int* n_src, const int* n_rcv, const float* Tdiff, const float* Tmax,
+                                         const float* level, const float* d_pos_src, const float* d_pos_rcv,
+                                         const float* d_orV_src, const float* d_orV_rcv, int spkr_pattern,
+                                         int mic_pattern, float fs, float c, const gpurir_options_t* opt,
+                                         const float* d_out, const float* d_grad_out, const long long* out_offsets,
+                                         float* d_grad_beta, float* d_grad_src, float* d_grad_rcv,
+                                         float* d_grad_level, void* stream);
+/* The same for a batch plan, using the plan's output buffer as the forward output. */
+int gpurir_batch_plan_adjoint(gpurir_plan_t plan, const float* d_grad_out, float* d_grad_beta, float* d_grad_src,
+                              float* d_grad_rcv, float* d_grad_level, void* stream);
+
 /* gpu_conv: replaces gpuRIR_bind::gpu_conv (src/python_bind.cpp:90-119) ->
  * gpuRIR_cuda::cuda_convolutions (src/gpuRIR_cuda.cu:933-1023).  Host arrays:
  * segments [M_src][seg_len], rirs [M_src][M_rcv][rir_len],

@@ -46,6 +46,8 @@ def read_traffic(name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "lut"],
+                    help="sinc evaluation of the batch measured by --only-batch")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rooms", type=int, default=4096)
@@ -80,44 +82,54 @@ def main():
         del src, dst
 
     # ---------------- batched: 4096 uneven rooms, one graph replay per step
-    log(f"[bench] generating {args.rooms} uneven rooms")
-    rooms = W.uneven_rooms(n_rooms=args.rooms, seed=0)
-    plan = T.BatchPlan(rooms, fs, seed=1234, call_offset=0)
-    n_rirs = plan.n_rirs
-    log(f"[bench] plan: {n_rirs} RIRs, {plan.out.numel() * 4 / 1e9:.2f} GB output")
-    taps = plan.count_taps()
-    log(f"[bench] in-window taps W = {taps:.4e}")
-    for _ in range(args.warmup):
-        plan.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.run()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    rirs_per_s = n_rirs / (ms_step / 1e3)
-    log(f"[bench] batched: {ms_step:.3f} ms/step, {rirs_per_s:.4e} RIRs/s")
-
-    # dominant kernel: accumulation, timed with HIP events on its launch stream
-    prof = [plan.profile() for _ in range(5)]
-    ms_acc = float(np.mean([p[0] for p in prof]))
-    ms_fin = float(np.mean([p[1] for p in prof]))
-    achieved = FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12
-    log(f"[bench] accumulate {ms_acc:.3f} ms, finalize {ms_fin:.3f} ms, {achieved:.2f} TFLOP/s algorithmic")
-
-    # accuracy of the batch against the FP64 oracle (same Philox stream)
     from oracle import oracle as orc
 
-    plan.run()
-    torch.cuda.synchronize()
-    errs = []
-    for k in range(3):
-        r = rooms[k]
-        o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
-                             Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
-        errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
-    log(f"[bench] batch rel-L2 vs FP64 (3 rooms): {max(errs):.3e}")
+    log(f"[bench] generating {args.rooms} uneven rooms")
+    rooms = W.uneven_rooms(n_rooms=args.rooms, seed=0)
+
+    def measure_batch(precision):
+        plan = T.BatchPlan(rooms, fs, precision=precision, seed=1234, call_offset=0)
+        n_rirs = plan.n_rirs
+        taps = plan.count_taps()
+        for _ in range(args.warmup):
+            plan.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run()
+        torch.cuda.synchronize()
+        ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+        # dominant kernel: accumulation, timed with HIP events on its launch stream
+        prof = [plan.profile() for _ in range(5)]
+        ms_acc = float(np.mean([p[0] for p in prof]))
+        ms_fin = float(np.mean([p[1] for p in prof]))
+        # accuracy of the batch against the FP64 oracle (exact sinc, same Philox stream)
+        plan.run()
+        torch.cuda.synchronize()
+        errs = []
+        for k in range(3):
+            r = rooms[k]
+            o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
+                                 Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
+            errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
+        res = {"precision": precision, "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
+               "rirs_per_s": n_rirs / (ms_step / 1e3), "accumulate_ms": ms_acc, "finalize_ms": ms_fin,
+               "tflops_algorithmic": FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12, "rel_l2_vs_fp64": max(errs),
+               "out_gb": plan.out.numel() * 4 / 1e9}
+        log(f"[bench] batched {precision}: {n_rirs} RIRs, W = {taps:.4e} taps, {ms_step:.3f} ms/step, "
+            f"{res['rirs_per_s']:.4e} RIRs/s; accumulate {ms_acc:.3f} ms ({res['tflops_algorithmic']:.2f} TFLOP/s "
+            f"algorithmic), finalize {ms_fin:.3f} ms; rel-L2 vs FP64 (3 rooms) {max(errs):.3e}")
+        del plan
+        torch.cuda.empty_cache()
+        return res
+
+    modes = [args.precision] if args.only_batch else ["fp32", "fp16", "lut"]
+    batch = {m: measure_batch(m) for m in modes}
+    main_mode = args.precision if args.only_batch else "fp32"
+    B = batch[main_mode]
+    n_rirs, taps, ms_step, rirs_per_s = B["rirs"], B["taps"], B["ms_per_step"], B["rirs_per_s"]
+    ms_acc, ms_fin, achieved, errs = B["accumulate_ms"], B["finalize_ms"], B["tflops_algorithmic"], \
+        [B["rel_l2_vs_fp64"]]
 
     if args.only_batch:
         print(json.dumps({"metric": "batched RIRs/s (profiler run)", "value": rirs_per_s, "unit": "RIRs/s",
@@ -257,6 +269,7 @@ def main():
         "mid_room_ms": ms_mid,
         "finalize_ms": ms_fin,
         "trajectory": {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj},
+        "batched_modes": batch,
         "gradients": {"workload": "256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 src x 4 rcv, Tmax 0.5 s, "
                                   "Tdiff at 20 dB), MSE to targets from perturbed betas and sources; "
                                   "d/d(6 betas, source xyz, receiver xyz, level)",

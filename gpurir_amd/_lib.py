@@ -22,7 +22,7 @@ class Options(ctypes.Structure):
                 ("chunk_images", ctypes.c_int), ("tile_samples", ctypes.c_int)]
 
 
-PREC_FP32, PREC_LUT = 0, 1
+PREC_FP32, PREC_LUT, PREC_FP16 = 0, 1, 2
 NOISE_LOGISTIC, NOISE_GAUSSIAN = 0, 1
 
 # every symbol of include/gpurir.h with (restype, argtypes)

@@ -195,10 +195,9 @@ int gpurir_activate_mixed_precision(int activate) {
         std::printf("Disabling the lookup table.\n");
         g_lookup_table = false;
     }
-    if (activate) std::printf("This feature requires the packed-FP16 path, which this build does not provide.\n");
     std::fflush(stdout);
-    g_mixed_precision = false;
-    return 0;
+    g_mixed_precision = activate != 0;
+    return g_mixed_precision ? 1 : 0;
 }
 
 int gpurir_activate_lut(int activate) {
@@ -246,7 +245,7 @@ int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float
     std::memset(&o, 0, sizeof(o));
     {
         std::lock_guard<std::mutex> lk(g_state_mu);
-        o.precision = g_lookup_table ? GPURIR_PREC_LUT : GPURIR_PREC_FP32;
+        o.precision = g_mixed_precision ? GPURIR_PREC_FP16 : (g_lookup_table ? GPURIR_PREC_LUT : GPURIR_PREC_FP32);
         o.seed = g_seed;
         o.call_offset = g_next_call++;
     }

@@ -44,6 +44,13 @@ constexpr int NWAVE = WG / 64;
 constexpr int QCAP = 512;
 constexpr int QPAD = 16;  // queue entries read ahead past the live part (4 per wave)
 
+// Packed FP16 pair (v_pk_* f16 instructions) for the mixed-precision sinc.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ inline float h2_bits(half2_t v) { return __builtin_bit_cast(float, v); }
+__device__ inline half2_t bits_h2(float v) { return __builtin_bit_cast(half2_t, v); }
+__device__ inline half2_t h2(float lo, float hi) { return half2_t{(_Float16)lo, (_Float16)hi}; }
+
 struct AccArgs {
     const RoomDev* rooms;
     const Item* items;
@@ -112,6 +119,16 @@ __device__ inline void lds_add(long long* p, float v) {
     asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(v));  // floor(v + 0.5) in one instruction
     __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p), (unsigned long long)(long long)q,
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Mixed-precision mode: 32-bit fixed point (ds_add_u32, half the LDS traffic of the
+// 64-bit sums); the workgroup moves these sums into an FP32 accumulator before
+// they can overflow.
+__device__ inline void lds_add32(int* p, float v) {
+    int q;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(v));
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned*>(p), (unsigned)q, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // sin(pi f) for f in [0, 1]: odd Taylor polynomial of sin(pi g), g = min(f, 1 - f) <= 1/2,
@@ -273,13 +290,40 @@ __device__ inline void load_orient(const float* o, float out[4]) {
 // lanes share the interpolation weight and read consecutive entries of one phase
 // row of the table.
 // All taps of one queued image; PMAX is the exact number of 64-lane passes (entry c = (acc index bits, f, B/2 | amp, B/2 cos), s = B/2 sin).
+// Mixed precision (MODE == PREC_FP16): passes are paired and both taps of a pair are
+// computed with packed FP16 instructions (offset FMA, window FMAs, product; the
+// reciprocals are FP16 too), the per-image amplitude B (with the fixed-point scale)
+// is applied in FP32 when each tap is converted, and the sum is the same 64-bit
+// fixed point as the FP32 mode.  Entry c = (acc index bits, (f, f), (cos/2, cos/2), B),
+// s = (sin/2, sin/2), with the FP16 cos/sin of 2 pi f / Tw.
 template <int MODE, int PMAX, bool MASK>
 __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, const float2* lut, int lane, float4 c,
                                            float s, const float* jf, const float* cj, const float* sj,
-                                           const float* hj) {
+                                           const float* hj, const half2_t* J2, const half2_t* H2, const half2_t* C2,
+                                           const half2_t* S2) {
     long long* ap = acc + __float_as_int(c.x) + lane;
     const float f = c.y;
-    if (MODE == PREC_LUT) {
+    if (MODE == PREC_FP16) {
+        const half2_t f2 = bits_h2(c.y), hc = bits_h2(c.z), hs = bits_h2(s);
+        const float B = c.w;
+        const half2_t half = {(_Float16)0.5f, (_Float16)0.5f};
+#pragma unroll
+        for (int pp = 0; pp < (PMAX + 1) / 2; pp++) {
+            const half2_t xs = __builtin_elementwise_fma(-H2[pp], f2, J2[pp]);
+            const half2_t r = {(_Float16)__builtin_amdgcn_rcph(xs.x), (_Float16)__builtin_amdgcn_rcph(xs.y)};
+            const half2_t w = __builtin_elementwise_fma(C2[pp], hc, __builtin_elementwise_fma(S2[pp], hs, half));
+            const half2_t v = w * r;
+            // widening multiply by B in one v_fma_mix_f32 per tap
+            float v0 = fmaf((float)v.x, B, 0.0f), v1 = fmaf((float)v.y, B, 0.0f);
+            if (MASK) {
+                v0 = fabsf((float)xs.x) < a.halfTw ? v0 : 0.0f;
+                v1 = fabsf((float)xs.y) < a.halfTw ? v1 : 0.0f;
+            }
+            int* ap32 = reinterpret_cast<int*>(acc) + __float_as_int(c.x) + lane;
+            lds_add32(ap32 + 64 * (2 * pp), v0);
+            if (2 * pp + 1 < PMAX) lds_add32(ap32 + 64 * (2 * pp + 1), v1);
+        }
+    } else if (MODE == PREC_LUT) {
         const float amp = c.z;
         const float m16 = -16.0f * f;
         const float k0 = floorf(m16);
@@ -345,6 +389,18 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         sj[p] = c.z;
         hj[p] = c.w;
     }
+    // FP16 mode: the same constants, paired by pass (a lone last pass is paired with itself)
+    half2_t J2[(PMAX + 1) / 2], H2[(PMAX + 1) / 2], C2[(PMAX + 1) / 2], S2[(PMAX + 1) / 2];
+    if (MODE == PREC_FP16) {
+#pragma unroll
+        for (int pp = 0; pp < (PMAX + 1) / 2; pp++) {
+            const int lo = 2 * pp, hi = 2 * pp + 1 < PMAX ? 2 * pp + 1 : 2 * pp;
+            J2[pp] = h2(jf[lo], jf[hi]);
+            H2[pp] = h2(hj[lo], hj[hi]);
+            C2[pp] = h2(cj[lo], cj[hi]);
+            S2[pp] = h2(sj[lo], sj[hi]);
+        }
+    }
     float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
     if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
     if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
@@ -362,7 +418,10 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         const float dx = ps[0] - pr[0], dy = ps[1] - pr[1], dz = ps[2] - pr[2];
         const float bound = __int_as_float(qcount[1]) * __int_as_float(qcount[2]) * __int_as_float(qcount[3]) *
                             __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
-        int E = (bound > 0.0f && bound < 3.0e38f) ? 29 - ilogbf(bound) : 0;
+        // |tap| * 2^E < 2^30 (64-bit sums), or < 2^21 in the 32-bit mixed-precision sums,
+        // which then take up to 2^10 taps per word between two moves into FP32
+        const int top = MODE == PREC_FP16 ? 20 : 29;
+        int E = (bound > 0.0f && bound < 3.0e38f) ? top - ilogbf(bound) : 0;
         E = min(max(E, -120), 120);
         scale = ldexpf(1.0f, E);
         inv_scale = ldexpf(1.0f, -E);
@@ -373,6 +432,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     const int hi_n0 = it.t_hi - 1 - a.J0;
     const int acc_shift = J1 - it.t_lo;    // acc index of tap j = n0 + acc_shift + (j - J0)
 
+    int since = 0;  // images queued since the last move of the 32-bit sums (mixed precision)
     for (int r0 = it.i0; r0 < it.i1; r0 += QCAP) {
         if (tid == 0) *qcount = 0;
         __syncthreads();
@@ -402,6 +462,22 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
                     const int q = base + pre;
                     if (MODE == PREC_LUT) {
                         q4[q] = make_float4(__int_as_float(n0 + acc_shift), f, amp * scale, 0.0f);
+                    } else if (MODE == PREC_FP16) {
+                        // f in [2^-12, 1 - 2^-11] keeps 1/x' inside the FP16 range at j = 0 and
+                        // j = 1 (an f rounded up to 1.0 would make x' = 0); B uses the FP16 f
+                        const float fh = (float)(_Float16)fminf(fmaxf(f, 0x1p-12f), 1.0f - 0x1p-11f);
+                        const float B = amp * scale * sinpi01(fh) * 0.31830988618379067f;
+                        const float ph = fh * a.inv_tw2;
+                        float sph, cph;
+                        if (a.inv_tw2 <= 0.0625f) {
+                            sincospi_small(ph, &sph, &cph);
+                        } else {
+                            sph = sinpif(ph);
+                            cph = cospif(ph);
+                        }
+                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), h2_bits(h2(fh, fh)),
+                                            h2_bits(h2(0.5f * cph, 0.5f * cph)), B);
+                        qS[q] = h2_bits(h2(0.5f * sph, 0.5f * sph));
                     } else {
                         // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
                         // 2^-30 samples changes no tap by more than ~1e-9 relative.
@@ -430,6 +506,21 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         // through per-lane pointers with immediate offsets; reads past the live part
         // stay inside the queue's padding and their values are never used.
         const int nq = __builtin_amdgcn_readfirstlane(*qcount);
+        if (MODE == PREC_FP16) {
+            // a word takes at most one tap per image: move the 32-bit sums into the FP32
+            // accumulator before this round could take them past 2^10 taps
+            if (since + nq > 1024) {
+                int* lo = reinterpret_cast<int*>(acc);
+                float* hi = reinterpret_cast<float*>(lo + a.acc_len);
+                for (int e = tid; e < a.acc_len; e += WG) {
+                    hi[e] += (float)lo[e] * inv_scale;
+                    lo[e] = 0;
+                }
+                __syncthreads();
+                since = 0;
+            }
+            since += nq;
+        }
         if (wave < nq) {
             const float4* pq = q4 + (tid >> 6);
             const float* pS = qS + (tid >> 6);
@@ -438,8 +529,8 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
             for (int e = wave;;) {
                 const float4 yA = pq[2 * NWAVE], yB = pq[3 * NWAVE];
                 const float syA = pS[2 * NWAVE], syB = pS[3 * NWAVE];
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xA, sxA, jf, cj, sj, hj);
-                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xB, sxB, jf, cj, sj, hj);
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xA, sxA, jf, cj, sj, hj, J2, H2, C2, S2);
+                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xB, sxB, jf, cj, sj, hj, J2, H2, C2, S2);
                 e += 2 * NWAVE;
                 if (e >= nq) break;
                 pq += 2 * NWAVE;
@@ -448,8 +539,8 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
                 xB = pq[3 * NWAVE];
                 sxA = pS[2 * NWAVE];
                 sxB = pS[3 * NWAVE];
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yA, syA, jf, cj, sj, hj);
-                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yB, syB, jf, cj, sj, hj);
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yA, syA, jf, cj, sj, hj, J2, H2, C2, S2);
+                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yB, syB, jf, cj, sj, hj, J2, H2, C2, S2);
                 e += 2 * NWAVE;
                 if (e >= nq) break;
                 pq += 2 * NWAVE;
@@ -468,7 +559,13 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         dst = a.out + R.out_off + (long long)it.rir * R.nSamples + it.t_lo;
     else
         dst = a.part + R.part_off + ((long long)it.rir * R.n_chunks + it.chunk) * R.nISM + it.t_lo;
-    for (int t = tid; t < nt; t += WG) dst[t] = (float)((double)src[t] * (double)inv_scale);
+    if (MODE == PREC_FP16) {
+        const int* lo = reinterpret_cast<const int*>(acc) + (J1 - a.J0);
+        const float* hi = reinterpret_cast<const float*>(reinterpret_cast<const int*>(acc) + a.acc_len) + (J1 - a.J0);
+        for (int t = tid; t < nt; t += WG) dst[t] = hi[t] + (float)lo[t] * inv_scale;
+    } else {
+        for (int t = tid; t < nt; t += WG) dst[t] = (float)((double)src[t] * (double)inv_scale);
+    }
 }
 
 __device__ inline void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1,
@@ -1384,7 +1481,8 @@ struct SimLaunch {
 };
 
 int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLaunch& S) {
-    GPURIR_CHECK(opt.precision == PREC_FP32 || opt.precision == PREC_LUT, "unknown precision mode");
+    GPURIR_CHECK(opt.precision == PREC_FP32 || opt.precision == PREC_LUT || opt.precision == PREC_FP16,
+                 "unknown precision mode");
     GPURIR_CHECK(opt.fs > 0.0f && opt.c > 0.0f, "fs and c must be positive");
     GPURIR_CHECK(!rooms.empty(), "no rooms");
     for (const RoomHost& h : rooms) {
@@ -1496,6 +1594,8 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     }
         if (S.prec == PREC_LUT) {
             GPURIR_ACC_PICK(PREC_LUT)
+        } else if (S.prec == PREC_FP16) {
+            GPURIR_ACC_PICK(PREC_FP16)
         } else {
             GPURIR_ACC_PICK(PREC_FP32)
         }

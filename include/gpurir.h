@@ -35,6 +35,7 @@ extern "C" {
 /* Evaluation of the windowed sinc (src/gpuRIR_cuda.h:29-33 flags mixed_precision / lookup_table) */
 #define GPURIR_PREC_FP32 0 /* exact FP32 sinc (reference: both flags off)            */
 #define GPURIR_PREC_LUT 1  /* 1/16-sample table + linear interpolation (lookup_table) */
+#define GPURIR_PREC_FP16 2 /* packed FP16 windowed sinc, 64-bit fixed-point sums (mixed_precision) */
 
 /* Diffuse-tail noise */
 #define GPURIR_NOISE_LOGISTIC 0 /* unit-variance logistic, as src/gpuRIR_cuda.cu:454-455 */
@@ -64,9 +65,9 @@ int gpurir_n_samples(float T, float fs);
 /* Toggles of the drop-in state, mutually exclusive as in the reference; return the new
  * state of the flag.  Replace gpuRIR_cuda::activate_mixed_precision / activate_lut
  * (src/gpuRIR_cuda.cu:1050-1073) bound as activate_mixed_precision_bind /
- * activate_lut_bind (src/python_bind.cpp:121-128,140-142).  Mixed precision is not provided
- * by this library: activating it prints the reference's conflict message (if the
- * table was active), disables the table and reports 0. */
+ * activate_lut_bind (src/python_bind.cpp:121-128,140-142).  Mixed precision selects
+ * GPURIR_PREC_FP16 for the plain entry point; its FP16 arithmetic is the packed
+ * instructions of gfx950 (the reference's Pascal-architecture check has no equivalent). */
 int gpurir_activate_mixed_precision(int activate);
 int gpurir_activate_lut(int activate);
 

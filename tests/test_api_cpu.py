@@ -92,12 +92,19 @@ def test_library_exports_every_header_symbol():
 
 
 def test_toggles_are_mutually_exclusive(capfd):
+    """src/gpuRIR_cuda.cu:1050-1073: each toggle disables the other, with the same messages."""
     lib = _lib.load()
     assert lib.gpurir_activate_lut(1) == 1
-    lib.gpurir_activate_mixed_precision(1)
+    assert lib.gpurir_activate_mixed_precision(1) == 1
     out = capfd.readouterr().out
+    assert "The mixed precision implementation is not compatible with the lookup table." in out
     assert "Disabling the lookup table." in out
     assert lib.gpurir_activate_lut(1) == 1
+    out = capfd.readouterr().out
+    assert "The lookup table is not compatible with the mixed precision implementation." in out
+    assert "Disabling the mixed precision implementation." in out
+    assert lib.gpurir_activate_mixed_precision(0) == 0
+    assert capfd.readouterr().out == ""
     gpuRIR.activateLUT(True)
     assert gpuRIR.activateMixedPrecision(False) is None
 

@@ -193,3 +193,50 @@ def test_batch_equals_single_rooms_and_graph_replay():
         x = torch.randn(1000, device=plan.out.device).sum().item()  # eager work between replays
         assert np.isfinite(x)
         assert rel_l2(plan.out.cpu().numpy(), ref.cpu().numpy()) < 2e-6, f"replay {rep}"
+
+
+# ---- mixed precision: packed FP16 windowed sinc, 64-bit fixed-point sums; bar 1e-3 vs FP64
+FP16_TOL = 1e-3
+
+
+def test_mid_room_fp16_packed():
+    cfg = W.mid_room()
+    g, o = run_both(cfg, precision="fp16")
+    e = check_parity(g, o, 3200, tol=FP16_TOL)
+    print(f"mid room FP16 rel-L2 {e:.3e}")
+
+
+@pytest.mark.parametrize("fs", [16000.0, 44100.0, 8000.0])
+def test_fp16_sampling_rates(fs):
+    cfg = W.plain_room()
+    cfg["fs"] = fs
+    cfg["Tmax"] = 0.12
+    cfg["Tdiff"] = 0.06
+    cfg["nb_img"] = gpuRIR.t2n(0.06, cfg["room_sz"])
+    g, o = run_both(cfg, precision="fp16")
+    e = check_parity(g, o, orc.n_samples(0.06, fs), tol=FP16_TOL)
+    print(f"fs {fs:.0f} FP16 rel-L2 {e:.3e}")
+
+
+def test_fp16_drop_in_toggle(capfd):
+    cfg = W.plain_room()
+    args = dict(room_sz=cfg["room_sz"], beta=cfg["beta"], pos_src=cfg["pos_src"], pos_rcv=cfg["pos_rcv"],
+                nb_img=cfg["nb_img"], Tmax=cfg["Tmax"], fs=cfg["fs"], Tdiff=cfg["Tdiff"])
+    gpuRIR.activateLUT(True)
+    gpuRIR.activateMixedPrecision(True)
+    assert "Disabling the lookup table." in capfd.readouterr().out
+    gpuRIR.set_seed(1234, 0)
+    g = gpuRIR.simulateRIR(**args)
+    o = orc.simulate_rir(**args, call_offset=0)
+    check_parity(g, o, 2800, tol=FP16_TOL)
+    gpuRIR.activateLUT(True)
+
+
+def test_fp16_batch_uneven_rooms():
+    rooms = W.uneven_rooms(n_rooms=8, seed=11)
+    _, views = T.simulate_batch(rooms, W.FS, precision="fp16")
+    torch.cuda.synchronize()
+    for k, r in enumerate(rooms):
+        o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
+                             Tdiff=r["Tdiff"], room_index=k)
+        check_parity(views[k].cpu().numpy(), o, orc.n_samples(r["Tdiff"], W.FS), tol=FP16_TOL)

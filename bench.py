@@ -46,6 +46,7 @@ def read_traffic(name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the planner's default work-item shape")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "lut"],
                     help="sinc evaluation of the batch measured by --only-batch")
     ap.add_argument("--steps", type=int, default=20)
@@ -91,6 +92,9 @@ def main():
         plan = T.BatchPlan(rooms, fs, precision=precision, seed=1234, call_offset=0)
         n_rirs = plan.n_rirs
         taps = plan.count_taps()
+        tune = plan.autotune() if not args.no_autotune else None
+        if tune:
+            log(f"[bench] autotuned {precision}: {tune}")
         for _ in range(args.warmup):
             plan.run()
         torch.cuda.synchronize()
@@ -112,7 +116,7 @@ def main():
             o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
                                  Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
             errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
-        res = {"precision": precision, "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
+        res = {"precision": precision, "autotune": tune, "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
                "rirs_per_s": n_rirs / (ms_step / 1e3), "accumulate_ms": ms_acc, "finalize_ms": ms_fin,
                "tflops_algorithmic": FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12, "rel_l2_vs_fp64": max(errs),
                "out_gb": plan.out.numel() * 4 / 1e9}

@@ -19,7 +19,7 @@ class Options(ctypes.Structure):
     """gpurir_options_t"""
     _fields_ = [("precision", ctypes.c_int), ("noise", ctypes.c_int), ("seed", ctypes.c_ulonglong),
                 ("call_offset", ctypes.c_uint), ("room_index", ctypes.c_uint), ("level", ctypes.c_float),
-                ("chunk_images", ctypes.c_int), ("tile_samples", ctypes.c_int)]
+                ("chunk_images", ctypes.c_int), ("tile_samples", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
 PREC_FP32, PREC_LUT, PREC_FP16 = 0, 1, 2
@@ -61,6 +61,8 @@ _SIG = {
     "gpurir_batch_plan_n_rirs": (ctypes.c_longlong, [ctypes.c_void_p]),
     "gpurir_batch_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "gpurir_batch_plan_count_taps": (ctypes.c_int, [ctypes.c_void_p, c_ll_p, ctypes.c_void_p]),
+    "gpurir_batch_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int_p, c_int_p, c_int_p,
+                                                  c_float_p]),
     "gpurir_batch_plan_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_float_p, c_float_p]),
     "gpurir_simulate_batch_adjoint_device": (ctypes.c_int, [ctypes.c_int, c_float_p, c_float_p, c_int_p, c_int_p,
                                                             c_int_p, c_float_p, c_float_p, c_float_p,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -178,7 +179,138 @@ struct gpurir_plan {
     AdjLaunch* adj = nullptr;
     void* d_adj_plan = nullptr;
     void* d_adj_ws = nullptr;
+    unsigned* d_counters = nullptr;  // persistent accumulation (chosen by the autotuner)
 };
+
+// ---- autotuner: work-item shape (images per item, samples per tile) and launch form
+namespace {
+struct TuneChoice {
+    int chunk = 0, tile = 0, persistent = 0;
+    float ms = 0.0f;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune_cache;
+
+void drop_graph(gpurir_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+// Replace the plan's launch by one prepared with so (and the persistent form if asked).
+int plan_rebuild(gpurir_plan* p, const SimOptions& so, bool persistent) {
+    SimLaunch* s = sim_new();
+    if (int rc = sim_prepare(p->rooms, so, *s)) {
+        sim_delete(s);
+        return rc;
+    }
+    void *dp = nullptr, *dpart = nullptr;
+    if (hipMalloc(&dp, sim_plan_bytes(*s)) != hipSuccess ||
+        (sim_partial_bytes(*s) > 0 && hipMalloc(&dpart, sim_partial_bytes(*s)) != hipSuccess)) {
+        (void)hipGetLastError();
+        if (dp) (void)hipFree(dp);
+        sim_delete(s);
+        set_error("out of device memory while re-planning");
+        return -3;
+    }
+    if (hipMemcpy(dp, sim_blob(*s), sim_plan_bytes(*s), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dp);
+        if (dpart) (void)hipFree(dpart);
+        sim_delete(s);
+        set_error("plan upload failed");
+        return -2;
+    }
+    if (persistent && !p->d_counters) {
+        if (hipMalloc(reinterpret_cast<void**>(&p->d_counters), 2 * sizeof(unsigned)) != hipSuccess ||
+            hipMemset(p->d_counters, 0, 2 * sizeof(unsigned)) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(dp);
+            if (dpart) (void)hipFree(dpart);
+            sim_delete(s);
+            set_error("out of device memory for the work counters");
+            return -3;
+        }
+    }
+    sim_set_counters(*s, persistent ? p->d_counters : nullptr);
+    drop_graph(p);
+    if (p->d_plan) (void)hipFree(p->d_plan);
+    if (p->d_part) (void)hipFree(p->d_part);
+    sim_delete(p->sim);
+    p->sim = s;
+    p->d_plan = dp;
+    p->d_part = static_cast<float*>(dpart);
+    p->so = so;
+    return 0;
+}
+
+// Plan the batch with so, run it eagerly three times and keep the fastest of the
+// last two (HIP events on the stream).  Writes the plan's output buffer.
+int time_config(gpurir_plan* p, const SimOptions& so, bool persistent, hipStream_t st, float* ms) {
+    *ms = -1.0f;
+    SimLaunch* s = sim_new();
+    if (sim_prepare(p->rooms, so, *s) != 0) {  // e.g. a tile over the LDS budget: not a candidate
+        sim_delete(s);
+        return 0;
+    }
+    void *dp = nullptr, *dpart = nullptr, *dc = nullptr;
+    int rc = 0;
+    if (hipMalloc(&dp, sim_plan_bytes(*s)) != hipSuccess ||
+        (sim_partial_bytes(*s) > 0 && hipMalloc(&dpart, sim_partial_bytes(*s)) != hipSuccess) ||
+        hipMalloc(&dc, 2 * sizeof(unsigned)) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = 1;  // not enough memory for this candidate: skip it
+    }
+    if (!rc && (hipMemcpy(dp, sim_blob(*s), sim_plan_bytes(*s), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(dc, 0, 2 * sizeof(unsigned)) != hipSuccess))
+        rc = -2;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = -2;
+    if (!rc) {
+        sim_set_counters(*s, persistent ? static_cast<unsigned*>(dc) : nullptr);
+        float best = 1e30f;
+        for (int k = 0; k < 3 && !rc; k++) {
+            if (hipEventRecord(e0, st) != hipSuccess) rc = -2;
+            if (!rc) rc = sim_enqueue(*s, dp, static_cast<float*>(dpart), p->pos_src, p->pos_rcv, p->orv_src,
+                                      p->orv_rcv, p->out, st);
+            if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = -2;
+            if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = -2;
+            float t = 0.0f;
+            if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = -2;
+            if (!rc && k > 0) best = std::min(best, t);
+        }
+        if (!rc) *ms = best;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dp) (void)hipFree(dp);
+    if (dpart) (void)hipFree(dpart);
+    if (dc) (void)hipFree(dc);
+    sim_delete(s);
+    if (rc == 1) return 0;
+    if (rc) set_error("HIP error while timing a plan candidate");
+    return rc;
+}
+
+std::string tune_key(const gpurir_plan* p) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    std::string arch = "?";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+    long long images = 0, rirs = 0;
+    int nism_max = 0;
+    for (const RoomHost& h : p->rooms) {
+        const long long M = (long long)h.n_src * h.n_rcv;
+        images += M * h.nb[0] * h.nb[1] * h.nb[2];
+        rirs += M;
+        nism_max = std::max(nism_max, n_samples_rule(h.Tdiff, p->so.fs));
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s|p%d|fs%g|r%zu|m%lld|i%lld|t%d|d%d,%d", arch.c_str(), p->so.precision,
+                  (double)p->so.fs, p->rooms.size(), rirs, images, nism_max, p->so.spkr, p->so.mic);
+    return buf;
+}
+}  // namespace
 
 extern "C" {
 
@@ -396,6 +528,16 @@ int gpurir_batch_plan_create(int n_rooms, const float* room_sz, const float* bet
     p->out_elems = elems;
     p->rooms = rooms;
     p->so = so;
+    if (opt && opt->persistent) {
+        if (hipMalloc(reinterpret_cast<void**>(&p->d_counters), 2 * sizeof(unsigned)) != hipSuccess ||
+            hipMemset(p->d_counters, 0, 2 * sizeof(unsigned)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (p->d_counters) (void)hipFree(p->d_counters);
+            set_error("out of device memory for the work counters");
+            return fail(-3);
+        }
+        sim_set_counters(*p->sim, p->d_counters);
+    }
     *plan = p;
     return 0;
 }
@@ -518,8 +660,63 @@ int gpurir_simulate_batch_adjoint_device(int n_rooms, const float* room_sz, cons
                           d_grad_src, d_grad_rcv, d_grad_level, static_cast<hipStream_t>(stream));
 }
 
+int gpurir_batch_plan_autotune(gpurir_plan_t p, void* stream, int* chunk_images, int* tile_samples,
+                               int* persistent, float* ms) {
+    GPURIR_CHECK(p != nullptr, "null plan");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const std::string key = tune_key(p);
+    TuneChoice best;
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto f = g_tune_cache.find(key);
+        if (f != g_tune_cache.end()) {
+            best = f->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        SimOptions base = p->so;
+        best.ms = 1e30f;
+        auto consider = [&](int chunk, int tile, int pers) -> int {
+            SimOptions so = base;
+            so.chunk_images = chunk;
+            so.tile_samples = tile;
+            float t = -1.0f;
+            if (int rc = time_config(p, so, pers != 0, st, &t)) return rc;
+            if (t > 0.0f && t < best.ms) {
+                best.chunk = chunk;
+                best.tile = tile;
+                best.persistent = pers;
+                best.ms = t;
+            }
+            return 0;
+        };
+        // samples per tile (planner default first), then images per item, then the launch form
+        for (int tile : {0, 1024, 2048, 3072, 4096, 6144})
+            if (int rc = consider(0, tile, 0)) return rc;
+        const int tile = best.tile;
+        for (int chunk : {8192, 16384, 32768, 65536})
+            if (int rc = consider(chunk, tile, 0)) return rc;
+        if (int rc = consider(best.chunk, best.tile, 1)) return rc;
+        GPURIR_CHECK(best.ms < 1e29f, "no plan candidate could be run");
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    SimOptions so = p->so;
+    so.chunk_images = best.chunk;
+    so.tile_samples = best.tile;
+    if (int rc = plan_rebuild(p, so, best.persistent != 0)) return rc;
+    if (chunk_images) *chunk_images = best.chunk;
+    if (tile_samples) *tile_samples = best.tile;
+    if (persistent) *persistent = best.persistent;
+    if (ms) *ms = best.ms;
+    return 0;
+}
+
 int gpurir_batch_plan_destroy(gpurir_plan_t p) {
     if (!p) return 0;
+    if (p->d_counters) (void)hipFree(p->d_counters);
     if (p->d_adj_plan) (void)hipFree(p->d_adj_plan);
     if (p->d_adj_ws) (void)hipFree(p->d_adj_ws);
     if (p->adj) adj_delete(p->adj);

@@ -353,10 +353,10 @@ __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, con
     }
 }
 
+// One work item: stages (a) and (b) for an image chunk of one RIR over one time tile.
 template <int MODE, int PMAX, bool MASK>
-__global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const Item it = a.items[blockIdx.x];
+__device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsigned char* smem_raw) {
+    const Item it = a.items[item];
     const RoomK R = load_room(a.rooms + it.room);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -565,6 +565,40 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
         for (int t = tid; t < nt; t += WG) dst[t] = hi[t] + (float)lo[t] * inv_scale;
     } else {
         for (int t = tid; t < nt; t += WG) dst[t] = (float)((double)src[t] * (double)inv_scale);
+    }
+}
+
+// One workgroup per work item.
+template <int MODE, int PMAX, bool MASK>
+__global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    accumulate_item<MODE, PMAX, MASK>(a, blockIdx.x, smem_raw);
+}
+
+// Persistent form: a grid sized to the resident workgroups takes items from a
+// counter in order (longest first).  counters[0] is the next item, counters[1] the
+// number of workgroups done; the last one resets both for the next launch (every
+// other workgroup has made its final fetch by then), so a captured graph can be
+// replayed without any memset node.
+template <int MODE, int PMAX, bool MASK>
+__global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a, int n_items,
+                                                                       unsigned* counters) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(counters, 1u);
+        __syncthreads();
+        const int item = next;
+        __syncthreads();  // every thread has read next; the item may overwrite all of LDS
+        if (item >= n_items) break;
+        accumulate_item<MODE, PMAX, MASK>(a, item, smem_raw);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(counters + 1, 1u) == gridDim.x - 1) {
+            atomicExch(counters, 0u);
+            atomicExch(counters + 1, 0u);
+        }
     }
 }
 
@@ -1478,6 +1512,7 @@ struct SimLaunch {
     size_t lds = 0;
     int acc_len = 0, ntab = 0;
     SimOptions opt;
+    unsigned* counters = nullptr;  // persistent launch: device item counter pair (zero between launches)
 };
 
 int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLaunch& S) {
@@ -1601,10 +1636,47 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         }
 #undef GPURIR_ACC_PICK
 #undef GPURIR_ACC_FN
+        if (S.counters) {
+            // the same instantiation in its persistent form
+#define GPURIR_ACC_FN(M, P) \
+    (mk ? reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, P, true>) \
+        : reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, P, false>))
+#define GPURIR_ACC_PICK(M)                                         \
+    switch (S.T.n_pass) {                                          \
+        case 1: fn = GPURIR_ACC_FN(M, 1); break;                   \
+        case 2: fn = GPURIR_ACC_FN(M, 2); break;                   \
+        case 3: fn = GPURIR_ACC_FN(M, 3); break;                   \
+        case 4: fn = GPURIR_ACC_FN(M, 4); break;                   \
+        case 6: fn = GPURIR_ACC_FN(M, 6); break;                   \
+        default: fn = GPURIR_ACC_FN(M, 12); break;                 \
+    }
+            if (S.prec == PREC_LUT) {
+                GPURIR_ACC_PICK(PREC_LUT)
+            } else if (S.prec == PREC_FP16) {
+                GPURIR_ACC_PICK(PREC_FP16)
+            } else {
+                GPURIR_ACC_PICK(PREC_FP32)
+            }
+#undef GPURIR_ACC_PICK
+#undef GPURIR_ACC_FN
+        }
         if (S.lds > 64 * 1024)
             GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds));
-        void* args[] = {&a};
-        GPURIR_HIP(hipLaunchKernel(fn, grid, dim3(WG), args, S.lds, stream));
+        if (S.counters) {
+            int per_cu = 0, dev = 0, n_cu = 0;
+            GPURIR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, S.lds));
+            GPURIR_HIP(hipGetDevice(&dev));
+            GPURIR_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+            const long long want = (long long)std::max(per_cu, 1) * n_cu;
+            int n_items = (int)S.P.items.size();
+            const unsigned g = (unsigned)std::min<long long>(want, n_items);
+            unsigned* cnt = S.counters;
+            void* args[] = {&a, &n_items, &cnt};
+            GPURIR_HIP(hipLaunchKernel(fn, dim3(g), dim3(WG), args, S.lds, stream));
+        } else {
+            void* args[] = {&a};
+            GPURIR_HIP(hipLaunchKernel(fn, grid, dim3(WG), args, S.lds, stream));
+        }
     }
     if (!(stage & 2)) return 0;
     if (!S.P.sum_blocks.empty()) {
@@ -1635,6 +1707,8 @@ int sim_enqueue(const SimLaunch& S, const void* d_plan, float* d_part, const flo
 }
 
 SimLaunch* sim_new() { return new SimLaunch(); }
+void sim_set_counters(SimLaunch& S, unsigned* d_counters) { S.counters = d_counters; }
+const SimOptions& sim_options_of(const SimLaunch& S) { return S.opt; }
 void sim_delete(SimLaunch* s) { delete s; }
 const void* sim_blob(const SimLaunch& S) { return S.blob.data(); }
 void sim_set_stream_opts(SimLaunch& S, unsigned long long seed, unsigned call_offset) {

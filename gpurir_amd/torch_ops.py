@@ -37,7 +37,7 @@ def _stream(dev):
 
 
 def options(precision="fp32", noise="logistic", seed=1234, call_offset=0, room_index=0, level=1.0,
-            chunk_images=0, tile_samples=0):
+            chunk_images=0, tile_samples=0, persistent=False):
     o = _lib.Options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.noise = NOISES[noise] if isinstance(noise, str) else int(noise)
@@ -47,6 +47,7 @@ def options(precision="fp32", noise="logistic", seed=1234, call_offset=0, room_i
     o.level = float(level)
     o.chunk_images = int(chunk_images)
     o.tile_samples = int(tile_samples)
+    o.persistent = int(bool(persistent))
     return o
 
 
@@ -174,14 +175,14 @@ class BatchPlan:
 
     def __init__(self, rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
                  noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
-                 device=None):
+                 persistent=False, device=None):
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dev = dev
         self.B = _BatchArgs(rooms, fs, dev)
         self.out = torch.empty(self.B.total, dtype=torch.float32, device=dev)
         self.rirs = self.B.split(self.out)
         o = options(precision, noise, seed, call_offset, room_index, chunk_images=chunk_images,
-                    tile_samples=tile_samples)
+                    tile_samples=tile_samples, persistent=persistent)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().gpurir_batch_plan_create(
             *self.B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),
@@ -195,6 +196,14 @@ class BatchPlan:
     def run(self):
         _lib.check(_lib.load().gpurir_batch_plan_run(self._h, _stream(self.dev)))
         return self.out
+
+    def autotune(self):
+        """Time candidate work-item shapes and launch forms, keep the fastest (cached per
+        process).  Returns dict(chunk_images, tile_samples, persistent, ms)."""
+        c, t, p, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _lib.check(_lib.load().gpurir_batch_plan_autotune(self._h, _stream(self.dev), ctypes.byref(c),
+                                                          ctypes.byref(t), ctypes.byref(p), ctypes.byref(ms)))
+        return {"chunk_images": c.value, "tile_samples": t.value, "persistent": bool(p.value), "ms": ms.value}
 
     def count_taps(self):
         """In-window taps (RIR, image, t) of the plan: its algorithmic work."""

@@ -50,6 +50,7 @@ typedef struct gpurir_options {
     float level;                /* diffuse-tail multiplier (1 = reference)               */
     int chunk_images;           /* images per work item, 0 = planner default             */
     int tile_samples;           /* ISM samples per LDS tile, 0 = planner default         */
+    int persistent;             /* batch plans: 1 = persistent accumulation launch       */
 } gpurir_options_t;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -128,6 +129,15 @@ int gpurir_batch_plan_run(gpurir_plan_t plan, void* stream);
 long long gpurir_batch_plan_out_elems(gpurir_plan_t plan);
 long long gpurir_batch_plan_n_rirs(gpurir_plan_t plan);
 int gpurir_batch_plan_destroy(gpurir_plan_t plan);
+/* Autotuner (no reference counterpart): times the plan's launch eagerly on the stream
+ * for candidate work-item shapes -- ISM samples per LDS tile, images per work item --
+ * and for the persistent launch form, keeps the fastest and re-plans with it (the next
+ * run captures a new graph).  Choices are cached per process by GPU architecture,
+ * precision and workload signature.  Synchronous; overwrites the plan's output.
+ * Outputs (any may be NULL): the chosen images per item and samples per tile (0 =
+ * planner default), 1 if the persistent form was chosen, and its time in ms. */
+int gpurir_batch_plan_autotune(gpurir_plan_t plan, void* stream, int* chunk_images, int* tile_samples,
+                               int* persistent, float* ms);
 /* Algorithmic work of the plan: the number of (RIR, image, t) with 0 <= t < nISM and
  * |t - tau| < Tw/2 (Tw/2 rounded down in table mode).  Synchronous.  No reference
  * counterpart (measurement support). */

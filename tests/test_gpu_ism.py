@@ -240,3 +240,27 @@ def test_fp16_batch_uneven_rooms():
         o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
                              Tdiff=r["Tdiff"], room_index=k)
         check_parity(views[k].cpu().numpy(), o, orc.n_samples(r["Tdiff"], W.FS), tol=FP16_TOL)
+
+
+def test_persistent_plan_and_autotune():
+    """The persistent launch gives the same bits as one workgroup per item, also on graph
+    replays (its work counters are reset by the kernel itself); the autotuner re-plans
+    to an equivalent result and caches its choice."""
+    rooms = W.uneven_rooms(n_rooms=24, seed=13)
+    a = T.BatchPlan(rooms, W.FS)
+    b = T.BatchPlan(rooms, W.FS, persistent=True)
+    ya = a.run().clone()
+    for _ in range(3):
+        yb = b.run()
+        torch.cuda.synchronize()
+        assert torch.equal(yb, ya)
+    choice = a.autotune()
+    assert choice["ms"] > 0
+    yc = a.run().clone()
+    torch.cuda.synchronize()
+    assert rel_l2(yc.cpu().numpy(), ya.cpu().numpy()) < 1e-6
+    r = rooms[0]
+    o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
+                         Tdiff=r["Tdiff"], room_index=0)
+    check_parity(a.rirs[0].cpu().numpy(), o, orc.n_samples(r["Tdiff"], W.FS))
+    assert T.BatchPlan(rooms, W.FS).autotune() == choice

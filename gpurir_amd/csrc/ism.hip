@@ -606,8 +606,11 @@ __device__ inline void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsi
                                      unsigned out[4]) {
 #pragma unroll
     for (int rr = 0; rr < 10; rr++) {
-        const unsigned lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const unsigned lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        // one 32x32->64 multiply (v_mad_u64_u32) per product instead of mul_lo + mul_hi
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+        const unsigned lo0 = (unsigned)p0, hi0 = (unsigned)(p0 >> 32);
+        const unsigned lo1 = (unsigned)p1, hi1 = (unsigned)(p1 >> 32);
         const unsigned n0 = hi1 ^ c1 ^ k0;
         const unsigned n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
@@ -626,11 +629,13 @@ __device__ inline void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsi
 // Noise value u (0..3) of one Philox block.  Logistic (the reference's
 // 0.551329 log(u / (1 - u + 1e-6)), src/gpuRIR_cuda.cu:454-455) with
 // u = (x+1)/2^32 and 1-u = (2^32-1-x)/2^32 as exact integers, or Box-Muller.
+// The ratio is formed with the hardware reciprocal and its logarithm with the hardware
+// log2 (absolute error of a few 1e-7 on z, against the 1e-5 bar of the tail).
 __device__ inline float tail_noise(const unsigned w[4], int u, int noise) {
     if (noise == 0) {
         const float num = (float)w[u] + 1.0f;
         const float den = (float)(0xFFFFFFFFu - w[u]) + 4294.967296f;
-        return 0.551329f * logf(num / den);
+        return (0.551329f * 0.69314718055994531f) * __builtin_amdgcn_logf(num * __builtin_amdgcn_rcpf(den));
     }
     const int pr2 = u >> 1;
     const float lnua = log1pf(-(float)(0xFFFFFFFFu - w[2 * pr2]) * 0x1p-32f);
@@ -651,10 +656,12 @@ __device__ inline double block_sum(double v, double* red) {
     return t;
 }
 
-// Sum of the partial rows of RIRs whose images were split into several chunks.
-// One workgroup covers 64 samples of one RIR: thread (g, l) adds chunks g, g+4,
-// g+8, ... of sample l with eight loads in flight, and the four group sums are
-// combined in a fixed order, so every run gives the same bits.
+// Sum of the partial rows of RIRs whose images were split into several chunks, in a
+// fixed order (every run gives the same bits).  A block (RIR, t0 | WIDE) covers either
+// 256 samples with one thread per sample adding all chunks (few chunks), or 64 samples
+// with four thread groups adding chunks g, g+4, ... and combined in order (many chunks);
+// eight loads are in flight per thread.
+constexpr int SUM_WIDE = 1 << 30;
 __global__ __launch_bounds__(WG) void partial_sum_kernel(const RoomDev* rooms, const int* rir_room,
                                                          const int2* blocks, const float* part, float* out) {
     __shared__ float red[4][64];
@@ -662,11 +669,29 @@ __global__ __launch_bounds__(WG) void partial_sum_kernel(const RoomDev* rooms, c
     const RoomDev* Rp = rooms + rir_room[b.x];
     const int C = Rp->n_chunks, nISM = Rp->nISM, nS = Rp->nSamples;
     const int m = b.x - Rp->rir_off;
+    const bool wide = (b.y & SUM_WIDE) != 0;
+    const int t0 = b.y & (SUM_WIDE - 1);
+    const float* row = part + Rp->part_off + (long long)m * C * nISM;
+    float* orow = out + Rp->out_off + (long long)m * nS;
+    if (wide) {
+        const int t = t0 + threadIdx.x;
+        if (t >= nISM) return;
+        const float* pp = row + t;
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = u < C ? pp[(long long)u * nISM] : 0.0f;
+        float acc = v[0];
+#pragma unroll
+        for (int u = 1; u < 8; u++)
+            if (u < C) acc += v[u];
+        orow[t] = acc;
+        return;
+    }
     const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int t = b.y + l;
+    const int t = t0 + l;
     float acc = 0.0f;
     if (t < nISM) {
-        const float* pp = part + Rp->part_off + (long long)m * C * nISM + t;
+        const float* pp = row + t;
         int c = g;
         for (; c + 28 < C; c += 32) {
             float v[8];
@@ -679,8 +704,7 @@ __global__ __launch_bounds__(WG) void partial_sum_kernel(const RoomDev* rooms, c
     }
     red[g][l] = acc;
     __syncthreads();
-    if (g == 0 && t < nISM)
-        out[Rp->out_off + (long long)m * nS + t] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+    if (g == 0 && t < nISM) orow[t] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
 }
 
 // One workgroup per RIR: the envelope of src/gpuRIR_cuda.cu:415-443 and the tail of :445-461.
@@ -740,7 +764,8 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
             const int sd = 4 * blk + u;
             if (sd < nDiff) {
                 const float z = tail_noise(w, u, a.noise);
-                const float env = expf(fmaf((float)sd, ha, base));
+                // exp(x) = 2^(x log2 e) with the hardware exp2 (relative error ~1e-7)
+                const float env = __builtin_amdgcn_exp2f(fmaf((float)sd, ha, base) * 1.4426950408889634f);
                 tail[sd] = gain * env * z;
             }
         }
@@ -1467,7 +1492,10 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const Tap
         P.tt_max = std::max(P.tt_max, TT);
         const int csz = (N + C - 1) / C;
         for (int m = 0; m < M; m++) {
-            if (C > 1)
+            if (C > 1 && C <= 8)
+                for (int t0 = 0; t0 < R.nISM; t0 += 256)
+                    P.sum_blocks.push_back(make_int2((int)(rir_off + m), t0 | SUM_WIDE));
+            else if (C > 8)
                 for (int t0 = 0; t0 < R.nISM; t0 += 64) P.sum_blocks.push_back(make_int2((int)(rir_off + m), t0));
             P.rir_room.push_back((int)k);
             for (int c = 0; c < C; c++)

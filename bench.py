@@ -261,7 +261,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.rooms} uneven rooms in one launch (3-10 x 3-8 x 2.5-4 m, T60 0.2-1.2 s, "
                                "1 src x 4-16 rcv, fs 16 kHz, Tmax 1.1 T60, Tdiff at 15 dB), FP32 exact sinc, "
-                               "hipGraph replay", "rirs": n_rirs, "rooms": args.rooms, "parallelism": "single GPU"},
+                               "autotuned work items, hipGraph replay", "rirs": n_rirs, "rooms": args.rooms, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "ism_accumulate_kernel", "kernel_ms": ms_acc,

@@ -33,27 +33,52 @@ polar_patterns = {
 }
 
 
+def _wall_areas(room_sz):
+    """Areas of the six walls in the order of beta: x0, x1 (L_y L_z), y0, y1 (L_x L_z), z0, z1 (L_x L_y)."""
+    lx, ly, lz = room_sz[0], room_sz[1], room_sz[2]
+    return (ly * lz, ly * lz, lx * lz, lx * lz, lx * ly, lx * ly)
+
+
 def beta_SabineEstimation(room_sz, T60, abs_weights=[1.0] * 6):
     """Reflection coefficients giving the requested T60 by Sabine's formula (gpuRIR/__init__.py:22-54).
 
-    Same objective, start point (0.5) and bounds ([0, 1]) as the reference, solved with
-    scipy.optimize.minimize; returns float32 [beta_x0, beta_x1, beta_y0, beta_y1, beta_z0, beta_z1].
+    The absorption of wall i is x * w_i with w the weights scaled to a maximum of 1; x in [0, 1]
+    minimises |T60 - 0.161 V / sum_i x w_i S_i| (0.161 V / 0 taken as 0 for an anechoic room),
+    found by scipy.optimize.minimize from x = 0.5 with bounds [0, 1] as in the reference, so the
+    float32 result is the reference's.  Returns [beta_x0, beta_x1, beta_y0, beta_y1, beta_z0, beta_z1].
     """
     from scipy.optimize import minimize
 
-    def t60error(x, T60, room_sz, abs_weights):
-        alpha = x * abs_weights
-        Sa = (alpha[0] + alpha[1]) * room_sz[1] * room_sz[2] + \
-            (alpha[2] + alpha[3]) * room_sz[0] * room_sz[2] + \
-            (alpha[4] + alpha[5]) * room_sz[0] * room_sz[1]
-        V = np.prod(room_sz)
-        if Sa == 0:
-            return T60 - 0  # anechoic chamber
-        return abs(T60 - 0.161 * V / Sa)  # Sabine's formula
+    areas = _wall_areas(room_sz)
+    volume = np.prod(room_sz)
+    w = abs_weights / np.array(abs_weights).max()
 
-    abs_weights /= np.array(abs_weights).max()
-    result = minimize(t60error, 0.5, args=(T60, room_sz, abs_weights), bounds=[[0, 1]])
-    return np.sqrt(1 - result.x * abs_weights).astype(np.float32)
+    def misfit(x):
+        a = x * w
+        absorption = (a[0] + a[1]) * areas[0] + (a[2] + a[3]) * areas[2] + (a[4] + a[5]) * areas[4]
+        if absorption == 0:
+            return T60
+        return abs(T60 - 0.161 * volume / absorption)
+
+    x = minimize(misfit, 0.5, bounds=[[0, 1]]).x
+    return np.sqrt(1 - x * w).astype(np.float32)
+
+
+def beta_SabineEstimation_batch(room_sz, T60, abs_weights=None):
+    """Vectorised Sabine prelude for many rooms (no reference counterpart; the reference solves
+    one room per scipy call).  room_sz [N, 3], T60 [N], abs_weights [6] or [N, 6].
+
+    The misfit above is zero at x = 0.161 V / (T60 sum_i w_i S_i); clipped to [0, 1] that is its
+    minimiser, which scipy's L-BFGS-B reaches to ~1e-8 (the betas agree to ~1e-5, float32).
+    Returns float32 [N, 6]."""
+    L = np.atleast_2d(np.asarray(room_sz, dtype=np.float64))
+    T60 = np.atleast_1d(np.asarray(T60, dtype=np.float64))
+    w = np.ones((L.shape[0], 6)) if abs_weights is None else np.broadcast_to(
+        np.asarray(abs_weights, dtype=np.float64), (L.shape[0], 6))
+    w = w / w.max(axis=1, keepdims=True)
+    S = np.stack(_wall_areas(L.T), axis=1)
+    x = np.clip(0.161 * L.prod(axis=1) / (T60 * (w * S).sum(axis=1)), 0.0, 1.0)
+    return np.sqrt(1.0 - x[:, None] * w).astype(np.float32)
 
 
 def att2t_SabineEstimator(att_dB, T60):

@@ -39,6 +39,22 @@ def test_beta_sabine_weights_and_default_untouched():
     assert default == [1.0] * 6
 
 
+def test_beta_sabine_batch_matches_scipy_per_room():
+    rng = np.random.default_rng(3)
+    L = np.stack([rng.uniform(3, 10, 20), rng.uniform(3, 8, 20), rng.uniform(2.5, 4, 20)], 1)
+    T60 = rng.uniform(0.2, 1.2, 20)
+    wts = [1.0, 0.8, 1.0, 0.6, 0.9, 0.3]
+    for weights in (None, wts):
+        b = gpuRIR.beta_SabineEstimation_batch(L, T60, weights)
+        assert b.dtype == np.float32 and b.shape == (20, 6)
+        for k in range(20):
+            ref = gpuRIR.beta_SabineEstimation(L[k], T60[k], [1.0] * 6 if weights is None else list(weights))
+            assert np.max(np.abs(b[k] - ref)) < 2e-4
+    # anechoic limit: T60 -> 0 gives x = 1 (clipped), beta = sqrt(1 - w)
+    b = gpuRIR.beta_SabineEstimation_batch([[4, 4, 3]], [1e-6], wts)
+    assert np.allclose(b[0], np.sqrt(1 - np.array(wts) / max(wts)), atol=1e-7)
+
+
 def test_polar_patterns_table():
     assert gpuRIR.polar_patterns == {"omni": 0, "homni": 1, "card": 2, "hypcard": 3, "subcard": 4, "bidir": 5}
     for name in gpuRIR.__all__:

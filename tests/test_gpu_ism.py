@@ -83,7 +83,12 @@ def test_image_offsets_identical(cfg_name):
     assert np.max(np.abs(a - oamp)) <= 2e-6 * np.max(np.abs(oamp))
 
 
-@pytest.mark.parametrize("fs", [16000.0, 44100.0, 8000.0])
+# 8 kHz: Tw = 64 (1 pass of 64 lanes), 16 kHz: 128 (2), 22.05 kHz: 176 (3), 32 kHz: 256 (4),
+# 44.1 kHz: 353 (odd, 6 with masked lanes), 48 kHz: 384 (6), 96 kHz: 768 (12)
+ALL_FS = [8000.0, 16000.0, 22050.0, 32000.0, 44100.0, 48000.0, 96000.0]
+
+
+@pytest.mark.parametrize("fs", ALL_FS)
 def test_sampling_rates_odd_and_even_windows(fs):
     cfg = W.plain_room()
     cfg["fs"] = fs
@@ -206,7 +211,7 @@ def test_mid_room_fp16_packed():
     print(f"mid room FP16 rel-L2 {e:.3e}")
 
 
-@pytest.mark.parametrize("fs", [16000.0, 44100.0, 8000.0])
+@pytest.mark.parametrize("fs", ALL_FS)
 def test_fp16_sampling_rates(fs):
     cfg = W.plain_room()
     cfg["fs"] = fs

@@ -213,20 +213,31 @@ def main():
     c5 = W.trajectory()
     rirs5 = T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs)
     sig5 = torch.as_tensor(c5["signal"], device=dev)
-    for _ in range(2):
+    for _ in range(3):
         T.trajectory(sig5, rirs5)
     torch.cuda.synchronize()
+    # device time of the convolution + overlap-add kernels (events on the stream they run on)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n_tr = 20
     t0 = time.perf_counter()
-    for _ in range(10):
+    e0.record()
+    for _ in range(n_tr):
         y5 = T.trajectory(sig5, rirs5)
+    e1.record()
     torch.cuda.synchronize()
-    ms_traj = (time.perf_counter() - t0) * 1e3 / 10
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    ms_traj_host = (time.perf_counter() - t0) * 1e3 / n_tr
+    ms_traj = e0.elapsed_time(e1) / n_tr
+    P5, R5, L5 = rirs5.shape
+    n5 = sig5.numel()
+    traj_bytes = 4 * P5 * R5 * L5 + 4 * n5 + 8 * (n5 + L5 - 1) * R5  # algorithmic HBM bytes of stage (d)
+    traj_gbs = traj_bytes / (ms_traj / 1e3) / 1e9
+    e0.record()
     T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs, out=rirs5)
+    e1.record()
     torch.cuda.synchronize()
-    ms_traj_rirs = (time.perf_counter() - t0) * 1e3
-    log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms, convolution+OLA {ms_traj:.3f} ms")
+    ms_traj_rirs = e0.elapsed_time(e1)
+    log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms; convolution+OLA {ms_traj:.3f} ms on the device "
+        f"({traj_gbs:.0f} GB/s of {traj_bytes / 1e6:.1f} MB algorithmic), {ms_traj_host:.3f} ms per call on the host")
     del y5
 
     # ---------------- CPU baseline: the FP64 oracle on a bounded sample of the same batch
@@ -272,7 +283,9 @@ def main():
         "batched_over_plain": rirs_per_s / plain_rirs_s,
         "mid_room_ms": ms_mid,
         "finalize_ms": ms_fin,
-        "trajectory": {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj},
+        "trajectory": {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj, "conv_ola_host_ms": ms_traj_host,
+                       "conv_ola_gbs": traj_gbs, "conv_ola_bytes": traj_bytes,
+                       "conv_ola_frac_hbm": traj_gbs / 8000.0},
         "batched_modes": batch,
         "gradients": {"workload": "256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 src x 4 rcv, Tmax 0.5 s, "
                                   "Tdiff at 20 dB), MSE to targets from perturbed betas and sources; "

@@ -7,11 +7,13 @@
 //
 // Design: a real length-N transform is a complex length-H = N/2 transform of
 // z[n] = x[2n] + i x[2n+1] plus an O(H) split.  One workgroup holds one
-// length-H complex sequence in LDS (H <= 16384, 128 KiB) and runs
-//   forward: radix-2 decimation in frequency (natural in, bit-reversed out),
-//   split + product with the segment spectrum + inverse split (pairs k, H-k),
-//   inverse: radix-2 decimation in time (bit-reversed in, natural out),
-// so no permutation pass is needed.  Segment spectra are computed once per
+// length-H complex sequence in LDS (H <= 16384, 128 KiB) and transforms it in
+// place with a Stockham (self-sorting) FFT: passes of radix 16 (then one of 8, 4
+// or 2), each thread reading its butterflies' inputs at stride H/R, applying the
+// pass twiddles, doing the R-point DFT in registers and writing the outputs at
+// stride Ns after a barrier -- natural order in and out, log16(H) + 1 LDS round
+// trips instead of log2(H).  The inverse is the forward transform of the
+// conjugate (conjugated again on the way out).  Segment spectra are computed once per
 // trajectory point and shared by that point's receivers through L2.  The
 // overlap-add is a gather: each output sample adds the segments that cover it
 // in ascending segment order, in FP64 (deterministic).
@@ -37,7 +39,7 @@ struct ConvArgs {
     const float* rirs;      // [P][R][L]
     const float2* spec;     // [P][H+1] segment spectra (natural order)
     float2* spec_out;
-    const float2* tw;       // [H/2] e^{-2 pi i m / H}
+    const float2* tw;       // [H] e^{-2 pi i m / H}
     const float2* twN;      // [H+1] e^{-2 pi i k / N}
     float* conv;            // [P][R][conv_stride]
     int P, R, L, H, logH;
@@ -52,39 +54,103 @@ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
-__device__ inline int brev(int k, int logH) { return logH ? (int)(__brev((unsigned)k) >> (32 - logH)) : 0; }
+// W16^m = exp(-2 pi i m / 16), m = 0..7
+__device__ constexpr float W16C[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f,
+                                      0.0f, -0.38268343236508978f, -0.70710678118654752f, -0.92387953251128674f};
+__device__ constexpr float W16S[8] = {0.0f, -0.38268343236508978f, -0.70710678118654752f, -0.92387953251128674f,
+                                      -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508978f};
 
-// In-place radix-2 DIF: natural order in, bit-reversed order out.
-__device__ void fft_dif(float2* x, const float2* __restrict__ tw, int H, int logH) {
-    for (int s = 0; s < logH; s++) {
-        const int h = H >> (s + 1);
-        const int tstride = 1 << s;  // W_{2h}^j = W_H^{j * H/(2h)}
-        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
-            const int j = b & (h - 1);
-            const int i = ((b >> (logH - s - 1)) << (logH - s)) + j;
-            const float2 a = x[i], c = x[i + h];
-            x[i] = cadd(a, c);
-            x[i + h] = cmul(csub(a, c), tw[j * tstride]);
+template <int M>
+__device__ __forceinline__ float2 w16mul(float2 a) {
+    if (M == 0) return a;
+    if (M == 4) return make_float2(a.y, -a.x);  // * (-i)
+    return make_float2(a.x * W16C[M] - a.y * W16S[M], a.x * W16S[M] + a.y * W16C[M]);
+}
+
+// One radix-2 stage of an R-point decimation-in-frequency DFT held in registers
+// (pairs i, i + h inside groups of 2h; twiddle W_{2h}^(i mod h) = W16^((i mod h) 16 / 2h)).
+template <int R, int H2, int I>
+__device__ __forceinline__ void dif_stage_pairs(float2* v) {
+    if constexpr (I < R) {
+        if constexpr ((I & H2) == 0) {
+            const float2 a = v[I], b = v[I + H2];
+            v[I] = cadd(a, b);
+            v[I + H2] = w16mul<(I & (H2 - 1)) * (16 / (2 * H2))>(csub(a, b));
         }
-        __syncthreads();
+        dif_stage_pairs<R, H2, I + 1>(v);
     }
 }
 
-// In-place radix-2 DIT with conjugate twiddles: bit-reversed in, natural out (unnormalized inverse).
-__device__ void ifft_dit(float2* x, const float2* __restrict__ tw, int H, int logH) {
-    for (int s = logH - 1; s >= 0; s--) {
-        const int h = H >> (s + 1);
-        const int tstride = 1 << s;
-        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
-            const int j = b & (h - 1);
-            const int i = ((b >> (logH - s - 1)) << (logH - s)) + j;
-            const float2 a = x[i];
-            const float2 c = cmulc(x[i + h], tw[j * tstride]);
-            x[i] = cadd(a, c);
-            x[i + h] = csub(a, c);
-        }
-        __syncthreads();
+template <int R, int H2>
+__device__ __forceinline__ void dif_stages(float2* v) {
+    if constexpr (H2 >= 1) {
+        dif_stage_pairs<R, H2, 0>(v);
+        dif_stages<R, H2 / 2>(v);
     }
+}
+
+template <int R>
+__device__ __forceinline__ constexpr int brev_c(int k) {
+    int r = 0;
+    for (int b = 1, c = R >> 1; b < R; b <<= 1, c >>= 1)
+        if (k & b) r |= c;
+    return r;
+}
+
+// y_k = sum_r v_r W_R^(r k), natural order in and out (R = 2, 4, 8, 16)
+template <int R>
+__device__ __forceinline__ void dft_reg(float2* v) {
+    dif_stages<R, R / 2>(v);
+    float2 t[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) t[k] = v[brev_c<R>(k)];
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = t[k];
+}
+
+// One Stockham pass of radix R over x[0:H] (in place): butterfly j (of H/R) reads
+// x[j + r H/R], multiplies input r by W_H^(r (j mod Ns) H/(Ns R)), does the R-point DFT
+// and writes x[(j / Ns) Ns R + (j mod Ns) + r Ns].  twH[m] = W_H^m, m < H.
+template <int R>
+__device__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, int Ns) {
+    constexpr int MAXB = 32 / R;  // butterflies per thread for H <= 16384, 512 threads
+    const int nbf = H / R;
+    const int step = H / (Ns * R);
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        const int j = (int)threadIdx.x + q * CWG;
+        if (j < nbf) {
+            const int k0 = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float2 a = x[j + r * nbf];
+                v[q][r] = (r == 0 || k0 == 0) ? a : cmul(a, twH[k0 * r * step]);
+            }
+            dft_reg<R>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        const int j = (int)threadIdx.x + q * CWG;
+        if (j < nbf) {
+            const int k0 = j & (Ns - 1);
+            const int base = (j - k0) * R + k0;
+#pragma unroll
+            for (int r = 0; r < R; r++) x[base + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Forward DFT of x[0:H] in place (natural order in and out), H = 2^logH <= 16384.
+__device__ void fft_stockham(float2* x, const float2* __restrict__ twH, int H, int logH) {
+    int Ns = 1, bits = logH;
+    for (; bits >= 4; bits -= 4, Ns <<= 4) stockham_pass<16>(x, twH, H, Ns);
+    if (bits == 3) stockham_pass<8>(x, twH, H, Ns);
+    if (bits == 2) stockham_pass<4>(x, twH, H, Ns);
+    if (bits == 1) stockham_pass<2>(x, twH, H, Ns);
 }
 
 // z[n] = x[2n] + i x[2n+1] for x = src[0:len] zero-padded to 2H.
@@ -96,12 +162,12 @@ __device__ void load_real(float2* z, const float* __restrict__ src, int len, int
     __syncthreads();
 }
 
-// X[k] and X[H-k] of the real transform from the bit-reversed half-length spectrum.
-__device__ inline void split_pair(const float2* z, const float2* __restrict__ twN, int k, int H, int logH, float2& Xk,
+// X[k] and X[H-k] of the real transform from the half-length spectrum (natural order).
+__device__ inline void split_pair(const float2* z, const float2* __restrict__ twN, int k, int H, float2& Xk,
                                   float2& Xj) {
     const int j = H - k;
-    const float2 A = z[brev(k & (H - 1), logH)];
-    const float2 Bc = conjf2(z[brev(j & (H - 1), logH)]);
+    const float2 A = z[k & (H - 1)];
+    const float2 Bc = conjf2(z[j & (H - 1)]);
     const float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
     const float2 D = csub(A, Bc);
     const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);  // -i D / 2
@@ -114,11 +180,11 @@ __global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
     const int p = blockIdx.x;
     load_real(z, a.src + a.seg_start[p], a.seg_len[p], a.H);
-    fft_dif(z, a.tw, a.H, a.logH);
+    fft_stockham(z, a.tw, a.H, a.logH);
     float2* S = a.spec_out + (long long)p * (a.H + 1);
     for (int k = threadIdx.x; k <= a.H / 2; k += blockDim.x) {
         float2 Xk, Xj;
-        split_pair(z, a.twN, k, a.H, a.logH, Xk, Xj);
+        split_pair(z, a.twN, k, a.H, Xk, Xj);
         S[k] = Xk;
         S[a.H - k] = Xj;
     }
@@ -130,14 +196,14 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
     const int p = blockIdx.x / a.R, r = blockIdx.x - p * a.R;
     const int H = a.H, logH = a.logH;
     load_real(z, a.rirs + ((long long)p * a.R + r) * a.L, a.L, H);
-    fft_dif(z, a.tw, H, logH);
+    fft_stockham(z, a.tw, H, logH);
     const float2* S = a.spec + (long long)p * (H + 1);
     const float scale = 1.0f / (float)H;
     // product and inverse split, pairs (k, H-k) handled by one thread (in place)
     for (int k = threadIdx.x; k <= H / 2; k += blockDim.x) {
         const int j = H - k;
         float2 Xk, Xj;
-        split_pair(z, a.twN, k, H, logH, Xk, Xj);
+        split_pair(z, a.twN, k, H, Xk, Xj);
         float2 Yk = cmul(Xk, S[k]), Yj = cmul(Xj, S[j]);
         Yk = make_float2(Yk.x * scale, Yk.y * scale);
         Yj = make_float2(Yj.x * scale, Yj.y * scale);
@@ -147,23 +213,22 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
         const float2 O = cmulc(make_float2(0.5f * Dm.x, 0.5f * Dm.y), a.twN[k]);  // (Yk - conj Yj) W^-k / 2
         const float2 Zk = make_float2(E.x - O.y, E.y + O.x);                      // E + i O
         const float2 Zj = make_float2(E.x + O.y, -E.y + O.x);                     // conj(E) + i conj(O)
-        // both positions were read above by this thread only (pairs are disjoint)
-        if (k == 0) {
-            z[0] = Zk;
-        } else if (k == H / 2) {
-            z[brev(k, logH)] = Zk;
+        // both positions were read above by this thread only (pairs are disjoint); the
+        // inverse transform is the forward one of the conjugate
+        if (k == 0 || k == H / 2) {
+            z[k] = conjf2(Zk);
         } else {
-            z[brev(k, logH)] = Zk;
-            z[brev(j, logH)] = Zj;
+            z[k] = conjf2(Zk);
+            z[j] = conjf2(Zj);
         }
     }
     __syncthreads();
-    ifft_dit(z, a.tw, H, logH);
+    fft_stockham(z, a.tw, H, logH);
     float* out = a.conv + ((long long)p * a.R + r) * a.conv_stride;
     const int len = a.seg_len[p] + a.L - 1;
     for (int t = threadIdx.x; t < len; t += blockDim.x) {
         const float2 v = z[t >> 1];
-        out[t] = (t & 1) ? v.y : v.x;
+        out[t] = (t & 1) ? -v.y : v.x;
     }
 }
 
@@ -210,8 +275,8 @@ struct FftTables {
 
 FftTables make_fft_tables(int H) {
     FftTables T;
-    T.tw.resize(std::max(H / 2, 1));
-    for (int m = 0; m < H / 2; m++) {
+    T.tw.resize(std::max(H, 1));
+    for (int m = 0; m < H; m++) {
         const double a = -2.0 * M_PI * (double)m / (double)H;
         T.tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
     }

@@ -47,8 +47,10 @@ struct Stage {
     hipEvent_t ev = nullptr;
     bool pending = false;
 };
+constexpr int N_STAGE = 4;  // pinned staging buffers per device, used in turn
 std::mutex g_stage_mu;
-Stage g_stage[MAX_DEV];
+Stage g_stage[MAX_DEV][N_STAGE];
+int g_stage_next[MAX_DEV];
 
 int current_device(int* dev) {
     GPURIR_HIP(hipGetDevice(dev));
@@ -86,7 +88,8 @@ int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
     int dev = 0;
     if (int rc = current_device(&dev)) return rc;
     std::lock_guard<std::mutex> lk(g_stage_mu);
-    Stage& s = g_stage[dev];
+    Stage& s = g_stage[dev][g_stage_next[dev]];
+    g_stage_next[dev] = (g_stage_next[dev] + 1) % N_STAGE;
     if (s.pending) {
         GPURIR_HIP(hipEventSynchronize(s.ev));  // the previous copy has read the buffer
         s.pending = false;
@@ -396,12 +399,14 @@ int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float
     float* d_os = d_pr + pr + pad;
     float* d_or = d_os + ps + pad;
     float* d_out = d_or + pr + pad;
-    std::vector<float> zs(ps, 0.0f), zr(pr, 0.0f);
+    // positions and orientations travel in one copy through the pinned staging buffer
     const hipStream_t st = nullptr;
-    GPURIR_HIP(hipMemcpyAsync(d_ps, pos_src, sizeof(float) * ps, hipMemcpyHostToDevice, st));
-    GPURIR_HIP(hipMemcpyAsync(d_pr, pos_rcv, sizeof(float) * pr, hipMemcpyHostToDevice, st));
-    GPURIR_HIP(hipMemcpyAsync(d_os, orV_src ? orV_src : zs.data(), sizeof(float) * ps, hipMemcpyHostToDevice, st));
-    GPURIR_HIP(hipMemcpyAsync(d_or, orV_rcv ? orV_rcv : zr.data(), sizeof(float) * pr, hipMemcpyHostToDevice, st));
+    std::vector<float> in(d_out - d_ps, 0.0f);
+    std::memcpy(in.data() + (d_ps - w), pos_src, sizeof(float) * ps);
+    std::memcpy(in.data() + (d_pr - w), pos_rcv, sizeof(float) * pr);
+    if (orV_src) std::memcpy(in.data() + (d_os - w), orV_src, sizeof(float) * ps);
+    if (orV_rcv) std::memcpy(in.data() + (d_or - w), orV_rcv, sizeof(float) * pr);
+    if (int rc = upload_async(w, in.data(), sizeof(float) * in.size(), st)) return rc;
     if (int rc = gpurir_simulate_rir_device(room_sz, beta, d_ps, M_src, d_pr, M_rcv, d_os, d_or, spkr_pattern,
                                             mic_pattern, nb_img, Tdiff, Tmax, fs, c, &o, d_out, st))
         return rc;

@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gpurir.h"
@@ -233,11 +235,15 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
 }
 
 // out[t][r] = sum over segments p covering t of conv[p][r][t - w_ini[p]], FP64, ascending p.
+// One thread per output element (t, r), r fastest: the float64 stores of a wave are
+// contiguous and each thread has only the segments of one receiver to add.
 __global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv, const int* __restrict__ w_ini,
                                                   int P, int R, int L, int conv_stride, long long n_out,
                                                   double* __restrict__ out) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_out) return;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_out * R) return;
+    const long long t = g / R;
+    const int r = (int)(g - t * R);
     // last p with w_ini[p] <= t
     int lo = 0, hi = P - 1, ph = -1;
     while (lo <= hi) {
@@ -262,11 +268,9 @@ __global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv
             lo = mid + 1;
         }
     }
-    for (int r = 0; r < R; r++) {
-        double acc = 0.0;
-        for (int p = pl; p <= ph; p++) acc += (double)conv[((long long)p * R + r) * conv_stride + (t - w_ini[p])];
-        out[t * R + r] = acc;
-    }
+    double acc = 0.0;
+    for (int p = pl; p <= ph; p++) acc += (double)conv[((long long)p * R + r) * conv_stride + (t - w_ini[p])];
+    out[g] = acc;
 }
 
 struct FftTables {
@@ -286,6 +290,35 @@ FftTables make_fft_tables(int H) {
         T.twN[k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
     return T;
+}
+
+// Twiddle tables of length-H transforms, built once per (device, H) and kept for the
+// life of the process (a few hundred KiB at most).
+std::mutex g_fft_mu;
+std::map<std::pair<int, int>, float2*> g_fft_tables;
+
+int fft_tables_device(int H, const float2** tw, const float2** twN) {
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    auto key = std::make_pair(dev, H);
+    auto f = g_fft_tables.find(key);
+    if (f == g_fft_tables.end()) {
+        FftTables T = make_fft_tables(H);
+        float2* d = nullptr;
+        const size_t n = T.tw.size() + T.twN.size();
+        if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(float2) * n) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("out of device memory for the FFT tables");
+            return -3;
+        }
+        GPURIR_HIP(hipMemcpy(d, T.tw.data(), sizeof(float2) * T.tw.size(), hipMemcpyHostToDevice));
+        GPURIR_HIP(hipMemcpy(d + T.tw.size(), T.twN.data(), sizeof(float2) * T.twN.size(), hipMemcpyHostToDevice));
+        f = g_fft_tables.emplace(key, d).first;
+    }
+    *tw = f->second;
+    *twN = f->second + std::max(H, 1);
+    return 0;
 }
 
 unsigned pow2roundup(unsigned x) {
@@ -308,28 +341,26 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     GPURIR_CHECK(H <= 16384, "convolution too long for one LDS-resident FFT (N_fft > 32768)");
     int logH = 0;
     while ((1 << logH) < H) logH++;
-    FftTables T = make_fft_tables(H);
+    const float2 *d_tw = nullptr, *d_twN = nullptr;
+    if (int rc = fft_tables_device(H, &d_tw, &d_twN)) return rc;
     const int stride = (int)((seg_max + L - 1 + 63) / 64 * 64);
     const size_t b_ss = al(sizeof(int) * P), b_sl = al(sizeof(int) * P);
-    const size_t b_tw = al(sizeof(float2) * T.tw.size()), b_twN = al(sizeof(float2) * T.twN.size());
     const size_t b_spec = al(sizeof(float2) * (size_t)P * (H + 1));
     const size_t b_conv = al(sizeof(float) * (size_t)P * R * stride);
-    const size_t b_small = b_ss + b_sl + b_tw + b_twN;
+    const size_t b_small = b_ss + b_sl;
     void* ws = nullptr;
     if (int rc = workspace(WS_CONV, b_small + b_spec + b_conv + al(extra_bytes), &ws)) return rc;
     unsigned char* w = static_cast<unsigned char*>(ws);
     std::vector<unsigned char> host(b_small, 0);
     std::memcpy(host.data(), seg_start.data(), sizeof(int) * P);
     std::memcpy(host.data() + b_ss, seg_len.data(), sizeof(int) * P);
-    std::memcpy(host.data() + b_ss + b_sl, T.tw.data(), sizeof(float2) * T.tw.size());
-    std::memcpy(host.data() + b_ss + b_sl + b_tw, T.twN.data(), sizeof(float2) * T.twN.size());
     if (int rc = upload_async(w, host.data(), b_small, stream)) return rc;
     ConvArgs a;
     a.src = d_src;
     a.seg_start = reinterpret_cast<const int*>(w);
     a.seg_len = reinterpret_cast<const int*>(w + b_ss);
-    a.tw = reinterpret_cast<const float2*>(w + b_ss + b_sl);
-    a.twN = reinterpret_cast<const float2*>(w + b_ss + b_sl + b_tw);
+    a.tw = d_tw;
+    a.twN = d_twN;
     a.spec_out = reinterpret_cast<float2*>(w + b_small);
     a.spec = a.spec_out;
     a.conv = reinterpret_cast<float*>(w + b_small + b_spec);
@@ -407,7 +438,9 @@ int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d
         return rc;
     if (int rc = upload_async(extra, w_ini, sizeof(int) * (n_pts + 1), st)) return rc;
     const long long n_out = (long long)n_signal + rir_len - 1;
-    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, st, conv,
+    const long long n_el = n_out * n_rcv;
+    GPURIR_CHECK(n_el / 256 < (1LL << 31), "trajectory output too long");
+    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)((n_el + 255) / 256)), dim3(256), 0, st, conv,
                        static_cast<const int*>(extra), n_pts, n_rcv, rir_len, stride, n_out, d_out);
     GPURIR_HIP(hipGetLastError());
     return 0;

@@ -9,23 +9,28 @@
 //   envPred_kernel :415-443, diffRev_kernel :445-461 (+ cuRAND :872-886)
 //                                 -> ism_finalize_kernel (Philox4x32-10 noise)
 //
-// Design (one workgroup = one work item = one RIR x one image chunk x one time tile):
+// Design (one work item = one RIR x one image chunk x one time tile, one workgroup at a time):
 //   * stage (a) per image in FP64 for the delay (image vector is exact for float
 //     inputs; sqrt and one multiply by fs/c), split into an integer sample n0 and
 //     a float fraction f;  amplitude, directivities and tap constants in FP32;
 //   * images whose window misses the tile are culled; live ones are compacted
 //     into an LDS queue with wave ballots;
 //   * stage (b) scatters each live image's taps into an LDS-resident partial RIR
-//     with ds_add_f32 (no global atomics).  Each lane owns one image and walks
-//     its taps; the windowed sinc uses the identities
+//     held in 64-bit fixed point (ds_add_u64: exact, order-independent sums; the
+//     scale is a power of two from a bound on every tap of the workgroup).  A wave
+//     takes one image at a time and its 64 lanes cover consecutive tap offsets j
+//     (conflict-free atomics, per-offset constants in registers); the windowed sinc
+//     uses the identities
 //        sin(pi (j - f))        = (-1)^(j+1) sin(pi f)
 //        cos(2 pi (j - f) / Tw) = cos(2 pi j/Tw) cos(2 pi f/Tw) + sin(2 pi j/Tw) sin(2 pi f/Tw)
-//     so a tap costs one subtraction, one reciprocal, two FMAs and one multiply;
+//     so a tap is one FMA for the offset, one reciprocal, two FMAs for the window,
+//     one product and the conversion to fixed point;
 //   * the tile is flushed with coalesced stores, either straight into the output
-//     (one chunk per RIR) or into a partial row that ism_finalize_kernel adds up
+//     (one chunk per RIR) or into a partial row that partial_sum_kernel adds up
 //     in a fixed order (deterministic across runs for a given plan);
-//   * ism_finalize_kernel also computes the Sabine envelope from the last 10 ms of
-//     the early part and writes the diffuse tail.
+//   * ism_finalize_kernel computes the Sabine envelope from the last 10 ms of the
+//     early part and writes the diffuse tail;
+//   * the adjoint kernels at the end of this file give the gradients.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

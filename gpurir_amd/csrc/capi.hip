@@ -616,7 +616,10 @@ int gpurir_batch_plan_adjoint(gpurir_plan_t p, const float* d_grad_out, float* d
     GPURIR_CHECK(p != nullptr && d_grad_out != nullptr, "null argument");
     if (!p->adj) {
         AdjLaunch* a = adj_new();
-        if (int rc = adj_prepare(p->rooms, p->so, *a)) {
+        SimOptions so = p->so;  // the adjoint plans its own work items
+        so.chunk_images = 0;
+        so.tile_samples = 0;
+        if (int rc = adj_prepare(p->rooms, so, *a)) {
             adj_delete(a);
             return rc;
         }

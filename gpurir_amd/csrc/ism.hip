@@ -874,6 +874,7 @@ struct AdjArgs {
     const float* rir_cw;  // per RIR: c_w
     const int* rir_ws;    // per RIR: w_start
     double* item_part;    // [n_items][12]: dbeta[6], dsrc[3], drcv[3]
+    const int* order;     // launch order of the items (longest first)
     int spkr, mic;
     int J0, nJ;
     float halfTw, inv_tw2, pi_tw;  // Tw/2, 2/Tw, pi/Tw
@@ -965,7 +966,7 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[NWAVE][12];
     __shared__ int wcnt[NWAVE];
-    const int bid = blockIdx.x;
+    const int bid = a.order[blockIdx.x];
     const Item it = a.items[bid];
     const RoomK R = load_room(a.rooms + it.room);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1805,7 +1806,8 @@ struct AdjLaunch {
     TapTables T;
     std::vector<float4> ktab;
     std::vector<int> rir_item_begin;
-    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, plan_bytes = 0;
+    std::vector<int> order;
+    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, off_ord = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
     size_t lds = 0;
     int ntab = 0, gp_len = 0, Tw = 0;
@@ -1840,6 +1842,16 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
     }
     SimOptions opt = opt_in;
     if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
+    if (opt.chunk_images <= 0) {
+        // about 8192 items (a few per resident workgroup) so that the launch ends evenly
+        double work = 0.0;
+        for (const RoomHost& h : rooms) {
+            const int nISM = n_samples_rule(h.Tdiff, opt.fs);
+            const int tiles = nISM > 0 ? (nISM + opt.tile_samples - 1) / opt.tile_samples : 0;
+            work += (double)h.n_src * h.n_rcv * h.nb[0] * h.nb[1] * (double)h.nb[2] * tiles;
+        }
+        opt.chunk_images = (int)std::min(65536.0, std::max(1024.0, std::ceil(work / 8192.0)));
+    }
     A.opt = opt;
     A.Tw = tw_rule(opt.fs);
     GPURIR_CHECK(A.Tw >= 2, "sampling frequency too low for the 8 ms window");
@@ -1864,18 +1876,26 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         for (const Item& it : P.items) cnt[P.rooms[it.room].rir_off + it.rir]++;
         for (int g = 0; g < n_rirs; g++) A.rir_item_begin[g + 1] = A.rir_item_begin[g] + cnt[g];
     }
+    // launch order: longest items first (the partial sums keep the item's own index)
+    A.order.resize(P.items.size());
+    for (size_t e = 0; e < P.items.size(); e++) A.order[e] = (int)e;
+    std::stable_sort(A.order.begin(), A.order.end(), [&](int x, int y) {
+        return (P.items[x].i1 - P.items[x].i0) > (P.items[y].i1 - P.items[y].i0);
+    });
     A.off_rooms = 0;
     A.off_items = align_up(sizeof(RoomDev) * P.rooms.size(), 256);
     A.off_rr = align_up(A.off_items + sizeof(Item) * P.items.size(), 256);
     A.off_kt = align_up(A.off_rr + sizeof(int) * P.rir_room.size(), 256);
     A.off_rib = align_up(A.off_kt + sizeof(float4) * A.ktab.size(), 256);
-    A.plan_bytes = align_up(A.off_rib + sizeof(int) * A.rir_item_begin.size(), 256);
+    A.off_ord = align_up(A.off_rib + sizeof(int) * A.rir_item_begin.size(), 256);
+    A.plan_bytes = align_up(A.off_ord + sizeof(int) * (A.order.size() + 1), 256);
     A.blob.assign(A.plan_bytes, 0);
     std::memcpy(A.blob.data() + A.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(A.blob.data() + A.off_items, P.items.data(), sizeof(Item) * P.items.size());
     std::memcpy(A.blob.data() + A.off_rr, P.rir_room.data(), sizeof(int) * P.rir_room.size());
     std::memcpy(A.blob.data() + A.off_kt, A.ktab.data(), sizeof(float4) * A.ktab.size());
     std::memcpy(A.blob.data() + A.off_rib, A.rir_item_begin.data(), sizeof(int) * A.rir_item_begin.size());
+    if (!A.order.empty()) std::memcpy(A.blob.data() + A.off_ord, A.order.data(), sizeof(int) * A.order.size());
     A.ntab = (P.ntab_max + 3) / 4 * 4;
     A.gp_len = (int)align_up((size_t)P.tt_max + 2 * (size_t)(A.T.nJ - 1) + 4, 4);
     A.lds = (sizeof(double) + 3 * sizeof(float)) * A.ntab + sizeof(float) * A.gp_len + (size_t)AQ * 12;
@@ -1934,6 +1954,7 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.rir_cw = rir_cw;
         a.rir_ws = rir_ws;
         a.item_part = item_part;
+        a.order = reinterpret_cast<const int*>(dp + A.off_ord);
         a.spkr = A.opt.spkr;
         a.mic = A.opt.mic;
         a.J0 = A.T.J0;

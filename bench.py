@@ -145,6 +145,7 @@ def main():
     c4 = W.gradient_rooms(256, seed=0)
     target = T.BatchPlan(W.gradient_targets(c4, seed=1), fs, seed=1234).run().clone()
     gplan = T.BatchPlan(c4, fs, seed=1234)
+    gtune = gplan.autotune() if not args.no_autotune else None
 
     def grad_step():
         y = gplan.run()
@@ -290,7 +291,8 @@ def main():
         "gradients": {"workload": "256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 src x 4 rcv, Tmax 0.5 s, "
                                   "Tdiff at 20 dB), MSE to targets from perturbed betas and sources; "
                                   "d/d(6 betas, source xyz, receiver xyz, level)",
-                      "grads_per_s": grads_per_s, "ms_per_fwd_bwd": ms_grad, "ms_fwd": ms_fwd4},
+                      "grads_per_s": grads_per_s, "ms_per_fwd_bwd": ms_grad, "ms_fwd": ms_fwd4,
+                      "forward_autotune": gtune},
     }
     print(json.dumps(line), flush=True)
 

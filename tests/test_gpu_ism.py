@@ -269,3 +269,47 @@ def test_persistent_plan_and_autotune():
                          Tdiff=r["Tdiff"], room_index=0)
     check_parity(a.rirs[0].cpu().numpy(), o, orc.n_samples(r["Tdiff"], W.FS))
     assert T.BatchPlan(rooms, W.FS).autotune() == choice
+
+
+def test_no_absorption_walls():
+    """beta = 1 on every wall: Sabine's T60 is infinite, alpha = 0 and the tail envelope is flat
+    (src/gpuRIR_cuda.cu:123-130 divides by a zero absorption area); GPU and oracle agree."""
+    cfg = W.plain_room()
+    cfg["beta"] = np.ones(6, np.float32)
+    cfg["Tmax"] = 0.1
+    cfg["Tdiff"] = 0.04
+    cfg["nb_img"] = gpuRIR.t2n(0.04, cfg["room_sz"])
+    g, o = run_both(cfg)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
+    check_parity(g, o, orc.n_samples(0.04, cfg["fs"]))
+
+
+def test_ragged_batch_several_sources_and_edge_rooms():
+    """Rooms with 1-3 sources and 1-5 receivers, one without a diffuse tail, one with a
+    one-image set and one whose early part is empty, all in one launch."""
+    rng = np.random.default_rng(21)
+    rooms = []
+    for k in range(7):
+        L = rng.uniform([3.0, 3.0, 2.5], [7.0, 6.0, 3.5])
+        ns, nr = int(rng.integers(1, 4)), int(rng.integers(1, 6))
+        Tmax = float(rng.uniform(0.05, 0.15))
+        Tdiff = float(rng.uniform(0.02, Tmax))
+        rooms.append(dict(room_sz=L.astype(np.float32), beta=rng.uniform(0.5, 0.95, 6).astype(np.float32),
+                          pos_src=(0.5 + rng.uniform(size=(ns, 3)) * (L - 1.0)).astype(np.float32),
+                          pos_rcv=(0.5 + rng.uniform(size=(nr, 3)) * (L - 1.0)).astype(np.float32),
+                          Tmax=Tmax, Tdiff=Tdiff, nb_img=gpuRIR.t2n(Tdiff, L)))
+    rooms[1]["Tdiff"] = rooms[1]["Tmax"]          # no diffuse tail
+    rooms[2]["nb_img"] = [1, 1, 1]                # direct path only
+    rooms[3]["Tdiff"] = 0.0                       # no early part: all tail (degenerate envelope)
+    _, views = T.simulate_batch(rooms, W.FS, seed=5, call_offset=2)
+    torch.cuda.synchronize()
+    for k, r in enumerate(rooms):
+        o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
+                             Tdiff=r["Tdiff"], seed=5, call_offset=2, room_index=k)
+        g = views[k].cpu().numpy()
+        assert g.shape == o.shape
+        if k == 3:
+            assert np.array_equal(np.isnan(g), np.isnan(o))
+            assert np.allclose(np.nan_to_num(g), np.nan_to_num(o), atol=1e-6)
+        else:
+            check_parity(g, o, orc.n_samples(r["Tdiff"], W.FS))

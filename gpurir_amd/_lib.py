@@ -19,7 +19,8 @@ class Options(ctypes.Structure):
     """gpurir_options_t"""
     _fields_ = [("precision", ctypes.c_int), ("noise", ctypes.c_int), ("seed", ctypes.c_ulonglong),
                 ("call_offset", ctypes.c_uint), ("room_index", ctypes.c_uint), ("level", ctypes.c_float),
-                ("chunk_images", ctypes.c_int), ("tile_samples", ctypes.c_int), ("persistent", ctypes.c_int)]
+                ("chunk_images", ctypes.c_int), ("tile_samples", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("crossfade", ctypes.c_int)]
 
 
 PREC_FP32, PREC_LUT, PREC_FP16 = 0, 1, 2

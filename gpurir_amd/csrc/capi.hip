@@ -154,6 +154,7 @@ SimOptions sim_options(const gpurir_options_t* o, int spkr, int mic, float fs, f
     s.call_offset = o ? o->call_offset : 0u;
     s.chunk_images = o ? o->chunk_images : 0;
     s.tile_samples = o ? o->tile_samples : 0;
+    s.crossfade = o ? std::max(o->crossfade, 0) : 0;
     s.spkr = spkr;
     s.mic = mic;
     s.fs = fs;

@@ -96,6 +96,7 @@ struct SimOptions {
     float fs, c;
     int chunk_images;        // 0: planner default
     int tile_samples;        // 0: planner default
+    int crossfade;           // samples of cross-fade before nISM (0: hard switch)
 };
 
 // Plans and launches a batch of rooms whose positions and output are on the device.

@@ -89,6 +89,7 @@ struct FinArgs {
     unsigned long long seed;
     unsigned call_offset;
     int noise;
+    int crossfade;
 };
 
 // Integer power by squaring (e >= 0); 0^0 = 1 as powf(0,0) in the reference.
@@ -759,6 +760,19 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     const float base = (float)(0.5 * alpha * ((double)nISM - tau_dp));
 
     const unsigned k0 = (unsigned)a.seed, k1 = (unsigned)(a.seed >> 32);
+    // optional cross-fade on [nISM - X, nISM): (1 - r) early part + r diffuse model, r = (k + 1/2)/X,
+    // with noise indices nDiff + k (the envelope above was fitted on the early part alone)
+    const int X = min(a.crossfade, nISM);
+    for (int k = tid; k < X; k += WG) {
+        const int sd = nDiff + k;
+        unsigned w[4];
+        philox4x32_10((unsigned)(sd >> 2), (unsigned)m, R.room_id, a.call_offset, k0, k1, w);
+        const float z = tail_noise(w, sd & 3, a.noise);
+        const int t = nISM - X + k;
+        const float env = __builtin_amdgcn_exp2f(fmaf((float)(t - nISM), ha, base) * 1.4426950408889634f);
+        const float r = ((float)k + 0.5f) / (float)X;
+        o[t] = fmaf(r, gain * env * z, (1.0f - r) * o[t]);
+    }
     float* tail = o + nISM;
     const int nblk = (nDiff + 3) >> 2;
     for (int blk = tid; blk < nblk; blk += WG) {
@@ -1730,6 +1744,7 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     f.seed = S.opt.seed;
     f.call_offset = S.opt.call_offset;
     f.noise = S.opt.noise;
+    f.crossfade = S.opt.crossfade;
     hipLaunchKernelGGL(ism_finalize_kernel, dim3((unsigned)S.P.rir_room.size()), dim3(WG), 0, stream, f);
     GPURIR_HIP(hipGetLastError());
     return 0;
@@ -1829,6 +1844,7 @@ size_t adj_ws_bytes(const AdjLaunch& A) {
 
 int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, AdjLaunch& A) {
     GPURIR_CHECK(opt_in.fs > 0.0f && opt_in.c > 0.0f, "fs and c must be positive");
+    GPURIR_CHECK(opt_in.crossfade == 0, "the adjoint does not support a cross-fade");
     GPURIR_CHECK(!rooms.empty(), "no rooms");
     int nism_max = 0;
     for (const RoomHost& h : rooms) {

@@ -51,6 +51,8 @@ typedef struct gpurir_options {
     int chunk_images;           /* images per work item, 0 = planner default             */
     int tile_samples;           /* ISM samples per LDS tile, 0 = planner default         */
     int persistent;             /* batch plans: 1 = persistent accumulation launch       */
+    int crossfade;              /* samples of linear cross-fade into the diffuse model   */
+                                /* before Tdiff (0 = the reference's hard switch)        */
 } gpurir_options_t;
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -45,13 +45,13 @@ def lib():
                                        ctypes.c_int, ctypes.c_int, ip, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
-                                       dp, dp]
+                                       ctypes.c_int, dp, dp]
         L.orc_simulate_rir.restype = ctypes.c_int
         L.orc_simulate_rir_d.argtypes = [dp, dp, dp, ctypes.c_int, dp, ctypes.c_int, dp, dp,
                                          ctypes.c_int, ctypes.c_int, ip, ctypes.c_float, ctypes.c_float,
                                          ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
-                                         dp, dp, fp, fp, fp, fp, fp, fp]
+                                         ctypes.c_int, dp, dp, fp, fp, fp, fp, fp, fp]
         L.orc_simulate_rir_d.restype = ctypes.c_int
         L.orc_images.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.c_int, fp, fp, ctypes.c_int,
                                  ctypes.c_int, ip, ctypes.c_float, ctypes.c_float,
@@ -104,8 +104,10 @@ def max_threads():
 
 def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, spkr_pattern="omni",
                  mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0, mode=EXACT_SCATTER,
-                 noise="logistic", seed=1234, room_index=0, call_offset=0, level=1.0, return_env=False):
-    """FP64 restatement of simulateRIR; returns float64 [M_src, M_rcv, nSamples]."""
+                 noise="logistic", seed=1234, room_index=0, call_offset=0, level=1.0, crossfade=0,
+                 return_env=False):
+    """FP64 restatement of simulateRIR; returns float64 [M_src, M_rcv, nSamples].
+    crossfade: samples of linear cross-fade before Tdiff (0 = the reference's hard switch)."""
     room = _f32(room_sz, (3,))
     b = _f32(beta, (6,))
     ps = _f32(pos_src).reshape(-1, 3)
@@ -125,7 +127,7 @@ def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, 
                            ps.shape[0], _ptr(pr, ctypes.c_float), pr.shape[0], _ptr(os_, ctypes.c_float),
                            _ptr(or_, ctypes.c_float), sp, mp, _ptr(nb, ctypes.c_int), float(Tdiff), float(Tmax),
                            float(fs), float(c), int(mode), kind, int(seed), int(room_index), int(call_offset),
-                           float(level), _ptr(out, ctypes.c_double), _ptr(env, ctypes.c_double))
+                           float(level), int(crossfade), _ptr(out, ctypes.c_double), _ptr(env, ctypes.c_double))
     if return_env:
         return out, env
     return out
@@ -137,7 +139,7 @@ def _f64(a):
 
 def simulate_rir_f64(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, spkr_pattern="omni",
                      mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0, mode=EXACT_SCATTER, noise="logistic",
-                     seed=1234, room_index=0, call_offset=0, level=1.0, return_env=False):
+                     seed=1234, room_index=0, call_offset=0, level=1.0, crossfade=0, return_env=False):
     """simulate_rir with double-precision geometry (room, betas, positions, orientations)."""
     room = _f64(room_sz).reshape(3)
     b = _f64(beta).reshape(6)
@@ -160,8 +162,8 @@ def simulate_rir_f64(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=No
                              ps.shape[0], _ptr(pr, ctypes.c_double), pr.shape[0], _ptr(os_, ctypes.c_double),
                              _ptr(or_, ctypes.c_double), sp, mp, _ptr(nb, ctypes.c_int), float(Tdiff), float(Tmax),
                              float(fs), float(c), int(mode), kind, int(seed), int(room_index), int(call_offset),
-                             float(level), _ptr(out, ctypes.c_double), _ptr(env, ctypes.c_double), nf, nf, nf, nf,
-                             nf, nf)
+                             float(level), int(crossfade), _ptr(out, ctypes.c_double), _ptr(env, ctypes.c_double),
+                             nf, nf, nf, nf, nf, nf)
     if return_env:
         return out, env
     return out

@@ -379,7 +379,7 @@ int orc_simulate_rir_d(const double room[3], const double beta_d[6], const doubl
                        const double* pos_rcv, int M_rcv, const double* orv_src, const double* orv_rcv,
                        int spkr, int mic, const int nb[3], float Tdiff, float Tmax, float fs, float c,
                        int mode, int noise_kind, uint64_t seed, uint32_t room_index, uint32_t call_offset,
-                       double level, double* out, double* env_out,
+                       double level, int crossfade, double* out, double* env_out,
                        const float* f32_room, const float* f32_beta, const float* f32_src, const float* f32_rcv,
                        const float* f32_osrc, const float* f32_orcv) {
     int nISM = orc_n_samples(Tdiff, fs);
@@ -485,6 +485,20 @@ int orc_simulate_rir_d(const double room[3], const double beta_d[6], const doubl
             double nz = orc_noise(seed, (uint32_t)m, room_index, call_offset, (uint32_t)sd, noise_kind);
             o[nISM + sd] = level * sqrt(env) * nz;
         }
+        /* optional cross-fade (no reference counterpart; 0 = the reference's hard switch):
+           on t in [nISM - X, nISM) the early part fades out linearly, r = (k + 1/2) / X with
+           k = t - (nISM - X), and the diffuse model, continued before nISM with noise
+           indices nDiff + k, fades in.  The envelope is fitted before the fade. */
+        int X = crossfade < nISM ? crossfade : nISM;
+        if (nDiff > 0 && X > 0) {
+            for (int k = 0; k < X; k++) {
+                int t = nISM - X + k;
+                double r = ((double)k + 0.5) / (double)X;
+                double env = e.A * exp(e.alpha * ((double)t - e.tau_dp));
+                double nz = orc_noise(seed, (uint32_t)m, room_index, call_offset, (uint32_t)(nDiff + k), noise_kind);
+                o[t] = (1.0 - r) * o[t] + r * level * sqrt(env) * nz;
+            }
+        }
     }
     free(part);
     free(lut.tab);
@@ -495,7 +509,8 @@ int orc_simulate_rir(const float room[3], const float beta_f[6], const float* po
                      const float* pos_rcv, int M_rcv, const float* orv_src, const float* orv_rcv,
                      int spkr, int mic, const int nb[3], float Tdiff, float Tmax, float fs, float c,
                      int mode, int noise_kind, uint64_t seed, uint32_t room_index, uint32_t call_offset,
-                     double level, double* out, double* env_out /* [M][3]: A, alpha, tau_dp or NULL */) {
+                     double level, int crossfade, double* out,
+                     double* env_out /* [M][3]: A, alpha, tau_dp or NULL */) {
     double L[3], b[6];
     for (int i = 0; i < 3; i++) L[i] = room[i];
     for (int i = 0; i < 6; i++) b[i] = beta_f[i];
@@ -506,7 +521,7 @@ int orc_simulate_rir(const float room[3], const float beta_f[6], const float* po
     for (int i = 0; i < 3 * M_src; i++) { ps[i] = pos_src[i]; os[i] = orv_src[i]; }
     for (int i = 0; i < 3 * M_rcv; i++) { pr[i] = pos_rcv[i]; orr[i] = orv_rcv[i]; }
     int rc = orc_simulate_rir_d(L, b, ps, M_src, pr, M_rcv, os, orr, spkr, mic, nb, Tdiff, Tmax, fs, c, mode,
-                                noise_kind, seed, room_index, call_offset, level, out, env_out, room, beta_f,
+                                noise_kind, seed, room_index, call_offset, level, crossfade, out, env_out, room, beta_f,
                                 pos_src, pos_rcv, orv_src, orv_rcv);
     free(ps); free(pr); free(os); free(orr);
     return rc;

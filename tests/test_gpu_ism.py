@@ -313,3 +313,26 @@ def test_ragged_batch_several_sources_and_edge_rooms():
             assert np.allclose(np.nan_to_num(g), np.nan_to_num(o), atol=1e-6)
         else:
             check_parity(g, o, orc.n_samples(r["Tdiff"], W.FS))
+
+
+def test_crossfade_option():
+    """Optional linear cross-fade into the diffuse model before Tdiff (0 = the reference's hard
+    switch) against the oracle's, on a single room and in a batch."""
+    cfg = W.mid_room(n_src=2, n_rcv=3, seed=4)
+    args = dict(room_sz=cfg["room_sz"], beta=cfg["beta"], pos_src=cfg["pos_src"], pos_rcv=cfg["pos_rcv"],
+                nb_img=cfg["nb_img"], Tmax=cfg["Tmax"], fs=cfg["fs"], Tdiff=cfg["Tdiff"],
+                spkr_pattern=cfg["spkr_pattern"], mic_pattern=cfg["mic_pattern"], orV_src=cfg["orV_src"],
+                orV_rcv=cfg["orV_rcv"])
+    nISM = orc.n_samples(cfg["Tdiff"], cfg["fs"])
+    for X in (1, 160, 1000):
+        g = T.simulate_rir(**args, crossfade=X, seed=3, call_offset=1).cpu().numpy()
+        o = orc.simulate_rir(**args, crossfade=X, seed=3, call_offset=1)
+        check_parity(g, o, nISM)
+        assert rel_l2(g[..., nISM - X:nISM], o[..., nISM - X:nISM]) < 1e-5
+    rooms = W.uneven_rooms(n_rooms=5, seed=8)
+    _, views = T.simulate_batch(rooms, W.FS, crossfade=64)
+    torch.cuda.synchronize()
+    for k, r in enumerate(rooms):
+        o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
+                             Tdiff=r["Tdiff"], room_index=k, crossfade=64)
+        check_parity(views[k].cpu().numpy(), o, orc.n_samples(r["Tdiff"], W.FS))

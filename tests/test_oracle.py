@@ -273,3 +273,28 @@ def test_trajectory_custom_timestamps_windows():
     w_ini, w_len = orc.trajectory_windows(1000, 4, timestamps=np.array([0.0, 0.01, 0.025, 0.05]), fs=16000)
     assert list(w_ini) == [0, 160, 400, 800, 1000]
     assert list(w_len) == [160, 240, 400, 200]
+
+
+def test_crossfade_region_is_the_stated_blend():
+    """Cross-fade (no reference counterpart; 0 = hard switch): on [nISM - X, nISM) the output is
+    (1 - r) ISM + r diffuse with r = (k + 1/2)/X, the diffuse model continued before nISM with
+    noise indices nDiff + k; everything else is untouched and X = 0 is the reference output."""
+    L, beta = [4.0, 3.5, 2.7], np.full(6, 0.85)
+    src, rcv = [[1.0, 1.2, 1.4]], [[2.9, 2.1, 1.1], [1.7, 2.6, 1.9]]
+    fs, Tmax, Tdiff, X = 16000.0, 0.08, 0.03, 40
+    kw = dict(room_sz=L, beta=beta, pos_src=src, pos_rcv=rcv, nb_img=[5, 5, 5], Tmax=Tmax, fs=fs, Tdiff=Tdiff,
+              seed=9, call_offset=4, level=0.8)
+    y0, env = orc.simulate_rir(**kw, return_env=True)
+    yx = orc.simulate_rir(**kw, crossfade=X)
+    nISM, nS = orc.n_samples(Tdiff, fs), orc.n_samples(Tmax, fs)
+    nDiff = nS - nISM
+    assert np.array_equal(y0[..., :nISM - X], yx[..., :nISM - X])
+    assert np.array_equal(y0[..., nISM:], yx[..., nISM:])
+    for m in range(2):
+        A, alpha, tdp = env[m]
+        for k in range(X):
+            t = nISM - X + k
+            r = (k + 0.5) / X
+            d = 0.8 * np.sqrt(A * np.exp(alpha * (t - tdp))) * orc.noise(9, m, 0, 4, nDiff + k, 0)
+            assert abs(yx[0, m, t] - ((1 - r) * y0[0, m, t] + r * d)) <= 1e-12 * max(1.0, abs(d))
+    assert np.array_equal(orc.simulate_rir(**kw, crossfade=0), y0)

@@ -203,10 +203,10 @@ def main():
         T.simulate_rir(**kw2d, out=out2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(20):
+    for _ in range(50):
         T.simulate_rir(**kw2d, out=out2)
     torch.cuda.synchronize()
-    ms_mid = (time.perf_counter() - t0) * 1e3 / 20
+    ms_mid = (time.perf_counter() - t0) * 1e3 / 50
     err_mid = rel_l2(out2.cpu().numpy(), orc.simulate_rir(**kw2))
     log(f"[bench] mid room: {ms_mid:.3f} ms/call ({128 / ms_mid * 1e3:.4e} RIRs/s), rel-L2 {err_mid:.3e}")
 

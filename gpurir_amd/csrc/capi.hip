@@ -83,6 +83,26 @@ int workspace(int slot, size_t bytes, void** ptr) {
     return 0;
 }
 
+int device_counters(unsigned** counters) {
+    int dev = 0;
+    if (int rc = current_device(&dev)) return rc;
+    static std::mutex mu;
+    static unsigned* per_dev[MAX_DEV] = {};
+    std::lock_guard<std::mutex> lk(mu);
+    if (!per_dev[dev]) {
+        unsigned* p = nullptr;
+        if (hipMalloc(reinterpret_cast<void**>(&p), 2 * sizeof(unsigned)) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("out of device memory for the work counters");
+            return -3;
+        }
+        GPURIR_HIP(hipMemset(p, 0, 2 * sizeof(unsigned)));
+        per_dev[dev] = p;
+    }
+    *counters = per_dev[dev];
+    return 0;
+}
+
 int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
     if (bytes == 0) return 0;
     int dev = 0;
@@ -701,13 +721,14 @@ int gpurir_batch_plan_autotune(gpurir_plan_t p, void* stream, int* chunk_images,
             }
             return 0;
         };
-        // samples per tile (planner default first), then images per item, then the launch form
-        for (int tile : {0, 1024, 2048, 3072, 4096, 6144})
-            if (int rc = consider(0, tile, 0)) return rc;
+        // samples per tile (planner default first) and images per item with the persistent
+        // launch, then the one-workgroup-per-item launch of the best shape
+        for (int tile : {0, 1024, 2048, 3072, 4096, 6144, 8192})
+            if (int rc = consider(0, tile, 1)) return rc;
         const int tile = best.tile;
-        for (int chunk : {8192, 16384, 32768, 65536})
-            if (int rc = consider(chunk, tile, 0)) return rc;
-        if (int rc = consider(best.chunk, best.tile, 1)) return rc;
+        for (int chunk : {4096, 8192, 16384, 32768, 65536})
+            if (int rc = consider(chunk, tile, 1)) return rc;
+        if (int rc = consider(best.chunk, best.tile, 0)) return rc;
         GPURIR_CHECK(best.ms < 1e29f, "no plan candidate could be run");
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = best;

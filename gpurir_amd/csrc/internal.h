@@ -37,6 +37,10 @@ const char* last_error();
 enum WsSlot { WS_PLAN = 0, WS_PARTIAL, WS_STAGE, WS_CONV, WS_GRAD, WS_ADJ_PLAN, WS_COUNT };
 int workspace(int slot, size_t bytes, void** ptr);
 
+// Work counters of the persistent accumulation for eager (non-plan) launches: two
+// unsigned per device, zero between launches (every launch leaves them zero).
+int device_counters(unsigned** counters);
+
 // Host-to-device copy through a pinned staging buffer; the host source may be
 // reused as soon as the call returns.  Must not be called during stream capture.
 int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream);

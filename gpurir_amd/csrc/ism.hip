@@ -36,7 +36,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "internal.h"
 
@@ -1536,7 +1538,7 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const Tap
         P.rooms.push_back(R);
     }
     // longest items first, so that the tail of the launch is short
-    if (sort_items && rh.size() > 1)
+    if (sort_items)
         std::stable_sort(P.items.begin(), P.items.end(),
                          [](const Item& x, const Item& y) { return (x.i1 - x.i0) > (y.i1 - y.i0); });
     (void)total_rirs;
@@ -1653,6 +1655,43 @@ int sim_count_taps(const SimLaunch& S, const void* d_plan, const float* d_pos_sr
     return 0;
 }
 
+// Raises a kernel's dynamic LDS limit above the 64 KiB default when needed (once per
+// device, kernel and size).
+static int allow_lds(const void* fn, size_t lds) {
+    if (lds <= 64 * 1024) return 0;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> done;
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& have = done[std::make_pair(dev, fn)];
+    if (have < lds) {
+        GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        have = lds;
+    }
+    return 0;
+}
+
+// Workgroups of a kernel that fit on the whole device at once (occupancy x CUs), cached
+// per (device, kernel, LDS bytes): the queries cost more than a small launch.
+static int resident_workgroups(const void* fn, size_t lds, int* out) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, size_t>, int> cache;
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_tuple(dev, fn, lds);
+    auto f = cache.find(key);
+    if (f == cache.end()) {
+        int per_cu = 0, n_cu = 0;
+        GPURIR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, lds));
+        GPURIR_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        f = cache.emplace(key, std::max(per_cu, 1) * std::max(n_cu, 1)).first;
+    }
+    *out = f->second;
+    return 0;
+}
+
 int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* d_part, const float* d_pos_src,
                       const float* d_pos_rcv, const float* d_orv_src, const float* d_orv_rcv, float* d_out,
                       hipStream_t stream) {
@@ -1708,14 +1747,11 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
 #undef GPURIR_ACC_PICK
 #undef GPURIR_ACC_FN
         }
-        if (S.lds > 64 * 1024)
-            GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds));
+        if (int rc = allow_lds(fn, S.lds)) return rc;
         if (S.counters) {
-            int per_cu = 0, dev = 0, n_cu = 0;
-            GPURIR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, S.lds));
-            GPURIR_HIP(hipGetDevice(&dev));
-            GPURIR_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-            const long long want = (long long)std::max(per_cu, 1) * n_cu;
+            int resident = 0;
+            if (int rc = resident_workgroups(fn, S.lds, &resident)) return rc;
+            const long long want = resident;
             int n_items = (int)S.P.items.size();
             const unsigned g = (unsigned)std::min<long long>(want, n_items);
             unsigned* cnt = S.counters;
@@ -1779,6 +1815,9 @@ int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src
         if (int rc = workspace(WS_PARTIAL, sim_partial_bytes(S), &pv)) return rc;
         dpart = static_cast<float*>(pv);
     }
+    unsigned* counters = nullptr;  // persistent launch: items taken longest first from a counter
+    if (int rc = device_counters(&counters)) return rc;
+    sim_set_counters(S, counters);
     return sim_enqueue(S, dplan, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
 }
 
@@ -1982,8 +2021,7 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.gp_len = A.gp_len;
         const void* fn = A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
                                 : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);
-        if (A.lds > 64 * 1024)
-            GPURIR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)A.lds));
+        if (int rc = allow_lds(fn, A.lds)) return rc;
         void* args[] = {&a};
         GPURIR_HIP(hipLaunchKernel(fn, dim3((unsigned)P.items.size()), dim3(WG), args, A.lds, stream));
     }

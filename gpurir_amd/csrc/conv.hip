@@ -113,9 +113,9 @@ __device__ __forceinline__ void dft_reg(float2* v) {
 // One Stockham pass of radix R over x[0:H] (in place): butterfly j (of H/R) reads
 // x[j + r H/R], multiplies input r by W_H^(r (j mod Ns) H/(Ns R)), does the R-point DFT
 // and writes x[(j / Ns) Ns R + (j mod Ns) + r Ns].  twH[m] = W_H^m, m < H.
-template <int R>
+template <int R, int VPT>
 __device__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, int Ns) {
-    constexpr int MAXB = 32 / R;  // butterflies per thread for H <= 16384, 512 threads
+    constexpr int MAXB = VPT / R > 0 ? VPT / R : 1;  // butterflies per thread (H <= VPT * 512)
     const int nbf = H / R;
     const int step = H / (Ns * R);
     float2 v[MAXB][R];
@@ -146,13 +146,16 @@ __device__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, 
     __syncthreads();
 }
 
-// Forward DFT of x[0:H] in place (natural order in and out), H = 2^logH <= 16384.
+// Forward DFT of x[0:H] in place (natural order in and out), H = 2^logH <= VPT * 512.
+// VPT (values per thread held across a pass) sets the register footprint: 16 keeps two
+// 512-thread workgroups per CU for H <= 8192; the largest radix is min(16, VPT).
+template <int VPT>
 __device__ void fft_stockham(float2* x, const float2* __restrict__ twH, int H, int logH) {
     int Ns = 1, bits = logH;
-    for (; bits >= 4; bits -= 4, Ns <<= 4) stockham_pass<16>(x, twH, H, Ns);
-    if (bits == 3) stockham_pass<8>(x, twH, H, Ns);
-    if (bits == 2) stockham_pass<4>(x, twH, H, Ns);
-    if (bits == 1) stockham_pass<2>(x, twH, H, Ns);
+    for (; bits >= 4; bits -= 4, Ns <<= 4) stockham_pass<16, VPT>(x, twH, H, Ns);
+    if (bits == 3) stockham_pass<8, VPT>(x, twH, H, Ns);
+    if (bits == 2) stockham_pass<4, VPT>(x, twH, H, Ns);
+    if (bits == 1) stockham_pass<2, VPT>(x, twH, H, Ns);
 }
 
 // z[n] = x[2n] + i x[2n+1] for x = src[0:len] zero-padded to 2H.
@@ -178,11 +181,12 @@ __device__ inline void split_pair(const float2* z, const float2* __restrict__ tw
     Xj = conjf2(csub(E, WO));
 }
 
+template <int VPT>
 __global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
     const int p = blockIdx.x;
     load_real(z, a.src + a.seg_start[p], a.seg_len[p], a.H);
-    fft_stockham(z, a.tw, a.H, a.logH);
+    fft_stockham<VPT>(z, a.tw, a.H, a.logH);
     float2* S = a.spec_out + (long long)p * (a.H + 1);
     for (int k = threadIdx.x; k <= a.H / 2; k += blockDim.x) {
         float2 Xk, Xj;
@@ -193,12 +197,13 @@ __global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
 }
 
 // One workgroup per (trajectory point p, receiver r): conv[p][r][0 : seg_len[p] + L - 1].
+template <int VPT>
 __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
     const int p = blockIdx.x / a.R, r = blockIdx.x - p * a.R;
     const int H = a.H, logH = a.logH;
     load_real(z, a.rirs + ((long long)p * a.R + r) * a.L, a.L, H);
-    fft_stockham(z, a.tw, H, logH);
+    fft_stockham<VPT>(z, a.tw, H, logH);
     const float2* S = a.spec + (long long)p * (H + 1);
     const float scale = 1.0f / (float)H;
     // product and inverse split, pairs (k, H-k) handled by one thread (in place)
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
         }
     }
     __syncthreads();
-    fft_stockham(z, a.tw, H, logH);
+    fft_stockham<VPT>(z, a.tw, H, logH);
     float* out = a.conv + ((long long)p * a.R + r) * a.conv_stride;
     const int len = a.seg_len[p] + a.L - 1;
     for (int t = threadIdx.x; t < len; t += blockDim.x) {
@@ -236,40 +241,69 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
 
 // out[t][r] = sum over segments p covering t of conv[p][r][t - w_ini[p]], FP64, ascending p.
 // One thread per output element (t, r), r fastest: the float64 stores of a wave are
-// contiguous and each thread has only the segments of one receiver to add.
-__global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv, const int* __restrict__ w_ini,
-                                                  int P, int R, int L, int conv_stride, long long n_out,
-                                                  double* __restrict__ out) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_out * R) return;
-    const long long t = g / R;
-    const int r = (int)(g - t * R);
-    // last p with w_ini[p] <= t
-    int lo = 0, hi = P - 1, ph = -1;
+// contiguous and each thread has only the segments of one receiver to add.  The range of
+// segments covering the workgroup's first sample is found once (binary search by thread 0);
+// each thread then steps from it to its own sample.
+__device__ inline int last_start_at_most(const int* w_ini, int P, long long t) {  // last p with w_ini[p] <= t
+    int lo = 0, hi = P - 1, res = -1;
     while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
         if ((long long)w_ini[mid] <= t) {
-            ph = mid;
+            res = mid;
             lo = mid + 1;
         } else {
             hi = mid - 1;
         }
     }
-    // first p with w_ini[p+1] + L - 1 > t
-    lo = 0;
-    hi = P - 1;
-    int pl = P;
+    return res;
+}
+
+__device__ inline int first_end_after(const int* w_ini, int P, int L, long long t) {  // first p with w_ini[p+1]+L-1 > t
+    int lo = 0, hi = P - 1, res = P;
     while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
         if ((long long)w_ini[mid + 1] + L - 1 > t) {
-            pl = mid;
+            res = mid;
             hi = mid - 1;
         } else {
             lo = mid + 1;
         }
     }
+    return res;
+}
+
+__global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv, const int* __restrict__ w_ini,
+                                                  int P, int R, int L, int conv_stride, long long n_out,
+                                                  double* __restrict__ out) {
+    __shared__ int range0[2];
+    const long long g0 = (long long)blockIdx.x * blockDim.x;
+    if (threadIdx.x == 0) {
+        const long long t0 = g0 / R;
+        range0[0] = first_end_after(w_ini, P, L, t0);
+        range0[1] = last_start_at_most(w_ini, P, t0);
+    }
+    __syncthreads();
+    const long long g = g0 + threadIdx.x;
+    if (g >= n_out * R) return;
+    const long long t = g / R;
+    const int r = (int)(g - t * R);
+    // both bounds are non-decreasing in t; t >= the workgroup's first sample
+    int pl = range0[0], ph = range0[1];
+    while (pl < P && (long long)w_ini[pl + 1] + L - 1 <= t) pl++;
+    while (ph + 1 < P && (long long)w_ini[ph + 1] <= t) ph++;
+    // up to eight loads in flight, then the float64 sum in ascending segment order
     double acc = 0.0;
-    for (int p = pl; p <= ph; p++) acc += (double)conv[((long long)p * R + r) * conv_stride + (t - w_ini[p])];
+    for (int p0 = pl; p0 <= ph; p0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int p = p0 + u;
+            v[u] = p <= ph ? conv[((long long)p * R + r) * conv_stride + (t - w_ini[p])] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (p0 + u <= ph) acc += (double)v[u];
+    }
     out[g] = acc;
 }
 
@@ -372,16 +406,18 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     a.logH = logH;
     a.conv_stride = stride;
     const size_t lds = sizeof(float2) * (size_t)H;
+    const bool big = H > 16 * CWG;  // 32 values per thread
+    const void* fs = big ? reinterpret_cast<const void*>(&segment_spectrum_kernel<32>)
+                         : reinterpret_cast<const void*>(&segment_spectrum_kernel<16>);
+    const void* fc = big ? reinterpret_cast<const void*>(&conv_kernel<32>)
+                         : reinterpret_cast<const void*>(&conv_kernel<16>);
     if (lds > 64 * 1024) {
-        GPURIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&segment_spectrum_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        GPURIR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        GPURIR_HIP(hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        GPURIR_HIP(hipFuncSetAttribute(fc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    hipLaunchKernelGGL(segment_spectrum_kernel, dim3(P), dim3(CWG), lds, stream, a);
-    GPURIR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(conv_kernel, dim3((unsigned)P * R), dim3(CWG), lds, stream, a);
-    GPURIR_HIP(hipGetLastError());
+    void* args[] = {&a};
+    GPURIR_HIP(hipLaunchKernel(fs, dim3(P), dim3(CWG), args, lds, stream));
+    GPURIR_HIP(hipLaunchKernel(fc, dim3((unsigned)P * R), dim3(CWG), args, lds, stream));
     *conv_out = a.conv;
     *conv_stride = stride;
     if (extra) *extra = (extra_bytes > 0) ? (w + b_small + b_spec + b_conv) : nullptr;

@@ -1723,6 +1723,7 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         }
 #undef GPURIR_ACC_PICK
 #undef GPURIR_ACC_FN
+        const void* fn_once = fn;  // one workgroup per item
         if (S.counters) {
             // the same instantiation in its persistent form
 #define GPURIR_ACC_FN(M, P) \
@@ -1748,15 +1749,21 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
 #undef GPURIR_ACC_FN
         }
         if (int rc = allow_lds(fn, S.lds)) return rc;
+        int resident = 0;
         if (S.counters) {
-            int resident = 0;
             if (int rc = resident_workgroups(fn, S.lds, &resident)) return rc;
-            const long long want = resident;
-            int n_items = (int)S.P.items.size();
-            const unsigned g = (unsigned)std::min<long long>(want, n_items);
+        }
+        int n_items = (int)S.P.items.size();
+        if (S.counters && n_items > resident) {
+            // more items than resident workgroups: each workgroup takes items off the counter
             unsigned* cnt = S.counters;
             void* args[] = {&a, &n_items, &cnt};
-            GPURIR_HIP(hipLaunchKernel(fn, dim3(g), dim3(WG), args, S.lds, stream));
+            GPURIR_HIP(hipLaunchKernel(fn, dim3((unsigned)resident), dim3(WG), args, S.lds, stream));
+        } else if (S.counters) {
+            // every item has a resident workgroup of its own: the counter would only add latency
+            if (int rc = allow_lds(fn_once, S.lds)) return rc;
+            void* args[] = {&a};
+            GPURIR_HIP(hipLaunchKernel(fn_once, grid, dim3(WG), args, S.lds, stream));
         } else {
             void* args[] = {&a};
             GPURIR_HIP(hipLaunchKernel(fn, grid, dim3(WG), args, S.lds, stream));

@@ -7,16 +7,16 @@
 //
 // Design: a real length-N transform is a complex length-H = N/2 transform of
 // z[n] = x[2n] + i x[2n+1] plus an O(H) split.  One workgroup holds one
-// length-H complex sequence in LDS (H <= 16384, 128 KiB) and transforms it in
-// place with a Stockham (self-sorting) FFT: passes of radix 16 (then one of 8, 4
-// or 2), each thread reading its butterflies' inputs at stride H/R, applying the
-// pass twiddles, doing the R-point DFT in registers and writing the outputs at
-// stride Ns after a barrier -- natural order in and out, log16(H) + 1 LDS round
-// trips instead of log2(H).  The inverse is the forward transform of the
-// conjugate (conjugated again on the way out).  Segment spectra are computed once per
-// trajectory point and shared by that point's receivers through L2.  The
-// overlap-add is a gather: each output sample adds the segments that cover it
-// in ascending segment order, in FP64 (deterministic).
+// length-H complex sequence in LDS (H <= 16384: 136 KiB with one float2 of padding
+// per 16 elements) and transforms it in place with a Stockham (self-sorting) FFT:
+// passes of radix 16 (then one of 8, 4 or 2), each thread reading its butterflies'
+// inputs at stride H/R, applying the pass twiddles, doing the R-point DFT in
+// registers and writing the outputs at stride Ns after a barrier -- natural order in
+// and out, log16(H) + 1 LDS round trips instead of log2(H).  The inverse is the
+// forward transform of the conjugate (conjugated again on the way out).  Segment
+// spectra are computed once per trajectory point and shared by that point's
+// receivers through L2.  The overlap-add is a gather: each output sample adds the
+// segments that cover it in ascending segment order, in FP64 (deterministic).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -55,6 +55,11 @@ __device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// LDS position of element i of a transform: one float2 of padding after every 16, so
+// the 16 lanes of one LDS cycle hit 16 different bank pairs also when each lane writes
+// a run of 16 consecutive elements (the first Stockham pass).
+__device__ __forceinline__ int pz(int i) { return i + (i >> 4); }
 
 // W16^m = exp(-2 pi i m / 16), m = 0..7
 __device__ constexpr float W16C[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f,
@@ -110,24 +115,33 @@ __device__ __forceinline__ void dft_reg(float2* v) {
     for (int k = 0; k < R; k++) v[k] = t[k];
 }
 
-// One Stockham pass of radix R over x[0:H] (in place): butterfly j (of H/R) reads
-// x[j + r H/R], multiplies input r by W_H^(r (j mod Ns) H/(Ns R)), does the R-point DFT
-// and writes x[(j / Ns) Ns R + (j mod Ns) + r Ns].  twH[m] = W_H^m, m < H.
-template <int R, int VPT>
-__device__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, int Ns) {
+// One Stockham pass of radix R over x[0:H] (in place, padded positions pz): butterfly j
+// (of H/R) reads x[j + r H/R], multiplies input r by W_H^(r (j mod Ns) H/(Ns R)), does the
+// R-point DFT and writes x[(j / Ns) Ns R + (j mod Ns) + r Ns].  twH[m] = W_H^m, m < H.
+// FIRST (Ns == 1) has no twiddles.  Inlined into the kernels, so that x is known to be
+// LDS and twH global memory: the twiddle loads of a butterfly are then issued together
+// and do not wait behind the LDS traffic.
+template <int R, int VPT, bool FIRST>
+__device__ __forceinline__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, int lNs,
+                                              int lstep) {
     constexpr int MAXB = VPT / R > 0 ? VPT / R : 1;  // butterflies per thread (H <= VPT * 512)
     const int nbf = H / R;
-    const int step = H / (Ns * R);
+    const int Ns = 1 << lNs;
     float2 v[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
         const int j = (int)threadIdx.x + q * CWG;
         if (j < nbf) {
             const int k0 = j & (Ns - 1);
+            float2 w[R];
+            if (!FIRST) {
+#pragma unroll
+                for (int r = 1; r < R; r++) w[r] = twH[(k0 * r) << lstep];
+            }
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const float2 a = x[j + r * nbf];
-                v[q][r] = (r == 0 || k0 == 0) ? a : cmul(a, twH[k0 * r * step]);
+                const float2 a = x[pz(j + r * nbf)];
+                v[q][r] = (FIRST || r == 0) ? a : cmul(a, w[r]);
             }
             dft_reg<R>(v[q]);
         }
@@ -140,7 +154,7 @@ __device__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, 
             const int k0 = j & (Ns - 1);
             const int base = (j - k0) * R + k0;
 #pragma unroll
-            for (int r = 0; r < R; r++) x[base + r * Ns] = v[q][r];
+            for (int r = 0; r < R; r++) x[pz(base + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
@@ -150,19 +164,24 @@ __device__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, 
 // VPT (values per thread held across a pass) sets the register footprint: 16 keeps two
 // 512-thread workgroups per CU for H <= 8192; the largest radix is min(16, VPT).
 template <int VPT>
-__device__ void fft_stockham(float2* x, const float2* __restrict__ twH, int H, int logH) {
-    int Ns = 1, bits = logH;
-    for (; bits >= 4; bits -= 4, Ns <<= 4) stockham_pass<16, VPT>(x, twH, H, Ns);
-    if (bits == 3) stockham_pass<8, VPT>(x, twH, H, Ns);
-    if (bits == 2) stockham_pass<4, VPT>(x, twH, H, Ns);
-    if (bits == 1) stockham_pass<2, VPT>(x, twH, H, Ns);
+__device__ __forceinline__ void fft_stockham(float2* x, const float2* __restrict__ twH, int H, int logH) {
+    int lNs = 0, bits = logH;
+    if (bits >= 4) {
+        stockham_pass<16, VPT, true>(x, twH, H, 0, 0);
+        bits -= 4;
+        lNs = 4;
+    }
+    for (; bits >= 4; bits -= 4, lNs += 4) stockham_pass<16, VPT, false>(x, twH, H, lNs, logH - lNs - 4);
+    if (bits == 3) stockham_pass<8, VPT, false>(x, twH, H, lNs, logH - lNs - 3);
+    if (bits == 2) stockham_pass<4, VPT, false>(x, twH, H, lNs, logH - lNs - 2);
+    if (bits == 1) stockham_pass<2, VPT, false>(x, twH, H, lNs, logH - lNs - 1);
 }
 
 // z[n] = x[2n] + i x[2n+1] for x = src[0:len] zero-padded to 2H.
 __device__ void load_real(float2* z, const float* __restrict__ src, int len, int H) {
     for (int n = threadIdx.x; n < H; n += blockDim.x) {
         const int e = 2 * n;
-        z[n] = make_float2(e < len ? src[e] : 0.0f, e + 1 < len ? src[e + 1] : 0.0f);
+        z[pz(n)] = make_float2(e < len ? src[e] : 0.0f, e + 1 < len ? src[e + 1] : 0.0f);
     }
     __syncthreads();
 }
@@ -171,8 +190,8 @@ __device__ void load_real(float2* z, const float* __restrict__ src, int len, int
 __device__ inline void split_pair(const float2* z, const float2* __restrict__ twN, int k, int H, float2& Xk,
                                   float2& Xj) {
     const int j = H - k;
-    const float2 A = z[k & (H - 1)];
-    const float2 Bc = conjf2(z[j & (H - 1)]);
+    const float2 A = z[pz(k & (H - 1))];
+    const float2 Bc = conjf2(z[pz(j & (H - 1))]);
     const float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
     const float2 D = csub(A, Bc);
     const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);  // -i D / 2
@@ -223,10 +242,10 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
         // both positions were read above by this thread only (pairs are disjoint); the
         // inverse transform is the forward one of the conjugate
         if (k == 0 || k == H / 2) {
-            z[k] = conjf2(Zk);
+            z[pz(k)] = conjf2(Zk);
         } else {
-            z[k] = conjf2(Zk);
-            z[j] = conjf2(Zj);
+            z[pz(k)] = conjf2(Zk);
+            z[pz(j)] = conjf2(Zj);
         }
     }
     __syncthreads();
@@ -234,16 +253,20 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
     float* out = a.conv + ((long long)p * a.R + r) * a.conv_stride;
     const int len = a.seg_len[p] + a.L - 1;
     for (int t = threadIdx.x; t < len; t += blockDim.x) {
-        const float2 v = z[t >> 1];
+        const float2 v = z[pz(t >> 1)];
         out[t] = (t & 1) ? -v.y : v.x;
     }
 }
 
 // out[t][r] = sum over segments p covering t of conv[p][r][t - w_ini[p]], FP64, ascending p.
-// One thread per output element (t, r), r fastest: the float64 stores of a wave are
-// contiguous and each thread has only the segments of one receiver to add.  The range of
-// segments covering the workgroup's first sample is found once (binary search by thread 0);
-// each thread then steps from it to its own sample.
+// A workgroup owns 64 consecutive samples (one per lane) and up to 16 receivers at a time
+// (wave w takes receivers w, w + 4, w + 8, w + 12): the reads of a wave are 64 consecutive
+// floats of one conv row, each thread keeps up to four loads in flight, and the sums go
+// through LDS so that the float64 stores of the [t][r] output are contiguous.  The range
+// of segments covering the workgroup's first sample is found once (binary search by
+// thread 0); each thread steps from it to its own sample.
+constexpr int OLA_T = 64, OLA_RPT = 4, OLA_RC = 4 * OLA_RPT;
+
 __device__ inline int last_start_at_most(const int* w_ini, int P, long long t) {  // last p with w_ini[p] <= t
     int lo = 0, hi = P - 1, res = -1;
     while (lo <= hi) {
@@ -276,35 +299,46 @@ __global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv
                                                   int P, int R, int L, int conv_stride, long long n_out,
                                                   double* __restrict__ out) {
     __shared__ int range0[2];
-    const long long g0 = (long long)blockIdx.x * blockDim.x;
+    __shared__ double tile[OLA_T * (OLA_RC + 1)];  // [sample][receiver], padded rows
+    const long long t0 = (long long)blockIdx.x * OLA_T;
     if (threadIdx.x == 0) {
-        const long long t0 = g0 / R;
         range0[0] = first_end_after(w_ini, P, L, t0);
         range0[1] = last_start_at_most(w_ini, P, t0);
     }
     __syncthreads();
-    const long long g = g0 + threadIdx.x;
-    if (g >= n_out * R) return;
-    const long long t = g / R;
-    const int r = (int)(g - t * R);
+    const int i = threadIdx.x & (OLA_T - 1), w = threadIdx.x >> 6;
+    const long long t = t0 + i;
+    const int nt = (int)(n_out - t0 < OLA_T ? n_out - t0 : OLA_T);
     // both bounds are non-decreasing in t; t >= the workgroup's first sample
     int pl = range0[0], ph = range0[1];
-    while (pl < P && (long long)w_ini[pl + 1] + L - 1 <= t) pl++;
-    while (ph + 1 < P && (long long)w_ini[ph + 1] <= t) ph++;
-    // up to eight loads in flight, then the float64 sum in ascending segment order
-    double acc = 0.0;
-    for (int p0 = pl; p0 <= ph; p0 += 8) {
-        float v[8];
+    if (t < n_out) {
+        while (pl < P && (long long)w_ini[pl + 1] + L - 1 <= t) pl++;
+        while (ph + 1 < P && (long long)w_ini[ph + 1] <= t) ph++;
+    } else {
+        ph = pl - 1;
+    }
+    for (int r0 = 0; r0 < R; r0 += OLA_RC) {
+        const int rc = R - r0 < OLA_RC ? R - r0 : OLA_RC;
+        double acc[OLA_RPT];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int p = p0 + u;
-            v[u] = p <= ph ? conv[((long long)p * R + r) * conv_stride + (t - w_ini[p])] : 0.0f;
+        for (int k = 0; k < OLA_RPT; k++) acc[k] = 0.0;
+        for (int p = pl; p <= ph; p++) {
+            const float* row = conv + ((long long)p * R + r0 + w) * conv_stride + (t - w_ini[p]);
+            float v[OLA_RPT];
+#pragma unroll
+            for (int k = 0; k < OLA_RPT; k++) v[k] = w + 4 * k < rc ? row[(long long)(4 * k) * conv_stride] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < OLA_RPT; k++) acc[k] += (double)v[k];
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-            if (p0 + u <= ph) acc += (double)v[u];
+        for (int k = 0; k < OLA_RPT; k++) tile[i * (OLA_RC + 1) + w + 4 * k] = acc[k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < nt * rc; e += 256) {
+            const int ii = e / rc, u = e - ii * rc;
+            out[(t0 + ii) * R + r0 + u] = tile[ii * (OLA_RC + 1) + u];
+        }
+        __syncthreads();
     }
-    out[g] = acc;
 }
 
 struct FftTables {
@@ -405,7 +439,7 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     a.H = H;
     a.logH = logH;
     a.conv_stride = stride;
-    const size_t lds = sizeof(float2) * (size_t)H;
+    const size_t lds = sizeof(float2) * (size_t)(H + H / 16);  // padded positions
     const bool big = H > 16 * CWG;  // 32 values per thread
     const void* fs = big ? reinterpret_cast<const void*>(&segment_spectrum_kernel<32>)
                          : reinterpret_cast<const void*>(&segment_spectrum_kernel<16>);
@@ -474,9 +508,8 @@ int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d
         return rc;
     if (int rc = upload_async(extra, w_ini, sizeof(int) * (n_pts + 1), st)) return rc;
     const long long n_out = (long long)n_signal + rir_len - 1;
-    const long long n_el = n_out * n_rcv;
-    GPURIR_CHECK(n_el / 256 < (1LL << 31), "trajectory output too long");
-    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)((n_el + 255) / 256)), dim3(256), 0, st, conv,
+    GPURIR_CHECK(n_out / OLA_T < (1LL << 31), "trajectory output too long");
+    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)((n_out + OLA_T - 1) / OLA_T)), dim3(256), 0, st, conv,
                        static_cast<const int*>(extra), n_pts, n_rcv, rir_len, stride, n_out, d_out);
     GPURIR_HIP(hipGetLastError());
     return 0;

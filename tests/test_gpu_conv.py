@@ -72,3 +72,37 @@ def test_simulate_then_trajectory_chain():
     o_rirs = orc.simulate_rir(cfg["room_sz"], cfg["beta"], cfg["pos_src"], cfg["pos_rcv"][:4], nb, Tm, cfg["fs"])
     want = orc.simulate_trajectory(cfg["signal"], o_rirs)
     assert rel_l2(got, want) < 1e-5
+
+
+def test_trajectory_wav_example_format(tmp_path):
+    # The reference's moving-source example (examples/simulate_trajectory.py:14-34): an int16
+    # WAV read with scipy, RIRs of a 64-point line trajectory seen by two cardioid mics, the
+    # float64 result written back as a WAV.  The int16 samples enter the convolution unscaled.
+    from scipy.io import wavfile
+    fs = 16000
+    rng = np.random.default_rng(7)
+    pcm = (rng.standard_normal(fs // 2) * 3000).astype(np.int16)
+    wavfile.write(tmp_path / "source_signal.wav", fs, pcm)
+    fs_r, source_signal = wavfile.read(tmp_path / "source_signal.wav")
+    assert fs_r == fs and source_signal.dtype == np.int16
+
+    room_sz = [3, 4, 2.5]
+    traj_pts = 64
+    pos_traj = np.tile(np.array([0.0, 3.0, 1.0]), (traj_pts, 1))
+    pos_traj[:, 0] = np.linspace(0.1, 2.9, traj_pts)
+    pos_rcv = np.array([[1.4, 1, 1.5], [1.6, 1, 1.5]])
+    orV_rcv = np.array([[-1, 0, 0], [1, 0, 0]])
+    beta = gpuRIR.beta_SabineEstimation(room_sz, 0.6)
+    Tdiff = gpuRIR.att2t_SabineEstimator(15.0, 0.6)
+    Tmax = 0.25
+    nb_img = gpuRIR.t2n(Tdiff, room_sz)
+    rirs = gpuRIR.simulateRIR(room_sz, beta, pos_traj, pos_rcv, nb_img, Tmax, fs, Tdiff=Tdiff, orV_rcv=orV_rcv,
+                              mic_pattern="card")
+    filtered = gpuRIR.simulateTrajectory(source_signal, rirs)
+    assert filtered.dtype == np.float64 and filtered.shape == (len(pcm) + rirs.shape[2] - 1, 2)
+    want = orc.simulate_trajectory(source_signal.astype(np.float64), rirs)
+    assert rel_l2(filtered, want) < 1e-5
+
+    wavfile.write(tmp_path / "filtered_signal.wav", fs, filtered)
+    fs_o, back = wavfile.read(tmp_path / "filtered_signal.wav")
+    assert fs_o == fs and back.dtype == np.float64 and np.array_equal(back, filtered)

@@ -302,8 +302,8 @@ __device__ inline void load_orient(const float* o, float out[4]) {
 // computed with packed FP16 instructions (offset FMA, window FMAs, product; the
 // reciprocals are FP16 too), the per-image amplitude B (with the fixed-point scale)
 // is applied in FP32 when each tap is converted, and the sum is the same 64-bit
-// fixed point as the FP32 mode.  Entry c = (acc index bits, (f, f), (cos/2, cos/2), B),
-// s = (sin/2, sin/2), with the FP16 cos/sin of 2 pi f / Tw.
+// fixed point as the FP32 mode.  Entry c = (acc index bits, B, (f, f), (cos/2, sin/2))
+// with the FP16 cos/sin of 2 pi f / Tw (s is not used), so a queue entry is one 16-byte LDS read.
 template <int MODE, int PMAX, bool MASK>
 __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, const float2* lut, int lane, float4 c,
                                            float s, const float* jf, const float* cj, const float* sj,
@@ -312,8 +312,10 @@ __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, con
     long long* ap = acc + __float_as_int(c.x) + lane;
     const float f = c.y;
     if (MODE == PREC_FP16) {
-        const half2_t f2 = bits_h2(c.y), hc = bits_h2(c.z), hs = bits_h2(s);
-        const float B = c.w;
+        // the whole entry is one 16-byte read: (acc index bits, B, (f, f), (cos/2, sin/2))
+        const half2_t f2 = bits_h2(c.z), cs = bits_h2(c.w);
+        const half2_t hc = __builtin_shufflevector(cs, cs, 0, 0), hs = __builtin_shufflevector(cs, cs, 1, 1);
+        const float B = c.y;
         const half2_t half = {(_Float16)0.5f, (_Float16)0.5f};
 #pragma unroll
         for (int pp = 0; pp < (PMAX + 1) / 2; pp++) {
@@ -483,9 +485,8 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
                             sph = sinpif(ph);
                             cph = cospif(ph);
                         }
-                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), h2_bits(h2(fh, fh)),
-                                            h2_bits(h2(0.5f * cph, 0.5f * cph)), B);
-                        qS[q] = h2_bits(h2(0.5f * sph, 0.5f * sph));
+                        q4[q] = make_float4(__int_as_float(n0 + acc_shift), B, h2_bits(h2(fh, fh)),
+                                            h2_bits(h2(0.5f * cph, 0.5f * sph)));
                     } else {
                         // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
                         // 2^-30 samples changes no tap by more than ~1e-9 relative.

@@ -1437,7 +1437,7 @@ struct Plan {
 };
 
 constexpr int MIN_CHUNK = 512;
-constexpr int MAX_CHUNK = 65536;      // images per work item at most (splits the largest rooms)
+constexpr int MAX_CHUNK = 32768;      // images per work item by default at most (splits the largest rooms)
 constexpr int TARGET_ITEMS = 2048;
 constexpr size_t LDS_PER_WG = 53 * 1024;  // three workgroups per CU (160 KiB of LDS)
 constexpr int MIN_TILE = 512;

@@ -934,18 +934,21 @@ __device__ inline void pattern_gain_d(double c, int p, double* D, double* Dp) {
 //   u = S/x, v = (C + u)/x:  sinc = sigma_k u, sinc' = -sigma_k v,
 //   sigma_k w = sigma_k/2 + (sigma_k ck) hc + (sigma_k sk) hs,  sigma_k w' = (sigma_k ck) ps - (sigma_k sk) pc,
 //   g h = g (sigma w) u,  g h' = g ((sigma w') u - (sigma w) v).
+// With r = 1/x, ws = sigma w, wps = sigma w':  g h = S (g r ws) and
+//   g h' = (g r) (A - S ws r),  A = S wps - C ws = kc.y al + kc.z be + kc.w ga,
+//   al = S ps - C hc, be = -S pc - C hs, ga = -C  (per image),
+// so a tap takes 12 VALU instructions; G0s collects g r ws and is scaled by S once per image.
 template <bool MASK>
-__device__ __forceinline__ void adj_tap(float4 kc, float g, float f, float S, float C, float hc, float hs, float pc,
-                                        float ps, float halfTw, float& G0, float& G1) {
+__device__ __forceinline__ void adj_tap(float4 kc, float g, float f, float S, float hc, float hs, float al, float be,
+                                        float ga, float halfTw, float& G0s, float& G1) {
     const float x = kc.x - f;
     if (MASK) g = fabsf(x) < halfTw ? g : 0.0f;
     const float r = __builtin_amdgcn_rcpf(x);
-    const float u = S * r;
-    const float v = (C + u) * r;
     const float ws = fmaf(kc.y, hc, fmaf(kc.z, hs, kc.w));
-    const float wps = fmaf(kc.y, ps, -kc.z * pc);
-    G0 = fmaf(g * ws, u, G0);
-    G1 = fmaf(g, fmaf(wps, u, -ws * v), G1);
+    const float A = fmaf(kc.y, al, fmaf(kc.z, be, kc.w * ga));
+    const float gr = g * r;
+    G0s = fmaf(gr, ws, G0s);
+    G1 = fmaf(gr, fmaf(-S * ws, r, A), G1);
 }
 
 // Taps k = 0 and k = 1, where |x| can be small: series for |x| < 1/16,
@@ -1059,16 +1062,18 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
         const float ph = f * a.inv_tw2;
         const float cph = cospif(ph), sph = sinpif(ph);
         const float hc = 0.5f * cph, hs = 0.5f * sph, pc = a.pi_tw * cph, pss = a.pi_tw * sph;
+        const float al = S * pss - C * hc, be = -S * pc - C * hs, ga = -C;
         const float* gk = gp + nrel;
-        float G0 = 0.0f, G1 = 0.0f;
+        float G0 = 0.0f, G0s = 0.0f, G1 = 0.0f;
 #pragma unroll 4
         for (int u = 0; u < u0; u++)
-            adj_tap<MASK>(a.ktab[u], gk[u], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+            adj_tap<MASK>(a.ktab[u], gk[u], f, S, hc, hs, al, be, ga, a.halfTw, G0s, G1);
         adj_tap_near<MASK>(a.ktab[u0], gk[u0], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
         adj_tap_near<MASK>(a.ktab[u0 + 1], gk[u0 + 1], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
 #pragma unroll 4
         for (int u = u0 + 2; u < a.nJ; u++)
-            adj_tap<MASK>(a.ktab[u], gk[u], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+            adj_tap<MASK>(a.ktab[u], gk[u], f, S, hc, hs, al, be, ga, a.halfTw, G0s, G1);
+        G0 = fmaf(S, G0s, G0);
 
         // chain rule through amp(beta, v) and tau(v) in FP64
         const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;

@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--cpu-sample-rooms", type=int, default=400)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-batch", action="store_true", help="time the batched workload only (for profiler runs)")
+    ap.add_argument("--only-trajectory", action="store_true",
+                    help="time the trajectory workload only: its RIRs, then convolution + overlap-add (profiler runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (profiler traffic reference)")
     args = ap.parse_args()
@@ -81,6 +83,50 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] reference copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
         del src, dst
+
+    def measure_trajectory():
+        # ---------------- trajectory: 128 points x 8 mics, full ISM 0.5 s, 10 s of noise
+        c5 = W.trajectory()
+        rirs5 = T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs)
+        sig5 = torch.as_tensor(c5["signal"], device=dev)
+        for _ in range(3):
+            T.trajectory(sig5, rirs5)
+        torch.cuda.synchronize()
+        # device time of the convolution + overlap-add kernels (events on the stream they run on)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        n_tr = 20
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(n_tr):
+            y5 = T.trajectory(sig5, rirs5)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_traj_host = (time.perf_counter() - t0) * 1e3 / n_tr
+        ms_traj = e0.elapsed_time(e1) / n_tr
+        P5, R5, L5 = rirs5.shape
+        n5 = sig5.numel()
+        traj_bytes = 4 * P5 * R5 * L5 + 4 * n5 + 8 * (n5 + L5 - 1) * R5  # algorithmic HBM bytes of stage (d)
+        traj_gbs = traj_bytes / (ms_traj / 1e3) / 1e9
+        # eager simulation of the 1024 trajectory RIRs (plan built on the host each call), mean of 3
+        T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs, out=rirs5)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(3):
+            T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
+                           out=rirs5)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_traj_rirs = e0.elapsed_time(e1) / 3
+        log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms; convolution+OLA {ms_traj:.3f} ms on the device "
+            f"({traj_gbs:.0f} GB/s of {traj_bytes / 1e6:.1f} MB algorithmic), {ms_traj_host:.3f} ms per call on the host")
+        del y5
+        return ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes
+
+    if args.only_trajectory:
+        ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes = measure_trajectory()
+        print(json.dumps({"metric": "trajectory (profiler run)", "rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj,
+                          "conv_ola_host_ms": ms_traj_host, "conv_ola_gbs": traj_gbs}), flush=True)
+        return
 
     # ---------------- batched: 4096 uneven rooms, one graph replay per step
     from oracle import oracle as orc
@@ -210,41 +256,7 @@ def main():
     err_mid = rel_l2(out2.cpu().numpy(), orc.simulate_rir(**kw2))
     log(f"[bench] mid room: {ms_mid:.3f} ms/call ({128 / ms_mid * 1e3:.4e} RIRs/s), rel-L2 {err_mid:.3e}")
 
-    # ---------------- trajectory: 128 points x 8 mics, full ISM 0.5 s, 10 s of noise
-    c5 = W.trajectory()
-    rirs5 = T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs)
-    sig5 = torch.as_tensor(c5["signal"], device=dev)
-    for _ in range(3):
-        T.trajectory(sig5, rirs5)
-    torch.cuda.synchronize()
-    # device time of the convolution + overlap-add kernels (events on the stream they run on)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    n_tr = 20
-    t0 = time.perf_counter()
-    e0.record()
-    for _ in range(n_tr):
-        y5 = T.trajectory(sig5, rirs5)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_traj_host = (time.perf_counter() - t0) * 1e3 / n_tr
-    ms_traj = e0.elapsed_time(e1) / n_tr
-    P5, R5, L5 = rirs5.shape
-    n5 = sig5.numel()
-    traj_bytes = 4 * P5 * R5 * L5 + 4 * n5 + 8 * (n5 + L5 - 1) * R5  # algorithmic HBM bytes of stage (d)
-    traj_gbs = traj_bytes / (ms_traj / 1e3) / 1e9
-    # eager simulation of the 1024 trajectory RIRs (plan built on the host each call), mean of 3
-    T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs, out=rirs5)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(3):
-        T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
-                       out=rirs5)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_traj_rirs = e0.elapsed_time(e1) / 3
-    log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms; convolution+OLA {ms_traj:.3f} ms on the device "
-        f"({traj_gbs:.0f} GB/s of {traj_bytes / 1e6:.1f} MB algorithmic), {ms_traj_host:.3f} ms per call on the host")
-    del y5
+    ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes = measure_trajectory()
 
     # ---------------- CPU baseline: the FP64 oracle on a bounded sample of the same batch
     cpu = None

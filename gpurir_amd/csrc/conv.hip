@@ -2,21 +2,22 @@
 //
 // Reference path (DavidDiazGuerra/gpuRIR v1.2.0):
 //   gpuRIR/__init__.py:177-221 simulateTrajectory (segmentation, float64 OLA on the host)
-//   src/gpuRIR_cuda.cu:933-1023 cuda_convolutions (cuFFT R2C/C2R, N_fft = pow2roundup(seg+L-1))
+//   src/gpuRIR_cuda.cu:933-1023 cuda_convolutions (cuFFT R2C/C2R, N_fft = pow2roundup(seg+L-1);
+//   here N_fft = 2^a or 3 2^a, whichever is smaller)
 //   src/gpuRIR_cuda.cu:463-482  complexPointwiseMulAndScale
 //
 // Design: a real length-N transform is a complex length-H = N/2 transform of
 // z[n] = x[2n] + i x[2n+1] plus an O(H) split.  One workgroup holds one
 // length-H complex sequence in LDS (H <= 16384: 136 KiB with one float2 of padding
 // per 16 elements) and transforms it in place with a Stockham (self-sorting) FFT:
-// passes of radix 16 (then one of 8, 4 or 2), each thread reading its butterflies'
-// inputs at stride H/R, applying the pass twiddles, doing the R-point DFT in
-// registers and writing the outputs at stride Ns after a barrier -- natural order in
-// and out, log16(H) + 1 LDS round trips instead of log2(H).  The inverse is the
-// forward transform of the conjugate (conjugated again on the way out).  Segment
-// spectra are computed once per trajectory point and shared by that point's
-// receivers through L2.  The overlap-add is a gather: each output sample adds the
-// segments that cover it in ascending segment order, in FP64 (deterministic).
+// passes of radix 16, then one of 8, 4 or 2, then one of 3 when H = 3 2^a; each
+// thread reads its butterflies' inputs at stride H/R, applies the pass twiddles, does
+// the R-point DFT in registers and writes the outputs at stride Ns after a barrier --
+// natural order in and out, about log16(H) + 1 LDS round trips instead of log2(H).
+// The inverse is the forward transform of the conjugate (conjugated again on the way
+// out).  Segment spectra are computed once per trajectory point and shared by that
+// point's receivers through L2.  The overlap-add is a gather: each output sample adds
+// the segments that cover it in ascending segment order, in FP64 (deterministic).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,7 +45,8 @@ struct ConvArgs {
     const float2* tw;       // [H] e^{-2 pi i m / H}
     const float2* twN;      // [H+1] e^{-2 pi i k / N}
     float* conv;            // [P][R][conv_stride]
-    int P, R, L, H, logH;
+    int P, R, L, H;
+    int l2, t3;             // H = 2^l2 * 3^t3, t3 in {0, 1}
     int conv_stride;
 };
 
@@ -115,16 +117,35 @@ __device__ __forceinline__ void dft_reg(float2* v) {
     for (int k = 0; k < R; k++) v[k] = t[k];
 }
 
+// y_k = sum_r v_r W_3^(r k), W_3 = exp(-2 pi i / 3)
+__device__ __forceinline__ void dft3(float2* v) {
+    constexpr float H3 = 0.86602540378443865f;  // sqrt(3) / 2
+    const float2 a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const float2 m = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+    const float2 d = make_float2(H3 * t2.x, H3 * t2.y);
+    v[0] = cadd(a, t1);
+    v[1] = make_float2(m.x + d.y, m.y - d.x);  // m - i d
+    v[2] = make_float2(m.x - d.y, m.y + d.x);  // m + i d
+}
+
+template <int R>
+__device__ __forceinline__ void dft_any(float2* v) {
+    if constexpr (R == 3)
+        dft3(v);
+    else
+        dft_reg<R>(v);
+}
+
 // One Stockham pass of radix R over x[0:H] (in place, padded positions pz): butterfly j
 // (of H/R) reads x[j + r H/R], multiplies input r by W_H^(r (j mod Ns) H/(Ns R)), does the
 // R-point DFT and writes x[(j / Ns) Ns R + (j mod Ns) + r Ns].  twH[m] = W_H^m, m < H.
-// FIRST (Ns == 1) has no twiddles.  Inlined into the kernels, so that x is known to be
-// LDS and twH global memory: the twiddle loads of a butterfly are then issued together
-// and do not wait behind the LDS traffic.
+// Ns is a power of two (the radix-3 pass of an H = 3 2^l2 transform comes last); FIRST
+// (Ns == 1) has no twiddles.  Inlined into the kernels, so that x is known to be LDS and
+// twH global memory: the twiddle loads of a butterfly are then issued together and do
+// not wait behind the LDS traffic.
 template <int R, int VPT, bool FIRST>
-__device__ __forceinline__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, int lNs,
-                                              int lstep) {
-    constexpr int MAXB = VPT / R > 0 ? VPT / R : 1;  // butterflies per thread (H <= VPT * 512)
+__device__ __forceinline__ void stockham_pass(float2* x, const float2* __restrict__ twH, int H, int lNs, int step) {
+    constexpr int MAXB = VPT / R > 0 ? VPT / R : 1;  // butterflies per thread
     const int nbf = H / R;
     const int Ns = 1 << lNs;
     float2 v[MAXB][R];
@@ -136,14 +157,14 @@ __device__ __forceinline__ void stockham_pass(float2* x, const float2* __restric
             float2 w[R];
             if (!FIRST) {
 #pragma unroll
-                for (int r = 1; r < R; r++) w[r] = twH[(k0 * r) << lstep];
+                for (int r = 1; r < R; r++) w[r] = twH[__mul24(k0 * r, step)];
             }
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const float2 a = x[pz(j + r * nbf)];
                 v[q][r] = (FIRST || r == 0) ? a : cmul(a, w[r]);
             }
-            dft_reg<R>(v[q]);
+            dft_any<R>(v[q]);
         }
     }
     __syncthreads();
@@ -160,21 +181,30 @@ __device__ __forceinline__ void stockham_pass(float2* x, const float2* __restric
     __syncthreads();
 }
 
-// Forward DFT of x[0:H] in place (natural order in and out), H = 2^logH <= VPT * 512.
-// VPT (values per thread held across a pass) sets the register footprint: 16 keeps two
-// 512-thread workgroups per CU for H <= 8192; the largest radix is min(16, VPT).
+// Forward DFT of x[0:H] in place (natural order in and out), H = 2^l2 3^t3 with at most
+// VPT values per thread in every pass.  VPT sets the register footprint: 16 keeps two
+// 512-thread workgroups per CU for H <= 8192 (three for H <= 6144); the largest radix is
+// min(16, VPT).  Radix-16 passes first, then one of 8, 4 or 2, then the radix-3 pass.
 template <int VPT>
-__device__ __forceinline__ void fft_stockham(float2* x, const float2* __restrict__ twH, int H, int logH) {
-    int lNs = 0, bits = logH;
+__device__ __forceinline__ void fft_stockham(float2* x, const float2* __restrict__ twH, int H, int l2, int t3) {
+    int lNs = 0, bits = l2;
+    // step = H / (Ns R) = 2^(l2 - lNs - log2 R) 3^t3
+    const int m3 = t3 ? 3 : 1;
     if (bits >= 4) {
         stockham_pass<16, VPT, true>(x, twH, H, 0, 0);
         bits -= 4;
         lNs = 4;
     }
-    for (; bits >= 4; bits -= 4, lNs += 4) stockham_pass<16, VPT, false>(x, twH, H, lNs, logH - lNs - 4);
-    if (bits == 3) stockham_pass<8, VPT, false>(x, twH, H, lNs, logH - lNs - 3);
-    if (bits == 2) stockham_pass<4, VPT, false>(x, twH, H, lNs, logH - lNs - 2);
-    if (bits == 1) stockham_pass<2, VPT, false>(x, twH, H, lNs, logH - lNs - 1);
+    for (; bits >= 4; bits -= 4, lNs += 4) stockham_pass<16, VPT, false>(x, twH, H, lNs, m3 << (l2 - lNs - 4));
+    if (bits == 3) stockham_pass<8, VPT, false>(x, twH, H, lNs, m3 << (l2 - lNs - 3));
+    if (bits == 2) stockham_pass<4, VPT, false>(x, twH, H, lNs, m3 << (l2 - lNs - 2));
+    if (bits == 1) stockham_pass<2, VPT, false>(x, twH, H, lNs, m3 << (l2 - lNs - 1));
+    if (t3) {
+        if (l2 == 0)
+            stockham_pass<3, VPT, true>(x, twH, H, 0, 0);
+        else
+            stockham_pass<3, VPT, false>(x, twH, H, l2, 1);
+    }
 }
 
 // z[n] = x[2n] + i x[2n+1] for x = src[0:len] zero-padded to 2H.
@@ -189,9 +219,9 @@ __device__ void load_real(float2* z, const float* __restrict__ src, int len, int
 // X[k] and X[H-k] of the real transform from the half-length spectrum (natural order).
 __device__ inline void split_pair(const float2* z, const float2* __restrict__ twN, int k, int H, float2& Xk,
                                   float2& Xj) {
-    const int j = H - k;
-    const float2 A = z[pz(k & (H - 1))];
-    const float2 Bc = conjf2(z[pz(j & (H - 1))]);
+    const int j = H - k;  // k in [0, H/2], so only j = H wraps (to 0)
+    const float2 A = z[pz(k)];
+    const float2 Bc = conjf2(z[pz(j == H ? 0 : j)]);
     const float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
     const float2 D = csub(A, Bc);
     const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);  // -i D / 2
@@ -205,7 +235,7 @@ __global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
     const int p = blockIdx.x;
     load_real(z, a.src + a.seg_start[p], a.seg_len[p], a.H);
-    fft_stockham<VPT>(z, a.tw, a.H, a.logH);
+    fft_stockham<VPT>(z, a.tw, a.H, a.l2, a.t3);
     float2* S = a.spec_out + (long long)p * (a.H + 1);
     for (int k = threadIdx.x; k <= a.H / 2; k += blockDim.x) {
         float2 Xk, Xj;
@@ -220,9 +250,9 @@ template <int VPT>
 __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
     const int p = blockIdx.x / a.R, r = blockIdx.x - p * a.R;
-    const int H = a.H, logH = a.logH;
+    const int H = a.H;
     load_real(z, a.rirs + ((long long)p * a.R + r) * a.L, a.L, H);
-    fft_stockham<VPT>(z, a.tw, H, logH);
+    fft_stockham<VPT>(z, a.tw, H, a.l2, a.t3);
     const float2* S = a.spec + (long long)p * (H + 1);
     const float scale = 1.0f / (float)H;
     // product and inverse split, pairs (k, H-k) handled by one thread (in place)
@@ -241,7 +271,7 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
         const float2 Zj = make_float2(E.x + O.y, -E.y + O.x);                     // conj(E) + i conj(O)
         // both positions were read above by this thread only (pairs are disjoint); the
         // inverse transform is the forward one of the conjugate
-        if (k == 0 || k == H / 2) {
+        if (k == 0 || 2 * k == H) {  // k == H - k (mod H): one position
             z[pz(k)] = conjf2(Zk);
         } else {
             z[pz(k)] = conjf2(Zk);
@@ -249,7 +279,7 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
         }
     }
     __syncthreads();
-    fft_stockham<VPT>(z, a.tw, H, logH);
+    fft_stockham<VPT>(z, a.tw, H, a.l2, a.t3);
     float* out = a.conv + ((long long)p * a.R + r) * a.conv_stride;
     const int len = a.seg_len[p] + a.L - 1;
     for (int t = threadIdx.x; t < len; t += blockDim.x) {
@@ -261,59 +291,27 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
 // out[t][r] = sum over segments p covering t of conv[p][r][t - w_ini[p]], FP64, ascending p.
 // A workgroup owns 64 consecutive samples (one per lane) and up to 16 receivers at a time
 // (wave w takes receivers w, w + 4, w + 8, w + 12): the reads of a wave are 64 consecutive
-// floats of one conv row, each thread keeps up to four loads in flight, and the sums go
-// through LDS so that the float64 stores of the [t][r] output are contiguous.  The range
-// of segments covering the workgroup's first sample is found once (binary search by
-// thread 0); each thread steps from it to its own sample.
-constexpr int OLA_T = 64, OLA_RPT = 4, OLA_RC = 4 * OLA_RPT;
+// floats of one conv row, each thread keeps up to sixteen loads in flight, and the sums go
+// through LDS so that the float64 stores of the [t][r] output are contiguous.  The host
+// gives each workgroup the segment range of its first sample and the last segment that
+// can reach it; the workgroup copies that part of w_ini into LDS (when it has at most
+// OLA_MAXSEG entries) and each thread steps from the range to its own sample.
+constexpr int OLA_T = 64, OLA_RPT = 4, OLA_RC = 4 * OLA_RPT, OLA_MAXSEG = 512;
 
-__device__ inline int last_start_at_most(const int* w_ini, int P, long long t) {  // last p with w_ini[p] <= t
-    int lo = 0, hi = P - 1, res = -1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((long long)w_ini[mid] <= t) {
-            res = mid;
-            lo = mid + 1;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    return res;
-}
-
-__device__ inline int first_end_after(const int* w_ini, int P, int L, long long t) {  // first p with w_ini[p+1]+L-1 > t
-    int lo = 0, hi = P - 1, res = P;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((long long)w_ini[mid + 1] + L - 1 > t) {
-            res = mid;
-            hi = mid - 1;
-        } else {
-            lo = mid + 1;
-        }
-    }
-    return res;
-}
-
-__global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv, const int* __restrict__ w_ini,
-                                                  int P, int R, int L, int conv_stride, long long n_out,
-                                                  double* __restrict__ out) {
-    __shared__ int range0[2];
-    __shared__ double tile[OLA_T * (OLA_RC + 1)];  // [sample][receiver], padded rows
+// rg = (first segment ending after the workgroup's first sample t0, last segment starting at
+// or before t0, last segment starting at or before its last sample); W reads w_ini[p].
+template <typename WI>
+__device__ __forceinline__ void ola_tile(const float* __restrict__ conv, WI wi, int4 rg, int P, int R, int L,
+                                         int conv_stride, long long n_out, double* __restrict__ out, double* tile) {
     const long long t0 = (long long)blockIdx.x * OLA_T;
-    if (threadIdx.x == 0) {
-        range0[0] = first_end_after(w_ini, P, L, t0);
-        range0[1] = last_start_at_most(w_ini, P, t0);
-    }
-    __syncthreads();
     const int i = threadIdx.x & (OLA_T - 1), w = threadIdx.x >> 6;
     const long long t = t0 + i;
     const int nt = (int)(n_out - t0 < OLA_T ? n_out - t0 : OLA_T);
     // both bounds are non-decreasing in t; t >= the workgroup's first sample
-    int pl = range0[0], ph = range0[1];
+    int pl = rg.x, ph = rg.y;
     if (t < n_out) {
-        while (pl < P && (long long)w_ini[pl + 1] + L - 1 <= t) pl++;
-        while (ph + 1 < P && (long long)w_ini[ph + 1] <= t) ph++;
+        while (pl < P && (long long)wi(pl + 1) + L - 1 <= t) pl++;
+        while (ph + 1 < P && (long long)wi(ph + 1) <= t) ph++;
     } else {
         ph = pl - 1;
     }
@@ -323,7 +321,7 @@ __global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv
 #pragma unroll
         for (int k = 0; k < OLA_RPT; k++) acc[k] = 0.0;
         for (int p = pl; p <= ph; p++) {
-            const float* row = conv + ((long long)p * R + r0 + w) * conv_stride + (t - w_ini[p]);
+            const float* row = conv + ((long long)p * R + r0 + w) * conv_stride + (t - wi(p));
             float v[OLA_RPT];
 #pragma unroll
             for (int k = 0; k < OLA_RPT; k++) v[k] = w + 4 * k < rc ? row[(long long)(4 * k) * conv_stride] : 0.0f;
@@ -338,6 +336,23 @@ __global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv
             out[(t0 + ii) * R + r0 + u] = tile[ii * (OLA_RC + 1) + u];
         }
         __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv, const int* __restrict__ w_ini,
+                                                  const int4* __restrict__ ranges, int P, int R, int L,
+                                                  int conv_stride, long long n_out, double* __restrict__ out) {
+    __shared__ int wsh[OLA_MAXSEG];
+    __shared__ double tile[OLA_T * (OLA_RC + 1)];  // [sample][receiver], padded rows
+    const int4 rg = ranges[blockIdx.x];
+    const int nst = rg.z + 2 - rg.x;  // w_ini[rg.x .. rg.z + 1] is all a thread reads
+    if (nst <= OLA_MAXSEG) {
+        for (int e = threadIdx.x; e < nst; e += 256) wsh[e] = w_ini[rg.x + e];
+        __syncthreads();
+        const int base = rg.x;
+        ola_tile(conv, [&](int p) { return wsh[p - base]; }, rg, P, R, L, conv_stride, n_out, out, tile);
+    } else {
+        ola_tile(conv, [&](int p) { return w_ini[p]; }, rg, P, R, L, conv_stride, n_out, out, tile);
     }
 }
 
@@ -389,10 +404,16 @@ int fft_tables_device(int H, const float2** tw, const float2** twN) {
     return 0;
 }
 
-unsigned pow2roundup(unsigned x) {
-    unsigned p = 1;
-    while (p < x) p <<= 1;
-    return p;
+// Transform length for a linear convolution of n outputs: the smallest N = 2^a or 3 2^a
+// (N even) with N >= n.  The reference rounds up to a power of two (pow2roundup,
+// src/gpuRIR_cuda.cu:95-104); any N >= n gives the same linear convolution, and 3 2^a
+// saves up to a third of the work (9249 outputs: 12288 instead of 16384).
+unsigned fft_length(unsigned n) {
+    unsigned p = 2;
+    while (p < n) p <<= 1;
+    unsigned q = 6;
+    while (q < n) q <<= 1;
+    return std::min(p, q);
 }
 
 size_t al(size_t x) { return (x + 255) / 256 * 256; }
@@ -404,11 +425,13 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
              size_t extra_bytes) {
     int seg_max = 0;
     for (int v : seg_len) seg_max = std::max(seg_max, v);
-    const unsigned N = pow2roundup((unsigned)std::max(seg_max + L - 1, 2));
+    const unsigned n_lin = (unsigned)std::max(seg_max + L - 1, 2);
+    GPURIR_CHECK(n_lin <= 32768, "convolution too long for one LDS-resident FFT (seg_len + L - 1 > 32768)");
+    const unsigned N = fft_length(n_lin);  // <= 32768
     const int H = (int)(N / 2);
-    GPURIR_CHECK(H <= 16384, "convolution too long for one LDS-resident FFT (N_fft > 32768)");
-    int logH = 0;
-    while ((1 << logH) < H) logH++;
+    const int t3 = H % 3 == 0 ? 1 : 0;
+    int l2 = 0;
+    while (((t3 ? 3 : 1) << l2) < H) l2++;
     const float2 *d_tw = nullptr, *d_twN = nullptr;
     if (int rc = fft_tables_device(H, &d_tw, &d_twN)) return rc;
     const int stride = (int)((seg_max + L - 1 + 63) / 64 * 64);
@@ -437,7 +460,8 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     a.R = R;
     a.L = L;
     a.H = H;
-    a.logH = logH;
+    a.l2 = l2;
+    a.t3 = t3;
     a.conv_stride = stride;
     const size_t lds = sizeof(float2) * (size_t)(H + H / 16);  // padded positions
     const bool big = H > 16 * CWG;  // 32 values per thread
@@ -503,14 +527,31 @@ int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d
     float* conv = nullptr;
     int stride = 0;
     void* extra = nullptr;
-    if (int rc = run_conv(d_signal, ss, sl, d_rirs, n_pts, n_rcv, rir_len, st, &conv, &stride, &extra,
-                          sizeof(int) * (n_pts + 1)))
-        return rc;
-    if (int rc = upload_async(extra, w_ini, sizeof(int) * (n_pts + 1), st)) return rc;
     const long long n_out = (long long)n_signal + rir_len - 1;
-    GPURIR_CHECK(n_out / OLA_T < (1LL << 31), "trajectory output too long");
-    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)((n_out + OLA_T - 1) / OLA_T)), dim3(256), 0, st, conv,
-                       static_cast<const int*>(extra), n_pts, n_rcv, rir_len, stride, n_out, d_out);
+    GPURIR_CHECK(n_out / OLA_T < (1LL << 28), "trajectory output too long");
+    const int n_wg = (int)((n_out + OLA_T - 1) / OLA_T);
+    // per workgroup: (first segment ending after t0, last starting at or before t0, last
+    // starting at or before t1), by three monotone sweeps
+    const size_t b_w = al(sizeof(int) * (n_pts + 1));
+    std::vector<unsigned char> host(b_w + sizeof(int4) * (size_t)n_wg, 0);
+    std::memcpy(host.data(), w_ini, sizeof(int) * (n_pts + 1));
+    int4* rg = reinterpret_cast<int4*>(host.data() + b_w);
+    {
+        int pa = 0, pb = -1, pc = -1;
+        for (int b = 0; b < n_wg; b++) {
+            const long long t0 = (long long)b * OLA_T, t1 = std::min(t0 + OLA_T - 1, n_out - 1);
+            while (pa < n_pts && (long long)w_ini[pa + 1] + rir_len - 1 <= t0) pa++;
+            while (pb + 1 < n_pts && (long long)w_ini[pb + 1] <= t0) pb++;
+            while (pc + 1 < n_pts && (long long)w_ini[pc + 1] <= t1) pc++;
+            rg[b] = make_int4(pa, pb, pc, 0);
+        }
+    }
+    if (int rc = run_conv(d_signal, ss, sl, d_rirs, n_pts, n_rcv, rir_len, st, &conv, &stride, &extra, host.size()))
+        return rc;
+    if (int rc = upload_async(extra, host.data(), host.size(), st)) return rc;
+    unsigned char* ex = static_cast<unsigned char*>(extra);
+    hipLaunchKernelGGL(ola_kernel, dim3((unsigned)n_wg), dim3(256), 0, st, conv, reinterpret_cast<const int*>(ex),
+                       reinterpret_cast<const int4*>(ex + b_w), n_pts, n_rcv, rir_len, stride, n_out, d_out);
     GPURIR_HIP(hipGetLastError());
     return 0;
 }

@@ -17,7 +17,12 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("P,S,R,L", [(3, 17, 2, 29), (8, 1250, 4, 800), (5, 100, 3, 4000), (1, 1, 1, 1)])
+# transform lengths N = 2^a or 3 2^a (N/2 = H complex points): (3,17,2,29) -> 48, (8,1250,4,800) -> 3072,
+# (5,100,3,4000) -> 6144, (1,1,1,1) -> 2, (3,2,2,2) -> 4, (2,2,1,4) -> 6 (H = 3: the radix-3 pass alone),
+# (2,5000,2,7000) -> 12288, (2,9000,1,15000) -> 24576 (32 values per thread), (1,10000,1,20000) -> 32768
+@pytest.mark.parametrize("P,S,R,L", [(3, 17, 2, 29), (8, 1250, 4, 800), (5, 100, 3, 4000), (1, 1, 1, 1),
+                                     (3, 2, 2, 2), (2, 2, 1, 4), (2, 5000, 2, 7000), (2, 9000, 1, 15000),
+                                     (1, 10000, 1, 20000)])
 def test_gpu_conv_host_entry(P, S, R, L):
     rng = np.random.default_rng(P * 1000 + L)
     seg = rng.standard_normal((P, S)).astype(np.float32)

@@ -48,6 +48,13 @@ struct ConvArgs {
     int P, R, L, H;
     int l2, t3;             // H = 2^l2 * 3^t3, t3 in {0, 1}
     int conv_stride;
+    // Partitioned convolution when seg_len + L - 1 exceeds one transform: segment p is cut
+    // into na blocks of A samples and each RIR into nb blocks of B samples (A + B - 1 <= N);
+    // block products go to blk[p][r][a][b][blk_stride] and are added up by conv_assemble_kernel.
+    // na = nb = 1: one transform per (p, r), written straight to conv.
+    int na, nb, A, B;
+    float* blk;
+    int blk_stride;
 };
 
 __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
@@ -230,13 +237,17 @@ __device__ inline void split_pair(const float2* z, const float2* __restrict__ tw
     Xj = conjf2(csub(E, WO));
 }
 
+// Samples of block q (of length Q) of a sequence of n samples: 0 for a block past the end.
+__device__ inline int block_len(int n, int q, int Q) { return max(0, min(Q, n - q * Q)); }
+
+// One workgroup per (trajectory point p, segment block a): spec[p * na + a].
 template <int VPT>
 __global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
-    const int p = blockIdx.x;
-    load_real(z, a.src + a.seg_start[p], a.seg_len[p], a.H);
+    const int p = blockIdx.x / a.na, q = blockIdx.x - p * a.na;
+    load_real(z, a.src + a.seg_start[p] + (long long)q * a.A, block_len(a.seg_len[p], q, a.A), a.H);
     fft_stockham<VPT>(z, a.tw, a.H, a.l2, a.t3);
-    float2* S = a.spec_out + (long long)p * (a.H + 1);
+    float2* S = a.spec_out + (long long)blockIdx.x * (a.H + 1);
     for (int k = threadIdx.x; k <= a.H / 2; k += blockDim.x) {
         float2 Xk, Xj;
         split_pair(z, a.twN, k, a.H, Xk, Xj);
@@ -245,15 +256,21 @@ __global__ __launch_bounds__(CWG) void segment_spectrum_kernel(ConvArgs a) {
     }
 }
 
-// One workgroup per (trajectory point p, receiver r): conv[p][r][0 : seg_len[p] + L - 1].
+// One workgroup per (trajectory point p, receiver r, segment block qa, RIR block qb), job
+// ((p R + r) na + qa) nb + qb: conv[p][r][0 : seg_len[p] + L - 1] when na = nb = 1, else the
+// product of the two blocks into blk.
 template <int VPT>
 __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
     extern __shared__ __align__(16) float2 z[];
-    const int p = blockIdx.x / a.R, r = blockIdx.x - p * a.R;
+    const int job = blockIdx.x;
+    const int qb = job % a.nb, pa = job / a.nb;
+    const int qa = pa % a.na, pr = pa / a.na;
+    const int p = pr / a.R;
     const int H = a.H;
-    load_real(z, a.rirs + ((long long)p * a.R + r) * a.L, a.L, H);
+    const int lb = block_len(a.L, qb, a.B);
+    load_real(z, a.rirs + (long long)pr * a.L + (long long)qb * a.B, lb, H);
     fft_stockham<VPT>(z, a.tw, H, a.l2, a.t3);
-    const float2* S = a.spec + (long long)p * (H + 1);
+    const float2* S = a.spec + (long long)(p * a.na + qa) * (H + 1);
     const float scale = 1.0f / (float)H;
     // product and inverse split, pairs (k, H-k) handled by one thread (in place)
     for (int k = threadIdx.x; k <= H / 2; k += blockDim.x) {
@@ -280,12 +297,41 @@ __global__ __launch_bounds__(CWG) void conv_kernel(ConvArgs a) {
     }
     __syncthreads();
     fft_stockham<VPT>(z, a.tw, H, a.l2, a.t3);
-    float* out = a.conv + ((long long)p * a.R + r) * a.conv_stride;
-    const int len = a.seg_len[p] + a.L - 1;
+    float* out;
+    int len;
+    if (a.na * a.nb == 1) {
+        out = a.conv + (long long)pr * a.conv_stride;
+        len = a.seg_len[p] + a.L - 1;
+    } else {
+        out = a.blk + (long long)job * a.blk_stride;
+        const int la = block_len(a.seg_len[p], qa, a.A);
+        len = la > 0 && lb > 0 ? la + lb - 1 : 0;
+    }
     for (int t = threadIdx.x; t < len; t += blockDim.x) {
         const float2 v = z[pz(t >> 1)];
         out[t] = (t & 1) ? -v.y : v.x;
     }
+}
+
+// conv[p][r][t] = sum over blocks (qa, qb) of blk[p][r][qa][qb][t - qa A - qb B], in
+// ascending (qa, qb) order in FP64: one thread per output sample, rows of the grid's y.
+__global__ __launch_bounds__(256) void conv_assemble_kernel(ConvArgs a) {
+    const int pr = blockIdx.y, p = pr / a.R;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int sl = a.seg_len[p];
+    if (t >= sl + a.L - 1) return;
+    double acc = 0.0;
+    for (int qa = 0; qa < a.na; qa++) {
+        const int la = block_len(sl, qa, a.A);
+        if (la == 0) break;
+        for (int qb = 0; qb < a.nb; qb++) {
+            const int lb = block_len(a.L, qb, a.B);
+            const int u = t - qa * a.A - qb * a.B;
+            if (u >= 0 && u < la + lb - 1)
+                acc += (double)a.blk[((long long)(pr * a.na + qa) * a.nb + qb) * a.blk_stride + u];
+        }
+    }
+    a.conv[(long long)pr * a.conv_stride + t] = (float)acc;
 }
 
 // out[t][r] = sum over segments p covering t of conv[p][r][t - w_ini[p]], FP64, ascending p.
@@ -425,8 +471,25 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
              size_t extra_bytes) {
     int seg_max = 0;
     for (int v : seg_len) seg_max = std::max(seg_max, v);
-    const unsigned n_lin = (unsigned)std::max(seg_max + L - 1, 2);
-    GPURIR_CHECK(n_lin <= 32768, "convolution too long for one LDS-resident FFT (seg_len + L - 1 > 32768)");
+    seg_max = std::max(seg_max, 1);
+    GPURIR_CHECK((long long)seg_max + L - 1 < (1LL << 30), "convolution too long");
+    // one transform of N <= 32768 points per (p, r) if the linear convolution fits, else
+    // blocks of the segment (A samples) and of the RIR (B samples) with A + B - 1 = 32768
+    constexpr int NMAX = 32768;
+    int A = seg_max, B = L, na = 1, nb = 1;
+    if (seg_max + L - 1 > NMAX) {
+        if (L <= NMAX / 2) {
+            A = NMAX - L + 1;
+        } else if (seg_max <= NMAX / 2) {
+            B = NMAX - seg_max + 1;
+        } else {
+            A = NMAX / 2;
+            B = NMAX / 2 + 1;
+        }
+        na = (seg_max + A - 1) / A;
+        nb = (L + B - 1) / B;
+    }
+    const unsigned n_lin = (unsigned)std::max(A + B - 1, 2);
     const unsigned N = fft_length(n_lin);  // <= 32768
     const int H = (int)(N / 2);
     const int t3 = H % 3 == 0 ? 1 : 0;
@@ -435,12 +498,16 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     const float2 *d_tw = nullptr, *d_twN = nullptr;
     if (int rc = fft_tables_device(H, &d_tw, &d_twN)) return rc;
     const int stride = (int)((seg_max + L - 1 + 63) / 64 * 64);
+    const int blk_stride = (int)((A + B - 1 + 63) / 64 * 64);
+    const long long n_jobs = (long long)P * R * na * nb;
+    GPURIR_CHECK(n_jobs < (1LL << 31) && (long long)P * na < (1LL << 31), "too many convolution blocks");
     const size_t b_ss = al(sizeof(int) * P), b_sl = al(sizeof(int) * P);
-    const size_t b_spec = al(sizeof(float2) * (size_t)P * (H + 1));
+    const size_t b_spec = al(sizeof(float2) * (size_t)P * na * (H + 1));
     const size_t b_conv = al(sizeof(float) * (size_t)P * R * stride);
+    const size_t b_blk = n_jobs > (long long)P * R ? al(sizeof(float) * (size_t)n_jobs * blk_stride) : 0;
     const size_t b_small = b_ss + b_sl;
     void* ws = nullptr;
-    if (int rc = workspace(WS_CONV, b_small + b_spec + b_conv + al(extra_bytes), &ws)) return rc;
+    if (int rc = workspace(WS_CONV, b_small + b_spec + b_conv + b_blk + al(extra_bytes), &ws)) return rc;
     unsigned char* w = static_cast<unsigned char*>(ws);
     std::vector<unsigned char> host(b_small, 0);
     std::memcpy(host.data(), seg_start.data(), sizeof(int) * P);
@@ -463,6 +530,12 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     a.l2 = l2;
     a.t3 = t3;
     a.conv_stride = stride;
+    a.na = na;
+    a.nb = nb;
+    a.A = A;
+    a.B = B;
+    a.blk = b_blk ? reinterpret_cast<float*>(w + b_small + b_spec + b_conv) : nullptr;
+    a.blk_stride = blk_stride;
     const size_t lds = sizeof(float2) * (size_t)(H + H / 16);  // padded positions
     const bool big = H > 16 * CWG;  // 32 values per thread
     const void* fs = big ? reinterpret_cast<const void*>(&segment_spectrum_kernel<32>)
@@ -474,11 +547,17 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
         GPURIR_HIP(hipFuncSetAttribute(fc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     void* args[] = {&a};
-    GPURIR_HIP(hipLaunchKernel(fs, dim3(P), dim3(CWG), args, lds, stream));
-    GPURIR_HIP(hipLaunchKernel(fc, dim3((unsigned)P * R), dim3(CWG), args, lds, stream));
+    GPURIR_HIP(hipLaunchKernel(fs, dim3((unsigned)(P * na)), dim3(CWG), args, lds, stream));
+    GPURIR_HIP(hipLaunchKernel(fc, dim3((unsigned)n_jobs), dim3(CWG), args, lds, stream));
+    if (b_blk) {
+        GPURIR_CHECK(P * R <= 65535, "too many (point, receiver) rows for a partitioned convolution");
+        hipLaunchKernelGGL(conv_assemble_kernel, dim3((unsigned)((seg_max + L - 1 + 255) / 256), (unsigned)(P * R)),
+                           dim3(256), 0, stream, a);
+        GPURIR_HIP(hipGetLastError());
+    }
     *conv_out = a.conv;
     *conv_stride = stride;
-    if (extra) *extra = (extra_bytes > 0) ? (w + b_small + b_spec + b_conv) : nullptr;
+    if (extra) *extra = (extra_bytes > 0) ? (w + b_small + b_spec + b_conv + b_blk) : nullptr;
     return 0;
 }
 

@@ -35,7 +35,7 @@ extern "C" {
 /* Evaluation of the windowed sinc (src/gpuRIR_cuda.h:29-33 flags mixed_precision / lookup_table) */
 #define GPURIR_PREC_FP32 0 /* exact FP32 sinc (reference: both flags off)            */
 #define GPURIR_PREC_LUT 1  /* 1/16-sample table + linear interpolation (lookup_table) */
-#define GPURIR_PREC_FP16 2 /* packed FP16 windowed sinc, 64-bit fixed-point sums (mixed_precision) */
+#define GPURIR_PREC_FP16 2 /* packed FP16 windowed sinc, FP32 accumulation (mixed_precision) */
 
 /* Diffuse-tail noise */
 #define GPURIR_NOISE_LOGISTIC 0 /* unit-variance logistic, as src/gpuRIR_cuda.cu:454-455 */
@@ -171,7 +171,8 @@ int gpurir_batch_plan_adjoint(gpurir_plan_t plan, const float* d_grad_out, float
 /* gpu_conv: replaces gpuRIR_bind::gpu_conv (src/python_bind.cpp:90-119) ->
  * gpuRIR_cuda::cuda_convolutions (src/gpuRIR_cuda.cu:933-1023).  Host arrays:
  * segments [M_src][seg_len], rirs [M_src][M_rcv][rir_len],
- * out [M_src][M_rcv][seg_len + rir_len - 1] (linear convolutions). */
+ * out [M_src][M_rcv][seg_len + rir_len - 1] (linear convolutions, of any length: beyond
+ * 32768 samples the segment and the RIR are cut into blocks whose products are added). */
 int gpurir_conv(const float* segments, int M_src, int seg_len, const float* rirs, int M_rcv, int rir_len,
                 float* out);
 

@@ -33,6 +33,31 @@ def test_gpu_conv_host_entry(P, S, R, L):
     assert rel_l2(out, want) < 1e-5
 
 
+# longer than one transform of 32768 points: partitioned into RIR blocks (nb = 2), segment
+# blocks (na = 2), or both
+@pytest.mark.parametrize("P,S,R,L", [(2, 1000, 2, 40000), (1, 40000, 1, 500), (1, 20000, 2, 20000)])
+def test_gpu_conv_partitioned(P, S, R, L):
+    rng = np.random.default_rng(S + L)
+    seg = rng.standard_normal((P, S)).astype(np.float32)
+    rir = (rng.standard_normal((P, R, L)) * np.exp(-np.arange(L) / (L / 4.0))).astype(np.float32)
+    out = np.zeros((P, R, S + L - 1), np.float32)
+    _lib.check(_lib.load().gpurir_conv(_lib.fptr(seg), P, S, _lib.fptr(rir), R, L, _lib.fptr(out)))
+    want = orc.gpu_conv(seg, rir)
+    assert rel_l2(out, want) < 1e-5
+
+
+def test_trajectory_long_rirs_partitioned():
+    # 1 s RIRs at 40 kHz with 4 trajectory points over 1.2 s: 12000 + 40000 - 1 samples per
+    # convolution, more than one 32768-point transform
+    rng = np.random.default_rng(8)
+    sig = rng.standard_normal(48000).astype(np.float32)
+    rirs = (rng.standard_normal((4, 3, 40000)) * np.exp(-np.arange(40000) / 8000.0)).astype(np.float32)
+    got = gpuRIR.simulateTrajectory(sig, rirs)
+    want = orc.simulate_trajectory(sig, rirs)
+    assert got.shape == (48000 + 39999, 3)
+    assert rel_l2(got, want) < 1e-5
+
+
 def test_trajectory_default_timestamps():
     rng = np.random.default_rng(4)
     sig = rng.standard_normal(20000).astype(np.float32)

@@ -107,16 +107,19 @@ def main():
         n5 = sig5.numel()
         traj_bytes = 4 * P5 * R5 * L5 + 4 * n5 + 8 * (n5 + L5 - 1) * R5  # algorithmic HBM bytes of stage (d)
         traj_gbs = traj_bytes / (ms_traj / 1e3) / 1e9
-        # eager simulation of the 1024 trajectory RIRs (plan built on the host each call), mean of 3
-        T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs, out=rirs5)
+        # eager simulation of the 1024 trajectory RIRs (plan built on the host each call): 4 warm-up
+        # calls (the first ones after lighter work run up to 20% slower), then the mean of 10
+        for _ in range(4):
+            T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
+                           out=rirs5)
         torch.cuda.synchronize()
         e0.record()
-        for _ in range(3):
+        for _ in range(10):
             T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
                            out=rirs5)
         e1.record()
         torch.cuda.synchronize()
-        ms_traj_rirs = e0.elapsed_time(e1) / 3
+        ms_traj_rirs = e0.elapsed_time(e1) / 10
         log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms; convolution+OLA {ms_traj:.3f} ms on the device "
             f"({traj_gbs:.0f} GB/s of {traj_bytes / 1e6:.1f} MB algorithmic), {ms_traj_host:.3f} ms per call on the host")
         del y5

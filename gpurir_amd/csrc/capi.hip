@@ -3,6 +3,7 @@
 // batch-plan object with hipGraph replay.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -119,7 +120,9 @@ int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
         if (s.host) GPURIR_HIP(hipHostFree(s.host));
         s.host = nullptr;
         s.size = 0;
-        const size_t want = bytes + bytes / 4 + 4096;
+        // at least 8 MiB: freeing pinned memory synchronizes the device, so a buffer that
+        // must grow between two eager calls would serialize the host with the GPU
+        const size_t want = std::max<size_t>(bytes + bytes / 2, (size_t)8 << 20);
         GPURIR_HIP(hipHostMalloc(&s.host, want, hipHostMallocDefault));
         s.size = want;
     }

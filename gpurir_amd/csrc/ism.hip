@@ -50,6 +50,10 @@ constexpr int WG = 256;
 constexpr int NWAVE = WG / 64;
 constexpr int QCAP = 512;
 constexpr int QPAD = 16;  // queue entries read ahead past the live part (4 per wave)
+// The B/2 sin values of the queue are stored per wave: image q (taken by wave q % NWAVE) at
+// qS[(q % NWAVE) * QSW + q / NWAVE], so one 16-byte read gives a wave its next four images.
+constexpr int QSW = (QCAP + QPAD) / NWAVE;
+static_assert(QSW % 4 == 0 && QSW >= QCAP / NWAVE + 4, "per-wave S rows: float4-aligned, one group of read-ahead");
 
 // Packed FP16 pair (v_pk_* f16 instructions) for the mixed-precision sinc.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
@@ -501,7 +505,7 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
                             cph = cospif(ph);
                         }
                         q4[q] = make_float4(__int_as_float(n0 + acc_shift), fc, 0.5f * B, 0.5f * B * cph);
-                        qS[q] = 0.5f * B * sph;
+                        qS[(q & (NWAVE - 1)) * QSW + q / NWAVE] = 0.5f * B * sph;
                     }
                 }
             }
@@ -513,7 +517,8 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
         // so the in-order LDS pipe never makes those reads wait behind the atomics.
         // The two register sets X and Y alternate (no copies), and the queue is read
         // through per-lane pointers with immediate offsets; reads past the live part
-        // stay inside the queue's padding and their values are never used.
+        // stay inside the queue's padding and their values are never used.  The sine
+        // terms of four images come in one read from the wave's S row.
         const int nq = __builtin_amdgcn_readfirstlane(*qcount);
         if (MODE == PREC_FP16) {
             // a word takes at most one tap per image: move the 32-bit sums into the FP32
@@ -532,28 +537,27 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
         }
         if (wave < nq) {
             const float4* pq = q4 + (tid >> 6);
-            const float* pS = qS + (tid >> 6);
+            // this wave's S row: group g holds its images 4g .. 4g + 3 (xA, xB, yA, yB)
+            const float4* pS4 = reinterpret_cast<const float4*>(qS + wave * QSW);
             float4 xA = pq[0], xB = pq[NWAVE];
-            float sxA = pS[0], sxB = pS[NWAVE];
+            float4 s4 = pS4[0];
             for (int e = wave;;) {
                 const float4 yA = pq[2 * NWAVE], yB = pq[3 * NWAVE];
-                const float syA = pS[2 * NWAVE], syB = pS[3 * NWAVE];
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xA, sxA, jf, cj, sj, hj, J2, H2, C2, S2);
-                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xB, sxB, jf, cj, sj, hj, J2, H2, C2, S2);
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xA, s4.x, jf, cj, sj, hj, J2, H2, C2, S2);
+                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xB, s4.y, jf, cj, sj, hj, J2, H2, C2, S2);
                 e += 2 * NWAVE;
                 if (e >= nq) break;
                 pq += 2 * NWAVE;
-                pS += 2 * NWAVE;
                 xA = pq[2 * NWAVE];
                 xB = pq[3 * NWAVE];
-                sxA = pS[2 * NWAVE];
-                sxB = pS[3 * NWAVE];
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yA, syA, jf, cj, sj, hj, J2, H2, C2, S2);
-                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yB, syB, jf, cj, sj, hj, J2, H2, C2, S2);
+                const float4 s4n = pS4[1];  // group <= 32: inside the row (QSW = 132 floats)
+                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yA, s4.z, jf, cj, sj, hj, J2, H2, C2, S2);
+                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yB, s4.w, jf, cj, sj, hj, J2, H2, C2, S2);
                 e += 2 * NWAVE;
                 if (e >= nq) break;
                 pq += 2 * NWAVE;
-                pS += 2 * NWAVE;
+                pS4 += 1;
+                s4 = s4n;
             }
         }
         __syncthreads();

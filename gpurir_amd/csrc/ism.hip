@@ -99,6 +99,22 @@ struct FinArgs {
 };
 
 // Integer power by squaring (e >= 0); 0^0 = 1 as powf(0,0) in the reference.
+// Correctly rounded square root of a positive normal double: the v_rsq_f64 estimate and two
+// Goldschmidt/Newton refinements with an exact residual, the sequence the compiler emits for
+// sqrt() without its range scaling (for inputs below 2^-767) and its zero/infinity/NaN
+// selects.  Image distances squared lie far inside that range, so the result equals sqrt().
+__device__ inline double sqrt_normal(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 __device__ inline double ipow(double b, int e) {
     double r = 1.0;
     while (e > 0) {
@@ -257,7 +273,7 @@ __device__ inline ImageVal image_eval(const RoomK& R, int i, const double* offs,
     double d2 = vx * vx;
     d2 = d2 + vy * vy;
     d2 = d2 + vz * vz;
-    const double dist = sqrt(d2);
+    const double dist = sqrt_normal(d2);  // d2 >= (direct-path distance)^2 > 0
     ImageVal r;
     r.tau = dist * R.fs_over_c;
     const float distf = (float)dist;

@@ -54,6 +54,9 @@ constexpr int QPAD = 16;  // queue entries read ahead past the live part (4 per 
 // qS[(q % NWAVE) * QSW + q / NWAVE], so one 16-byte read gives a wave its next four images.
 constexpr int QSW = (QCAP + QPAD) / NWAVE;
 static_assert(QSW % 4 == 0 && QSW >= QCAP / NWAVE + 4, "per-wave S rows: float4-aligned, one group of read-ahead");
+// Mixed precision: images (so taps per word) between two moves of the 32-bit sums into FP32.
+constexpr int FP16_FLUSH = 1023;
+static_assert(FP16_FLUSH >= QCAP, "a queue round must fit between two moves");
 
 // Packed FP16 pair (v_pk_* f16 instructions) for the mixed-precision sinc.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
@@ -449,7 +452,9 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
         const float bound = __int_as_float(qcount[1]) * __int_as_float(qcount[2]) * __int_as_float(qcount[3]) *
                             __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
         // |tap| * 2^E < 2^30 (64-bit sums), or < 2^21 in the 32-bit mixed-precision sums,
-        // which then take up to 2^10 taps per word between two moves into FP32
+        // which then take up to FP16_FLUSH = 1023 taps per word (< 2^31 in all) between two
+        // moves into FP32.  (The quantum, 2^-20 of the largest tap, is a large part of this
+        // mode's error: 2^-16 measured 4e-3 relative L2 on the 128-RIR directional room.)
         const int top = MODE == PREC_FP16 ? 20 : 29;
         int E = (bound > 0.0f && bound < 3.0e38f) ? top - ilogbf(bound) : 0;
         E = min(max(E, -120), 120);
@@ -538,13 +543,20 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
         const int nq = __builtin_amdgcn_readfirstlane(*qcount);
         if (MODE == PREC_FP16) {
             // a word takes at most one tap per image: move the 32-bit sums into the FP32
-            // accumulator before this round could take them past 2^10 taps
-            if (since + nq > 1024) {
-                int* lo = reinterpret_cast<int*>(acc);
-                float* hi = reinterpret_cast<float*>(lo + a.acc_len);
-                for (int e = tid; e < a.acc_len; e += WG) {
-                    hi[e] += (float)lo[e] * inv_scale;
-                    lo[e] = 0;
+            // accumulator before this round could take them past FP16_FLUSH taps
+            if (since + nq > FP16_FLUSH) {
+                // four words per thread and step (acc_len is a multiple of 4): 16-byte LDS accesses
+                int4* lo = reinterpret_cast<int4*>(acc);
+                float4* hi = reinterpret_cast<float4*>(reinterpret_cast<int*>(acc) + a.acc_len);
+                for (int e = tid; e < a.acc_len / 4; e += WG) {
+                    const int4 l = lo[e];
+                    float4 h = hi[e];
+                    h.x += (float)l.x * inv_scale;
+                    h.y += (float)l.y * inv_scale;
+                    h.z += (float)l.z * inv_scale;
+                    h.w += (float)l.w * inv_scale;
+                    hi[e] = h;
+                    lo[e] = make_int4(0, 0, 0, 0);
                 }
                 __syncthreads();
                 since = 0;

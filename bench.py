@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--cpu-sample-rooms", type=int, default=400)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-batch", action="store_true", help="time the batched workload only (for profiler runs)")
+    ap.add_argument("--only-gradients", action="store_true",
+                    help="time the 256-room forward + adjoint step only (profiler runs)")
     ap.add_argument("--only-trajectory", action="store_true",
                     help="time the trajectory workload only: its RIRs, then convolution + overlap-add (profiler runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -125,6 +127,44 @@ def main():
         del y5
         return ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes
 
+    def measure_gradients():
+        # ---------------- gradients: 256 rooms, MSE against targets simulated from perturbed
+        # betas and source positions; one step = forward (graph) + dL/dout + adjoint kernels
+        c4 = W.gradient_rooms(256, seed=0)
+        target = T.BatchPlan(W.gradient_targets(c4, seed=1), fs, seed=1234).run().clone()
+        gplan = T.BatchPlan(c4, fs, seed=1234)
+        gtune = gplan.autotune() if not args.no_autotune else None
+
+        def grad_step():
+            y = gplan.run()
+            return gplan.adjoint((2.0 / y.numel()) * (y - target))
+
+        for _ in range(max(args.warmup, 1)):
+            grads = grad_step()
+        torch.cuda.synchronize()
+        n_grad = max(args.steps // 2, 3)
+        t0 = time.perf_counter()
+        for _ in range(n_grad):
+            grads = grad_step()
+        torch.cuda.synchronize()
+        ms_grad = (time.perf_counter() - t0) * 1e3 / n_grad
+        t0 = time.perf_counter()
+        for _ in range(n_grad):
+            gplan.run()
+        torch.cuda.synchronize()
+        ms_fwd4 = (time.perf_counter() - t0) * 1e3 / n_grad
+        grads_per_s = len(c4) / (ms_grad / 1e3)
+        gfinite = all(bool(torch.isfinite(gk).all()) for gk in grads)
+        log(f"[bench] gradients: {ms_grad:.3f} ms per fwd+bwd step of {len(c4)} rooms ({grads_per_s:.4e} rooms/s), "
+            f"forward alone {ms_fwd4:.3f} ms, finite={gfinite}")
+        return ms_grad, ms_fwd4, grads_per_s, gtune
+
+    if args.only_gradients:
+        ms_grad, ms_fwd4, grads_per_s, gtune = measure_gradients()
+        print(json.dumps({"metric": "fwd+bwd grads/s (profiler run)", "value": grads_per_s, "unit": "rooms/s",
+                          "ms_per_fwd_bwd": ms_grad, "ms_fwd": ms_fwd4}), flush=True)
+        return
+
     if args.only_trajectory:
         ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes = measure_trajectory()
         print(json.dumps({"metric": "trajectory (profiler run)", "rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj,
@@ -189,35 +229,7 @@ def main():
                           "ms_per_step": ms_step, "kernel_ms": ms_acc, "taps": taps}), flush=True)
         return
 
-    # ---------------- gradients: 256 rooms, MSE against targets simulated from perturbed
-    # betas and source positions; one step = forward (graph) + dL/dout + adjoint kernels
-    c4 = W.gradient_rooms(256, seed=0)
-    target = T.BatchPlan(W.gradient_targets(c4, seed=1), fs, seed=1234).run().clone()
-    gplan = T.BatchPlan(c4, fs, seed=1234)
-    gtune = gplan.autotune() if not args.no_autotune else None
-
-    def grad_step():
-        y = gplan.run()
-        return gplan.adjoint((2.0 / y.numel()) * (y - target))
-
-    for _ in range(max(args.warmup, 1)):
-        grads = grad_step()
-    torch.cuda.synchronize()
-    n_grad = max(args.steps // 2, 3)
-    t0 = time.perf_counter()
-    for _ in range(n_grad):
-        grads = grad_step()
-    torch.cuda.synchronize()
-    ms_grad = (time.perf_counter() - t0) * 1e3 / n_grad
-    t0 = time.perf_counter()
-    for _ in range(n_grad):
-        gplan.run()
-    torch.cuda.synchronize()
-    ms_fwd4 = (time.perf_counter() - t0) * 1e3 / n_grad
-    grads_per_s = len(c4) / (ms_grad / 1e3)
-    gfinite = all(bool(torch.isfinite(gk).all()) for gk in grads)
-    log(f"[bench] gradients: {ms_grad:.3f} ms per fwd+bwd step of {len(c4)} rooms ({grads_per_s:.4e} rooms/s), "
-        f"forward alone {ms_fwd4:.3f} ms, finite={gfinite}")
+    ms_grad, ms_fwd4, grads_per_s, gtune = measure_gradients()
 
     # ---------------- plain drop-in: one room, numpy in / numpy out, FP32 exact sinc
     c1 = W.plain_room()

@@ -917,7 +917,7 @@ struct AdjArgs {
     const float* pos_rcv;
     const float* orv_src;
     const float* orv_rcv;
-    const float4* ktab;   // [nJ]: (k, sigma_k cos 2pi k/Tw, sigma_k sin 2pi k/Tw, sigma_k / 2), k = J0 + u
+    const float4* ktab;   // [nJ]: (k, cos 2pi k/Tw, sin 2pi k/Tw, sigma_k / 2), k = J0 + u, sigma_k = (-1)^(k+1)
     const float* y;       // forward output
     const float* gy;      // dL/dout, same layout
     const float* rir_cw;  // per RIR: c_w
@@ -964,23 +964,25 @@ __device__ inline void pattern_gain_d(double c, int p, double* D, double* Dp) {
 // One tap of G0, G1 at offset k (uniform constants kc = ktab[k - J0]):
 //   x = k - f, sin(pi x)/pi = sigma_k S, cos(pi x) = -sigma_k C (S = sin(pi f)/pi, C = cos(pi f)),
 //   u = S/x, v = (C + u)/x:  sinc = sigma_k u, sinc' = -sigma_k v,
-//   sigma_k w = sigma_k/2 + (sigma_k ck) hc + (sigma_k sk) hs,  sigma_k w' = (sigma_k ck) ps - (sigma_k sk) pc,
-//   g h = g (sigma w) u,  g h' = g ((sigma w') u - (sigma w) v).
-// With r = 1/x, ws = sigma w, wps = sigma w':  g h = S (g r ws) and
-//   g h' = (g r) (A - S ws r),  A = S wps - C ws = kc.y al + kc.z be + kc.w ga,
-//   al = S ps - C hc, be = -S pc - C hs, ga = -C  (per image),
-// so a tap takes 12 VALU instructions; G0s collects g r ws and is scaled by S once per image.
+//   w = 1/2 + ck hc + sk hs,  w' = ck ps - sk pc  (ck, sk = cos, sin 2 pi k/Tw),
+//   g h = g sigma_k w u,  g h' = g sigma_k (w' u - w v).
+// The sign sigma_k goes with g: the kernel stores g~(t) = (-1)^(t+1) g'(t), which is
+// sigma_k g' times (-1)^n0, the image's own sign, applied once per image.  With r = 1/x:
+//   g h = S (g r w),  g h' = (g r) (A - S w r),  A = S w' - C w = ck al + sk be + ga,
+//   al = S ps - C hc, be = -S pc - C hs, ga = -C/2  (per image),
+// so a tap takes 11 VALU instructions, each with at most one scalar operand; G0s collects
+// g r w and is scaled by S once per image.
 template <bool MASK>
 __device__ __forceinline__ void adj_tap(float4 kc, float g, float f, float S, float hc, float hs, float al, float be,
                                         float ga, float halfTw, float& G0s, float& G1) {
     const float x = kc.x - f;
     if (MASK) g = fabsf(x) < halfTw ? g : 0.0f;
     const float r = __builtin_amdgcn_rcpf(x);
-    const float ws = fmaf(kc.y, hc, fmaf(kc.z, hs, kc.w));
-    const float A = fmaf(kc.y, al, fmaf(kc.z, be, kc.w * ga));
+    const float w = fmaf(kc.y, hc, fmaf(kc.z, hs, 0.5f));
+    const float A = fmaf(kc.y, al, fmaf(kc.z, be, ga));
     const float gr = g * r;
-    G0s = fmaf(gr, ws, G0s);
-    G1 = fmaf(gr, fmaf(-S * ws, r, A), G1);
+    G0s = fmaf(gr, w, G0s);
+    G1 = fmaf(gr, fmaf(-S * w, r, A), G1);
 }
 
 // Taps k = 0 and k = 1, where |x| can be small: series for |x| < 1/16,
@@ -1003,8 +1005,8 @@ __device__ __forceinline__ void adj_tap_near(float4 kc, float g, float f, float 
         u = S * r;
         v = (C + u) * r;
     }
-    const float ws = fmaf(kc.y, hc, fmaf(kc.z, hs, kc.w));
-    const float wps = fmaf(kc.y, ps, -kc.z * pc);
+    const float ws = sg * fmaf(kc.y, hc, fmaf(kc.z, hs, 0.5f));  // sigma_k w
+    const float wps = sg * fmaf(kc.y, ps, -kc.z * pc);             // sigma_k w'
     G0 = fmaf(g * ws, u, G0);
     G1 = fmaf(g, fmaf(wps, u, -ws * v), G1);
 }
@@ -1060,7 +1062,7 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
                 g = a.gy[row + t];
                 if (t >= ws) g = fmaf(cw, a.y[row + t], g);
             }
-            gp[u] = g;
+            gp[u] = (t & 1) ? g : -g;  // (-1)^(t+1) g'(t): see adj_tap
         }
     }
     // unit orientations (sigma applied per image for the source)
@@ -1094,18 +1096,21 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
         const float ph = f * a.inv_tw2;
         const float cph = cospif(ph), sph = sinpif(ph);
         const float hc = 0.5f * cph, hs = 0.5f * sph, pc = a.pi_tw * cph, pss = a.pi_tw * sph;
-        const float al = S * pss - C * hc, be = -S * pc - C * hs, ga = -C;
+        const float al = S * pss - C * hc, be = -S * pc - C * hs, ga = -0.5f * C;
         const float* gk = gp + nrel;
-        float G0 = 0.0f, G0s = 0.0f, G1 = 0.0f;
+        // (-1)^n0 (gk[u] = (-1)^n0 sigma_k g'); sigma_0 = -1, sigma_1 = +1 for the near taps
+        const float pn = ((nrel - a.J0 + tb) & 1) ? -1.0f : 1.0f;
+        float G0 = 0.0f, G0s = 0.0f, G1 = 0.0f, G1f = 0.0f;
 #pragma unroll 4
         for (int u = 0; u < u0; u++)
-            adj_tap<MASK>(a.ktab[u], gk[u], f, S, hc, hs, al, be, ga, a.halfTw, G0s, G1);
-        adj_tap_near<MASK>(a.ktab[u0], gk[u0], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
-        adj_tap_near<MASK>(a.ktab[u0 + 1], gk[u0 + 1], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+            adj_tap<MASK>(a.ktab[u], gk[u], f, S, hc, hs, al, be, ga, a.halfTw, G0s, G1f);
+        adj_tap_near<MASK>(a.ktab[u0], -pn * gk[u0], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
+        adj_tap_near<MASK>(a.ktab[u0 + 1], pn * gk[u0 + 1], f, S, C, hc, hs, pc, pss, a.halfTw, G0, G1);
 #pragma unroll 4
         for (int u = u0 + 2; u < a.nJ; u++)
-            adj_tap<MASK>(a.ktab[u], gk[u], f, S, hc, hs, al, be, ga, a.halfTw, G0s, G1);
-        G0 = fmaf(S, G0s, G0);
+            adj_tap<MASK>(a.ktab[u], gk[u], f, S, hc, hs, al, be, ga, a.halfTw, G0s, G1f);
+        G0 = fmaf(S * pn, G0s, G0);
+        G1 = fmaf(pn, G1f, G1);
 
         // chain rule through amp(beta, v) and tau(v) in FP64
         const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
@@ -1966,7 +1971,7 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         const int k = A.T.J0 + u;
         const double sg = (k & 1) ? 1.0 : -1.0;  // (-1)^(k+1)
         const double th = 2.0 * M_PI * (double)k / (double)A.Tw;
-        A.ktab[u] = make_float4((float)k, (float)(sg * std::cos(th)), (float)(sg * std::sin(th)), (float)(0.5 * sg));
+        A.ktab[u] = make_float4((float)k, (float)std::cos(th), (float)std::sin(th), (float)(0.5 * sg));
     }
     const int n_rirs = (int)P.rir_room.size();
     A.rir_item_begin.assign(n_rirs + 1, 0);

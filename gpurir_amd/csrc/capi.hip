@@ -412,27 +412,14 @@ int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float
     o.level = 1.0f;
     const int nS = n_samples_rule(Tmax, fs);
     const size_t n_out = (size_t)M_src * M_rcv * nS;
-    const size_t ps = (size_t)M_src * 3, pr = (size_t)M_rcv * 3;
-    const size_t pad = 64;
-    const size_t total = (2 * ps + 2 * pr + 4 * pad) + n_out;
     void* wsv = nullptr;
-    if (int rc = workspace(WS_STAGE, sizeof(float) * total, &wsv)) return rc;
-    float* w = static_cast<float*>(wsv);
-    float* d_ps = w;
-    float* d_pr = d_ps + ps + pad;
-    float* d_os = d_pr + pr + pad;
-    float* d_or = d_os + ps + pad;
-    float* d_out = d_or + pr + pad;
-    // positions and orientations travel in one copy through the pinned staging buffer
+    if (int rc = workspace(WS_STAGE, sizeof(float) * n_out, &wsv)) return rc;
+    float* d_out = static_cast<float*>(wsv);
+    // positions and orientations travel with the plan in one host-to-device copy
     const hipStream_t st = nullptr;
-    std::vector<float> in(d_out - d_ps, 0.0f);
-    std::memcpy(in.data() + (d_ps - w), pos_src, sizeof(float) * ps);
-    std::memcpy(in.data() + (d_pr - w), pos_rcv, sizeof(float) * pr);
-    if (orV_src) std::memcpy(in.data() + (d_os - w), orV_src, sizeof(float) * ps);
-    if (orV_rcv) std::memcpy(in.data() + (d_or - w), orV_rcv, sizeof(float) * pr);
-    if (int rc = upload_async(w, in.data(), sizeof(float) * in.size(), st)) return rc;
-    if (int rc = gpurir_simulate_rir_device(room_sz, beta, d_ps, M_src, d_pr, M_rcv, d_os, d_or, spkr_pattern,
-                                            mic_pattern, nb_img, Tdiff, Tmax, fs, c, &o, d_out, st))
+    std::vector<RoomHost> rooms(1, room_from(room_sz, beta, nb_img, M_src, M_rcv, Tdiff, Tmax, 0u, 1.0f));
+    SimOptions so = sim_options(&o, spkr_pattern, mic_pattern, fs, c);
+    if (int rc = launch_simulation_host(rooms, pos_src, M_src, pos_rcv, M_rcv, orV_src, orV_rcv, so, d_out, st))
         return rc;
     GPURIR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * n_out, hipMemcpyDeviceToHost, st));
     GPURIR_HIP(hipStreamSynchronize(st));

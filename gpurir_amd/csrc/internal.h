@@ -146,5 +146,10 @@ int launch_image_list(const RoomHost& room, const float* d_pos_src, const float*
 int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
                       const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, float* d_out,
                       hipStream_t stream);
+// The same with the source/receiver positions and orientations in host memory ([n][3] each;
+// orientations may be null): they travel in the plan's upload, one host-to-device copy.
+int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_src, int n_src, const float* pos_rcv,
+                           int n_rcv, const float* orv_src, const float* orv_rcv, const SimOptions& opt, float* d_out,
+                           hipStream_t stream);
 
 }  // namespace gpurir

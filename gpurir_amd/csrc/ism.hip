@@ -1871,6 +1871,38 @@ int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src
     return sim_enqueue(S, dplan, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
 }
 
+int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_src, int n_src, const float* pos_rcv,
+                           int n_rcv, const float* orv_src, const float* orv_rcv, const SimOptions& opt, float* d_out,
+                           hipStream_t stream) {
+    SimLaunch S;
+    if (int rc = sim_prepare(rooms, opt, S)) return rc;
+    // plan | pos_src | pos_rcv | orv_src | orv_rcv, 256-byte aligned sections
+    const size_t bs = align_up(sizeof(float) * 3 * (size_t)n_src, 256), br = align_up(sizeof(float) * 3 * (size_t)n_rcv, 256);
+    const size_t o_ps = align_up(S.plan_bytes, 256), o_pr = o_ps + bs, o_os = o_pr + br, o_or = o_os + bs;
+    const size_t total = o_or + br;
+    S.blob.resize(total, 0);
+    unsigned char* h = S.blob.data();
+    std::memcpy(h + o_ps, pos_src, sizeof(float) * 3 * n_src);
+    std::memcpy(h + o_pr, pos_rcv, sizeof(float) * 3 * n_rcv);
+    if (orv_src) std::memcpy(h + o_os, orv_src, sizeof(float) * 3 * n_src);
+    if (orv_rcv) std::memcpy(h + o_or, orv_rcv, sizeof(float) * 3 * n_rcv);
+    void* dplan = nullptr;
+    if (int rc = workspace(WS_PLAN, total, &dplan)) return rc;
+    if (int rc = upload_async(dplan, h, total, stream)) return rc;
+    float* dpart = nullptr;
+    if (S.P.part_elems > 0) {
+        void* pv = nullptr;
+        if (int rc = workspace(WS_PARTIAL, sim_partial_bytes(S), &pv)) return rc;
+        dpart = static_cast<float*>(pv);
+    }
+    unsigned* counters = nullptr;
+    if (int rc = device_counters(&counters)) return rc;
+    sim_set_counters(S, counters);
+    const unsigned char* d = static_cast<const unsigned char*>(dplan);
+    return sim_enqueue(S, dplan, dpart, reinterpret_cast<const float*>(d + o_ps), reinterpret_cast<const float*>(d + o_pr),
+                       reinterpret_cast<const float*>(d + o_os), reinterpret_cast<const float*>(d + o_or), d_out, stream);
+}
+
 int launch_image_list(const RoomHost& room, const float* d_pos_src, const float* d_pos_rcv, const float* d_orv_src,
                       const float* d_orv_rcv, const SimOptions& opt, int* d_n0, float* d_frac, float* d_amp,
                       hipStream_t stream) {

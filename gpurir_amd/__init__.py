@@ -120,8 +120,8 @@ def simulateRIR(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, s
         mic_pattern = "omni"
     if spkr_pattern is None:
         spkr_pattern = "omni"
-    orV_rcv = np.zeros_like(pos_rcv) if orV_rcv is None else _f32c(orV_rcv)
-    orV_src = np.zeros_like(pos_src) if orV_src is None else _f32c(orV_src)
+    orV_rcv = None if orV_rcv is None else _f32c(orV_rcv)  # absent orientations go as NULL (zero vectors)
+    orV_src = None if orV_src is None else _f32c(orV_src)
     spkr = polar_patterns[spkr_pattern]  # KeyError for an unknown pattern, as the reference
     mic = polar_patterns[mic_pattern]
 
@@ -129,14 +129,16 @@ def simulateRIR(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, s
     b = _f32c(beta).reshape(6)
     nb = np.ascontiguousarray(np.asarray(nb_img, dtype=np.int32).reshape(3))
     assert pos_src.ndim == 2 and pos_src.shape[1] == 3 and pos_rcv.ndim == 2 and pos_rcv.shape[1] == 3
-    assert orV_src.shape == pos_src.shape and orV_rcv.shape == pos_rcv.shape
+    assert orV_src is None or orV_src.shape == pos_src.shape
+    assert orV_rcv is None or orV_rcv.shape == pos_rcv.shape
     lib = _lib.load()
     ns = lib.gpurir_n_samples(float(Tmax), float(fs))
     out = np.empty((pos_src.shape[0], pos_rcv.shape[0], ns), dtype=np.float32)
-    _lib.check(lib.gpurir_simulate_rir(_lib.fptr(room), _lib.fptr(b), _lib.fptr(pos_src), pos_src.shape[0],
-                                       _lib.fptr(pos_rcv), pos_rcv.shape[0], _lib.fptr(orV_src), _lib.fptr(orV_rcv),
-                                       spkr, mic, _lib.iptr(nb), float(Tdiff), float(Tmax), float(fs), float(c),
-                                       _lib.fptr(out)))
+    addr = _lib.addr
+    _lib.check(lib.gpurir_simulate_rir(addr(room), addr(b), addr(pos_src), pos_src.shape[0], addr(pos_rcv),
+                                       pos_rcv.shape[0], None if orV_src is None else addr(orV_src),
+                                       None if orV_rcv is None else addr(orV_rcv), spkr, mic, addr(nb),
+                                       float(Tdiff), float(Tmax), float(fs), float(c), addr(out)))
     return out
 
 

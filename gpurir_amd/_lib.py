@@ -34,9 +34,11 @@ _SIG = {
     "gpurir_activate_mixed_precision": (ctypes.c_int, [ctypes.c_int]),
     "gpurir_activate_lut": (ctypes.c_int, [ctypes.c_int]),
     "gpurir_set_seed": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_uint]),
-    "gpurir_simulate_rir": (ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int, c_float_p, ctypes.c_int,
-                                           c_float_p, c_float_p, ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_float,
-                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, c_float_p]),
+    # host arrays as addresses (c_void_p): the call sits on the per-RIR latency path
+    "gpurir_simulate_rir": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_float,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]),
     "gpurir_simulate_rir_device": (ctypes.c_int, [c_float_p, c_float_p, ctypes.c_void_p, ctypes.c_int,
                                                   ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_float,
@@ -123,3 +125,8 @@ def fptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(c_int_p)
+
+
+def addr(a):
+    """Address of a C-contiguous numpy array's first element (for c_void_p arguments)."""
+    return a.__array_interface__["data"][0]

@@ -399,6 +399,8 @@ int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float
                         float fs, float c, float* out) {
     if (int rc = check_common(M_src, M_rcv, spkr_pattern, mic_pattern, nb_img)) return rc;
     GPURIR_CHECK(room_sz && beta && pos_src && pos_rcv && out, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || orV_rcv, "the mics are not omni but their orientation is undefined");
     GPURIR_CHECK(Tdiff <= Tmax, "Tmax must be equal or greater than Tdiff");
     gpurir_options_t o;
     std::memset(&o, 0, sizeof(o));

@@ -82,7 +82,8 @@ int gpurir_set_seed(unsigned long long seed, unsigned int next_call_offset);
 /* simulateRIR: replaces gpuRIR_bind::simulateRIR_bind (src/python_bind.cpp:29-88)
  * -> gpuRIR_cuda::cuda_simulateRIR (src/gpuRIR_cuda.cu:766-931).  Host arrays, C order:
  * pos_src/orV_src [M_src][3], pos_rcv/orV_rcv [M_rcv][3], out [M_src][M_rcv][nSamples]
- * with nSamples = gpurir_n_samples(Tmax, fs).  Uses the drop-in precision and noise state. */
+ * with nSamples = gpurir_n_samples(Tmax, fs).  Uses the drop-in precision and noise state.
+ * orV_src / orV_rcv may be NULL when the corresponding pattern is omni (0). */
 int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float* pos_src, int M_src,
                         const float* pos_rcv, int M_rcv, const float* orV_src, const float* orV_rcv,
                         int spkr_pattern, int mic_pattern, const int nb_img[3], float Tdiff, float Tmax,

@@ -137,6 +137,10 @@ def test_c_abi_rejects_bad_arguments_without_gpu():
                                            _lib.fptr(p), _lib.fptr(p), 0, 0, _lib.iptr(nb), 0.1, 0.1, 16000.0, 343.0,
                                            _lib.fptr(out)))
     nb = np.array([3, 3, 3], np.int32)
+    with pytest.raises(ValueError, match="orientation is undefined"):
+        _lib.check(lib.gpurir_simulate_rir(_lib.addr(room), _lib.addr(beta), _lib.addr(p), 1, _lib.addr(p), 1,
+                                           None, None, 2, 0, _lib.addr(nb), 0.1, 0.1, 16000.0, 343.0,
+                                           _lib.addr(out)))
     with pytest.raises(ValueError, match="spkr_pattern"):
         _lib.check(lib.gpurir_simulate_rir(_lib.fptr(room), _lib.fptr(beta), _lib.fptr(p), 1, _lib.fptr(p), 1,
                                            _lib.fptr(p), _lib.fptr(p), 9, 0, _lib.iptr(nb), 0.1, 0.1, 16000.0, 343.0,

@@ -20,11 +20,12 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("noise", ctypes.c_int), ("seed", ctypes.c_ulonglong),
                 ("call_offset", ctypes.c_uint), ("room_index", ctypes.c_uint), ("level", ctypes.c_float),
                 ("chunk_images", ctypes.c_int), ("tile_samples", ctypes.c_int), ("persistent", ctypes.c_int),
-                ("crossfade", ctypes.c_int)]
+                ("crossfade", ctypes.c_int), ("method", ctypes.c_int)]
 
 
 PREC_FP32, PREC_LUT, PREC_FP16 = 0, 1, 2
 NOISE_LOGISTIC, NOISE_GAUSSIAN = 0, 1
+METHOD_AUTO, METHOD_TAPS, METHOD_BASIS = 0, 1, 2
 
 # every symbol of include/gpurir.h with (restype, argtypes)
 _SIG = {

@@ -178,6 +178,7 @@ SimOptions sim_options(const gpurir_options_t* o, int spkr, int mic, float fs, f
     s.chunk_images = o ? o->chunk_images : 0;
     s.tile_samples = o ? o->tile_samples : 0;
     s.crossfade = o ? std::max(o->crossfade, 0) : 0;
+    s.method = o ? o->method : METHOD_AUTO;
     s.spkr = spkr;
     s.mic = mic;
     s.fs = fs;

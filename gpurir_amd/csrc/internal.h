@@ -90,6 +90,7 @@ struct RoomHost {
 
 // precision modes (keep in sync with include/gpurir.h)
 enum { PREC_FP32 = 0, PREC_LUT = 1, PREC_FP16 = 2 };
+enum { METHOD_AUTO = 0, METHOD_TAPS = 1, METHOD_BASIS = 2 };
 
 struct SimOptions {
     int precision;
@@ -101,6 +102,7 @@ struct SimOptions {
     int chunk_images;        // 0: planner default
     int tile_samples;        // 0: planner default
     int crossfade;           // samples of cross-fade before nISM (0: hard switch)
+    int method;              // FP32: METHOD_TAPS or METHOD_BASIS (0 = basis)
 };
 
 // Plans and launches a batch of rooms whose positions and output are on the device.

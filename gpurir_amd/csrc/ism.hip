@@ -85,6 +85,10 @@ struct AccArgs {
     int lut_center, lut_row, lut_k_lo;  // LUT: center index, row length, first k of a row
     int ntab_max;         // max Nx+Ny+Nz over the rooms of the launch
     int acc_len;          // 64-bit words of the LDS accumulator
+    // basis method: reversed Chebyshev tap coefficients [NB][bcl], K steps per lane group
+    // of the Toeplitz product, and the length of one basis row of the tile
+    const float* ctab;
+    int bks, bdl, bcl;
 };
 
 struct FinArgs {
@@ -263,6 +267,29 @@ struct ImageVal {
     float amp;
 };
 
+// Polar patterns of the receiver and the source on an image's amplitude
+// (src/gpuRIR_cuda.cu:355-368): (nx, ny, nz) is the image, (vx, vy, vz) its vector
+// from the receiver, os / orr the orientations with their inverse norms in [3].
+__device__ inline float apply_patterns(const RoomK& R, float amp, double vx, double vy, double vz, float distf,
+                                       int nx, int ny, int nz, int spkr, int mic, const float os[4],
+                                       const float orr[4]) {
+    const float inv_d = __builtin_amdgcn_rcpf(distf);
+    const float fx = (float)vx, fy = (float)vy, fz = (float)vz;
+    if (mic != 0) {
+        // doa = rcv -> image, receiver orientation as given
+        const float c = (fx * orr[0] + fy * orr[1] + fz * orr[2]) * inv_d * orr[3];
+        amp *= pattern_gain(c, mic);
+    }
+    if (spkr != 0) {
+        // doa = image -> rcv, source orientation mirrored per reflected axis
+        const int qx = (nx - R.Nx / 2) & 1, qy = (ny - R.Ny / 2) & 1, qz = (nz - R.Nz / 2) & 1;
+        const float ox = qx ? -os[0] : os[0], oy = qy ? -os[1] : os[1], oz = qz ? -os[2] : os[2];
+        const float c = -(fx * ox + fy * oy + fz * oz) * inv_d * os[3];
+        amp *= pattern_gain(c, spkr);
+    }
+    return amp;
+}
+
 // Stage (a) for image i of one RIR (src/gpuRIR_cuda.cu:336-369).
 __device__ inline ImageVal image_eval(const RoomK& R, int i, const double* offs, const float* att, int spkr,
                                       int mic, const float os[4], const float orr[4]) {
@@ -282,22 +309,7 @@ __device__ inline ImageVal image_eval(const RoomK& R, int i, const double* offs,
     const float distf = (float)dist;
     float amp = att[nx] * att[Nx + ny] * att[Nx + Ny + nz];
     amp *= __builtin_amdgcn_rcpf(12.566370614359172f * distf);
-    if (spkr != 0 || mic != 0) {
-        const float inv_d = __builtin_amdgcn_rcpf(distf);
-        const float fx = (float)vx, fy = (float)vy, fz = (float)vz;
-        if (mic != 0) {
-            // doa = rcv -> image, receiver orientation as given
-            const float c = (fx * orr[0] + fy * orr[1] + fz * orr[2]) * inv_d * orr[3];
-            amp *= pattern_gain(c, mic);
-        }
-        if (spkr != 0) {
-            // doa = image -> rcv, source orientation mirrored per reflected axis
-            const int qx = (nx - Nx / 2) & 1, qy = (ny - Ny / 2) & 1, qz = (nz - Nz / 2) & 1;
-            const float ox = qx ? -os[0] : os[0], oy = qy ? -os[1] : os[1], oz = qz ? -os[2] : os[2];
-            const float c = -(fx * ox + fy * oy + fz * oz) * inv_d * os[3];
-            amp *= pattern_gain(c, spkr);
-        }
-    }
+    if (spkr != 0 || mic != 0) amp = apply_patterns(R, amp, vx, vy, vz, distf, nx, ny, nz, spkr, mic, os, orr);
     r.amp = amp;
     return r;
 }
@@ -633,6 +645,349 @@ __global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a
         __syncthreads();  // every thread has read next; the item may overwrite all of LDS
         if (item >= n_items) break;
         accumulate_item<MODE, PMAX, MASK>(a, item, smem_raw);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(counters + 1, 1u) == gridDim.x - 1) {
+            atomicExch(counters, 0u);
+            atomicExch(counters + 1, 0u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Basis method (FP32 exact sinc, the default for that mode)
+// ---------------------------------------------------------------------------
+//
+// For an image at tau = n0 + f (0 <= f < 1) the tap at offset j = t - n0 is
+// h_j(f) = hann(j - f) sinc(j - f), an entire function of f on [0, 1] for every j
+// whose window membership does not depend on f (all j for an even Tw; for an odd Tw
+// the two end offsets are cut where their taps are below 1e-7).  Its Chebyshev
+// interpolant of degree NB - 1 in u = 2 f - 1,
+//     h_j(f) ~ sum_m C[m][j] T_m(u)        (within 5e-7 of h for NB = 8),
+// turns the scatter of Tw taps per image into NB sums per integer delay,
+//     D_m[n] = sum over images with n0 = n of amp T_m(u),
+// followed by NB fixed Tw-tap filters, y[t] = sum_m sum_j C[m][j] D_m[t - j], which are
+// Toeplitz products: they run on the matrix cores (v_mfma_f32_16x16x4_f32).
+//
+// A work item is one RIR over a range of BT-sample tiles, one 512-thread workgroup.
+//   * The axis tables are sorted by |offset| once per item.  For a tile only images
+//     whose n0 can fall in [t_lo - J1, t_hi - 1 - J0] are visited: a row (x, y) is
+//     enumerated if x^2 + y^2 is inside the tile's outer radius, and its z entries in
+//     the distance shell are one contiguous range of the sorted z table (two binary
+//     searches).  Rows are taken 512 at a time (one per thread), their images written
+//     to an LDS queue by a block-wide prefix sum, and the queue is evaluated one image
+//     per lane with the same FP64 operations as image_eval (identical n0 and f).
+//   * Each live image adds amp scale T_m(u), m < NB, into D_m (64-bit fixed point,
+//     ds_add_u64: exact and order-independent), T_m by the three-term recurrence.
+//   * D is converted to FP32 in place (rows padded by 4 floats per 16, so that the
+//     16-byte operand reads of the 16 columns of a product fall in distinct banks)
+//     and the tile is y = sum_m A_m X_m with A_m[r][k] = C[m][J1 - k + r] (Toeplitz)
+//     and X_m[k][b] = D_m[16 b + k] (Hankel): 16 samples by 16 blocks per product,
+//     two products per 512-sample tile, NB/4 basis rows per wave, and the four
+//     partial sums of a sample added in a fixed order.
+constexpr int BT = 512;     // samples per tile
+constexpr int WGB = 512;    // threads per workgroup
+constexpr int NWB = WGB / 64;
+constexpr int QB = 2048;    // queue entries per round
+constexpr int BASIS_NB = 8; // Chebyshev terms
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// number of a[0..n) <= key (a ascending)
+__device__ inline int count_le(const double* a, int n, double key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// number of a[0..n) < key (a ascending)
+__device__ inline int count_lt(const double* a, int n, double key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Exclusive prefix sum of one int per thread over the WGB threads; *total gets the sum.
+// Every thread must call it (two barriers); tmp holds NWB ints.
+__device__ inline int block_scan_excl(int v, int* tmp, int* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < NWB; i++) {
+        const int t = tmp[i];
+        before += i < w ? t : 0;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+template <int NB>
+__device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
+    static_assert(NB % 4 == 0, "NB/4 basis rows per wave");
+    const Item it = a.items[item];
+    const RoomK R = load_room(a.rooms + it.room);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ntab = a.ntab_max, DL = a.bdl, CL = a.bcl, KS = a.bks;
+    const int DLP = DL + DL / 4;
+
+    double* offs = reinterpret_cast<double*>(smem_raw);  // [ntab] axis offsets (original order)
+    double* sq = offs + ntab;                            // [ntab] squared offsets, sorted per axis
+    float* att = reinterpret_cast<float*>(sq + ntab);    // [ntab]
+    int* perm = reinterpret_cast<int*>(att + ntab);      // [ntab] sorted rank -> index in the axis
+    int* rowStart = perm + ntab;                         // [ntab + 4] rows before sorted x entry ix
+    long long* D = reinterpret_cast<long long*>(rowStart + ntab + 4);  // [NB][DL] fixed point
+    float* ctab = reinterpret_cast<float*>(D + NB * DL);              // [NB][CL]
+    double* rb2 = reinterpret_cast<double*>(ctab + NB * CL);          // [WGB] row x^2 + y^2
+    float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
+    int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
+    int* queue = rxy + WGB;                                           // [QB] slot << 12 | nz
+    int* misc = queue + QB;                                           // [16] scan scratch, att maxima
+    float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
+    float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
+
+    const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
+    const float* ps = a.pos_src + 3 * (R.src_off + s);
+    const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    if (tid < 4) misc[8 + tid] = 0;
+    build_axis_tables(R, ps, pr, offs, att, tid, WGB);
+    for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
+    for (int e = tid; e < NB * CL; e += WGB) ctab[e] = a.ctab[e];
+    float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
+    if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
+    if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
+    __syncthreads();
+
+    const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz, ntot = Nx + Ny + Nz;
+    // sort every axis by |offset| (rank by counting, ties by index) and take the
+    // per-axis maxima of |att| for the fixed-point scale
+    for (int e = tid; e < ntot; e += WGB) {
+        const int ax = e < Nx ? 0 : (e < Nx + Ny ? 1 : 2);
+        const int lo = ax == 0 ? 0 : (ax == 1 ? Nx : Nx + Ny);
+        const int hi = ax == 0 ? Nx : (ax == 1 ? Nx + Ny : ntot);
+        const double v = fabs(offs[e]);
+        int rank = 0;
+        for (int k = lo; k < hi; k++) {
+            const double w = fabs(offs[k]);
+            rank += (w < v || (w == v && k < e)) ? 1 : 0;
+        }
+        sq[lo + rank] = offs[e] * offs[e];
+        perm[lo + rank] = e - lo;
+        atomicMax(misc + 9 + ax, __float_as_int(fabsf(att[e])));
+    }
+    __syncthreads();
+    float scale, inv_scale;
+    {
+        // |amp T_m| <= max|att| / (4 pi d_direct): < 2^30 after scaling, 64-bit sums
+        const float dx = ps[0] - pr[0], dy = ps[1] - pr[1], dz = ps[2] - pr[2];
+        const float bound = __int_as_float(misc[9]) * __int_as_float(misc[10]) * __int_as_float(misc[11]) *
+                            __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
+        int E = (bound > 0.0f && bound < 3.0e38f) ? 29 - ilogbf(bound) : 0;
+        E = min(max(E, -120), 120);
+        scale = ldexpf(1.0f, E);
+        inv_scale = ldexpf(1.0f, -E);
+    }
+    const double fsc = R.fs_over_c;
+    const double* sqX = sq;
+    const double* sqY = sq + Nx;
+    const double* sqZ = sq + Nx + Ny;
+    const int* pX = perm;
+    const int* pY = perm + Nx;
+    const int* pZ = perm + Nx + Ny;
+    const int J0 = a.J0, J1 = a.J0 + a.nJ - 1;
+    const bool pattern = a.spkr != 0 || a.mic != 0;
+    float* orow = a.out + R.out_off + (long long)it.rir * R.nSamples;
+
+    for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
+        const int th = min(tl + BT, it.t_hi);
+        const int lo_n0 = tl - J1, hi_n0 = th - 1 - J0;
+        // distance shell of those delays, one sample wider on each side (the exact
+        // n0 test below decides)
+        const double rlo = (double)(lo_n0 - 1) / fsc, rhi = (double)(hi_n0 + 2) / fsc;
+        const double d2lo = lo_n0 >= 1 ? rlo * rlo : -1.0, d2hi = rhi * rhi;
+        // rows: sorted y entries per sorted x entry inside the outer radius
+        int carry = 0;
+        for (int ix0 = 0; ix0 < Nx; ix0 += WGB) {
+            const int ix = ix0 + tid;
+            int c = 0;
+            if (ix < Nx) {
+                const double rem = d2hi - sqX[ix];
+                c = rem >= 0.0 ? count_le(sqY, Ny, rem) : 0;
+            }
+            int tot;
+            const int ex = block_scan_excl(c, misc, &tot);
+            if (ix < Nx) rowStart[ix + 1] = carry + ex + c;
+            carry += tot;
+        }
+        if (tid == 0) rowStart[0] = 0;
+        __syncthreads();
+        const int nrows = carry;
+        for (int r0 = 0; r0 < nrows; r0 += WGB) {
+            const int row = r0 + tid;
+            int cnt = 0, zlo = 0;
+            if (row < nrows) {
+                int lo = 0, hi = Nx;  // rowStart[lo] <= row < rowStart[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rowStart[mid] <= row) lo = mid;
+                    else hi = mid;
+                }
+                const int nx = pX[lo], ny = pY[row - rowStart[lo]];
+                const double vx = offs[nx], vy = offs[Nx + ny];
+                double b2 = vx * vx;
+                b2 = b2 + vy * vy;
+                const int zhi = count_le(sqZ, Nz, d2hi - b2);
+                zlo = count_lt(sqZ, Nz, d2lo - b2);
+                cnt = max(zhi - zlo, 0);
+                rb2[tid] = b2;
+                ratt[tid] = att[nx] * att[Nx + ny];
+                rxy[tid] = nx | (ny << 16);
+            }
+            int tot;
+            const int ex = block_scan_excl(cnt, misc, &tot);
+            for (int q0 = 0; q0 < tot; q0 += QB) {
+                const int e0 = max(ex, q0), e1 = min(ex + cnt, q0 + QB);
+                for (int e = e0; e < e1; e++) queue[e - q0] = (tid << 12) | pZ[zlo + (e - ex)];
+                __syncthreads();
+                const int nq = min(QB, tot - q0);
+                for (int k = tid; k < nq; k += WGB) {
+                    const int qe = queue[k];
+                    const int slot = qe >> 12, nz = qe & 4095;
+                    const double vz = offs[Nx + Ny + nz];
+                    double d2 = rb2[slot];
+                    d2 = d2 + vz * vz;
+                    const double dist = sqrt_normal(d2);
+                    const double tau = dist * fsc;
+                    const double fl = floor(tau);
+                    const int n0 = (int)fl;
+                    if (n0 < lo_n0 || n0 > hi_n0) continue;
+                    const float f = (float)(tau - fl);
+                    const float distf = (float)dist;
+                    float amp = ratt[slot] * att[Nx + Ny + nz];
+                    amp *= __builtin_amdgcn_rcpf(12.566370614359172f * distf);
+                    if (pattern) {
+                        const int xy = rxy[slot];
+                        const int nx = xy & 0xffff, ny = xy >> 16;
+                        amp = apply_patterns(R, amp, offs[nx], offs[Nx + ny], vz, distf, nx, ny, nz, a.spkr, a.mic,
+                                             os, orr);
+                    }
+                    if (amp == 0.0f) continue;
+                    const float v = amp * scale;
+                    const float u = fmaf(2.0f, f, -1.0f);
+                    const float u2 = u + u;
+                    long long* dp = D + (n0 - lo_n0);
+                    float b0 = v, b1 = v * u;
+                    lds_add(dp, b0);
+                    lds_add(dp + DL, b1);
+#pragma unroll
+                    for (int m = 2; m < NB; m++) {
+                        const float b2 = fmaf(u2, b1, -b0);
+                        lds_add(dp + m * DL, b2);
+                        b0 = b1;
+                        b1 = b2;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+
+        // fixed point -> FP32 in output units, in place: element e = m DL + d goes to
+        // m DLP + d + 4 (d >> 4) < 5 e / 4 + 4 floats, below the 8-byte words that later
+        // rounds read (all reads of a round precede its writes)
+        constexpr int CH = 8;
+        for (int base = 0; base < NB * DL; base += WGB * CH) {
+            long long v[CH];
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const int e = base + c * WGB + tid;
+                v[c] = e < NB * DL ? D[e] : 0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const int e = base + c * WGB + tid;
+                if (e < NB * DL) {
+                    const int m = e / DL, d = e - m * DL;
+                    Df[m * DLP + d + 4 * (d >> 4)] = (float)((double)v[c] * (double)inv_scale);
+                }
+            }
+            __syncthreads();
+        }
+
+        // Toeplitz products: wave w takes samples [256 (w & 1), +256) and basis rows
+        // (w >> 1) NB/4 .. +NB/4.  Lane (c, g) = (lane & 15, lane >> 4) supplies
+        // A[c][k] and X[k][c] for k = g KS + i, i < KS (one k per lane group and step)
+        {
+            const int ot = wave & 1, mq = wave >> 1;
+            const int c = lane & 15, g = lane >> 4;
+            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int mm = 0; mm < NB / 4; mm++) {
+                const int m = mq * (NB / 4) + mm;
+                const float* Ap = ctab + m * CL + g * KS + 16 - c;
+                const float* Bp = Df + m * DLP;
+                const int kb = 256 * ot + 16 * c + g * KS;
+                for (int t4 = 0; t4 < KS / 4; t4++) {
+                    const int dl = kb + 4 * t4;
+                    const float4 xv = *reinterpret_cast<const float4*>(Bp + dl + 4 * (dl >> 4));
+                    const float* ap = Ap + 4 * t4;
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[0], xv.x, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[1], xv.y, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[2], xv.z, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[3], xv.w, acc, 0, 0, 0);
+                }
+            }
+            // result element (row 4 g + i, column c) is sample 16 c + 4 g + i of the half
+            *reinterpret_cast<float4*>(ybuf + mq * BT + 256 * ot + 16 * c + 4 * g) =
+                make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        __syncthreads();
+        for (int t = tid; t < th - tl; t += WGB)
+            orow[tl + t] = ((ybuf[t] + ybuf[BT + t]) + ybuf[2 * BT + t]) + ybuf[3 * BT + t];
+        __syncthreads();
+        for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
+        // the row scan of the next tile has barriers before its first atomic
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(WGB) void ism_basis_kernel(AccArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    basis_item<NB>(a, blockIdx.x, smem_raw);
+}
+
+// Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
+template <int NB>
+__global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(counters, 1u);
+        __syncthreads();
+        const int item = next;
+        __syncthreads();
+        if (item >= n_items) break;
+        basis_item<NB>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -1590,6 +1945,116 @@ Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const Tap
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Basis method tables: Chebyshev interpolation coefficients of every tap offset
+// (FP64 on the host, NB Chebyshev nodes in f), stored reversed in j with 16 zeros in
+// front, ctab[m][16 + x] = C[m][J1 - x], for the Toeplitz operand of the products.
+struct BasisTables {
+    int ks = 0;   // K steps per lane group: 4 ks >= 16 + nJ - 1, ks a multiple of 4
+    int cl = 0;   // row length of ctab: 4 ks + 16
+    int dl = 0;   // basis row of a tile: >= BT + nJ - 1 and >= 496 + 4 ks, a multiple of 16
+    std::vector<float> ctab;
+};
+
+double windowed_sinc(double x, int Tw) {
+    if (!(std::fabs(x) < Tw / 2.0)) return 0.0;  // src/gpuRIR_cuda.cu:110-121
+    const double w = 0.5 * (1.0 + std::cos(2.0 * M_PI * x / Tw));
+    return x == 0.0 ? w : w * std::sin(M_PI * x) / (M_PI * x);
+}
+
+BasisTables make_basis_tables(const TapTables& T, int Tw) {
+    constexpr int NB = BASIS_NB;
+    BasisTables B;
+    B.ks = 4 * ((T.nJ + 15 + 15) / 16);
+    B.cl = 4 * B.ks + 16;
+    B.dl = (int)align_up((size_t)std::max(BT + T.nJ - 1, 496 + 4 * B.ks), 16);
+    B.ctab.assign((size_t)NB * B.cl, 0.0f);
+    const int J1 = T.J0 + T.nJ - 1;
+    double fn[NB], val[NB];
+    for (int i = 0; i < NB; i++) fn[i] = 0.5 + 0.5 * std::cos(M_PI * (i + 0.5) / NB);  // u_i = 2 f_i - 1
+    for (int x = 0; x < T.nJ; x++) {
+        const int j = J1 - x;
+        for (int i = 0; i < NB; i++) val[i] = windowed_sinc((double)j - fn[i], Tw);
+        for (int m = 0; m < NB; m++) {
+            double c = 0.0;
+            for (int i = 0; i < NB; i++) c += val[i] * std::cos(m * M_PI * (i + 0.5) / NB);
+            c *= (m == 0 ? 1.0 : 2.0) / NB;
+            B.ctab[(size_t)m * B.cl + 16 + x] = (float)c;
+        }
+    }
+    return B;
+}
+
+size_t basis_lds_bytes(int ntab, const BasisTables& B) {
+    return 8 * (size_t)ntab * 2 + 4 * (size_t)ntab * 2 + 4 * ((size_t)ntab + 4) + 8 * (size_t)BASIS_NB * B.dl +
+           4 * (size_t)BASIS_NB * B.cl + (8 + 4 + 4) * (size_t)WGB + 4 * (size_t)QB + 4 * 16;
+}
+
+// Basis plan: one item per RIR and range of BT-sample tiles (the whole early part
+// by default, or tile_samples rounded up to BT), longest expected work first: the
+// images in a shell grow with the cube of its outer radius and with 1 / volume.
+Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt) {
+    Plan P;
+    int ntab = 0;
+    for (const RoomHost& h : rh) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
+    P.ntab_max = ntab;
+    std::vector<std::pair<double, Item>> work;
+    long long rir_off = 0;
+    for (size_t k = 0; k < rh.size(); k++) {
+        const RoomHost& h = rh[k];
+        RoomDev R;
+        std::memset(&R, 0, sizeof(R));
+        for (int d = 0; d < 3; d++) {
+            R.L[d] = h.L[d];
+            R.nb[d] = h.nb[d];
+        }
+        for (int d = 0; d < 6; d++) R.beta[d] = h.beta[d];
+        R.fs_over_c = (double)opt.fs / (double)opt.c;
+        R.inv_nyz = 1.0 / ((double)h.nb[1] * (double)h.nb[2]);
+        R.inv_nz = 1.0 / (double)h.nb[2];
+        R.n_src = h.n_src;
+        R.n_rcv = h.n_rcv;
+        R.src_off = h.src_off;
+        R.rcv_off = h.rcv_off;
+        R.nISM = n_samples_rule(h.Tdiff, opt.fs);
+        R.nSamples = n_samples_rule(h.Tmax, opt.fs);
+        R.rir_off = (int)rir_off;
+        R.out_off = h.out_off;
+        R.room_id = h.room_id;
+        R.level = h.level;
+        R.n_chunks = 1;
+        const int N = h.nb[0] * h.nb[1] * h.nb[2];
+        const int M = h.n_src * h.n_rcv;
+        int per = opt.tile_samples > 0 ? (int)align_up((size_t)opt.tile_samples, BT) : (int)align_up((size_t)R.nISM, BT);
+        per = std::max(per, BT);
+        const double vol = (double)h.L[0] * h.L[1] * h.L[2];
+        for (int m = 0; m < M; m++) {
+            P.rir_room.push_back((int)k);
+            for (int t0 = 0; t0 < R.nISM; t0 += per) {
+                Item it;
+                it.room = (int)k;
+                it.rir = m;
+                it.i0 = 0;
+                it.i1 = N;
+                it.t_lo = t0;
+                it.t_hi = std::min(R.nISM, t0 + per);
+                it.chunk = 0;
+                it.pad = 0;
+                const double r1 = (double)(it.t_hi + Tt.nJ), r0 = (double)std::max(it.t_lo - Tt.nJ, 0);
+                const double tiles = (double)((it.t_hi - it.t_lo + BT - 1) / BT);
+                work.emplace_back((r1 * r1 * r1 - r0 * r0 * r0) / vol + 64.0 * tiles, it);
+            }
+        }
+        rir_off += M;
+        P.rooms.push_back(R);
+    }
+    std::stable_sort(work.begin(), work.end(),
+                     [](const std::pair<double, Item>& x, const std::pair<double, Item>& y) { return x.first > y.first; });
+    P.items.reserve(work.size());
+    for (const auto& w : work) P.items.push_back(w.second);
+    P.tt_max = BT;
+    return P;
+}
+
 }  // namespace
 
 // Host description of one simulation launch (two kernels).  The plan blob
@@ -1598,9 +2063,11 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 struct SimLaunch {
     Plan P;
     TapTables T;
+    BasisTables Bt;
+    bool basis = false;  // FP32 accumulation by the basis method
     int prec = PREC_FP32;
     int Tw = 0;
-    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_jt = 0, off_lut = 0, off_sb = 0, plan_bytes = 0;
+    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_jt = 0, off_lut = 0, off_sb = 0, off_ct = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
     size_t lds = 0;
     int acc_len = 0, ntab = 0;
@@ -1626,7 +2093,15 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     GPURIR_CHECK(S.Tw >= 2, "sampling frequency too low for the 8 ms window");
     S.prec = opt.precision;
     S.T = make_tap_tables(S.prec, S.Tw);
-    S.P = make_plan(rooms, opt, S.T, true);
+    GPURIR_CHECK(opt.method >= METHOD_AUTO && opt.method <= METHOD_BASIS, "unknown accumulation method");
+    S.basis = S.prec == PREC_FP32 && opt.method != METHOD_TAPS;
+    if (S.basis) {
+        S.Bt = make_basis_tables(S.T, S.Tw);
+        S.P = make_plan_basis(rooms, opt, S.T);
+    } else {
+        S.Bt = BasisTables();
+        S.P = make_plan(rooms, opt, S.T, true);
+    }
     const Plan& P = S.P;
     const TapTables& T = S.T;
     S.off_rooms = 0;
@@ -1635,7 +2110,8 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.off_jt = align_up(S.off_rr + sizeof(int) * P.rir_room.size(), 256);
     S.off_lut = align_up(S.off_jt + sizeof(float4) * T.jtab.size(), 256);
     S.off_sb = align_up(S.off_lut + sizeof(float2) * (T.lut.size() + 1), 256);
-    S.plan_bytes = align_up(S.off_sb + sizeof(int2) * (P.sum_blocks.size() + 1), 256);
+    S.off_ct = align_up(S.off_sb + sizeof(int2) * (P.sum_blocks.size() + 1), 256);
+    S.plan_bytes = align_up(S.off_ct + sizeof(float) * (S.Bt.ctab.size() + 1), 256);
     S.blob.assign(S.plan_bytes, 0);
     std::memcpy(S.blob.data() + S.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(S.blob.data() + S.off_items, P.items.data(), sizeof(Item) * P.items.size());
@@ -1644,11 +2120,14 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     if (!T.lut.empty()) std::memcpy(S.blob.data() + S.off_lut, T.lut.data(), sizeof(float2) * T.lut.size());
     if (!P.sum_blocks.empty())
         std::memcpy(S.blob.data() + S.off_sb, P.sum_blocks.data(), sizeof(int2) * P.sum_blocks.size());
+    if (!S.Bt.ctab.empty())
+        std::memcpy(S.blob.data() + S.off_ct, S.Bt.ctab.data(), sizeof(float) * S.Bt.ctab.size());
     S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
     // acc index = n0 - t_lo + J1 + (j - J0), n0 <= t_hi - 1 - J0, j - J0 < 64 n_pass
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
             (size_t)(QCAP + QPAD) * 5 * 4 + 16 + sizeof(float2) * T.lut.size();
+    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt);
     GPURIR_CHECK(T.n_pass <= 12, "sampling frequency too high: at most 768 window taps are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -1685,6 +2164,10 @@ static AccArgs acc_args(const SimLaunch& S, const void* d_plan, float* d_part, c
     a.inv_tw2 = 2.0f / (float)S.Tw;
     a.ntab_max = S.ntab;
     a.acc_len = S.acc_len;
+    a.ctab = reinterpret_cast<const float*>(dp + S.off_ct);
+    a.bks = S.Bt.ks;
+    a.bdl = S.Bt.dl;
+    a.bcl = S.Bt.cl;
     return a;
 }
 
@@ -1717,7 +2200,7 @@ static int allow_lds(const void* fn, size_t lds) {
 
 // Workgroups of a kernel that fit on the whole device at once (occupancy x CUs), cached
 // per (device, kernel, LDS bytes): the queries cost more than a small launch.
-static int resident_workgroups(const void* fn, size_t lds, int* out) {
+static int resident_workgroups(const void* fn, size_t lds, int* out, int block = WG) {
     static std::mutex mu;
     static std::map<std::tuple<int, const void*, size_t>, int> cache;
     int dev = 0;
@@ -1727,7 +2210,7 @@ static int resident_workgroups(const void* fn, size_t lds, int* out) {
     auto f = cache.find(key);
     if (f == cache.end()) {
         int per_cu = 0, n_cu = 0;
-        GPURIR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, lds));
+        GPURIR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds));
         GPURIR_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
         f = cache.emplace(key, std::max(per_cu, 1) * std::max(n_cu, 1)).first;
     }
@@ -1740,7 +2223,26 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
                       hipStream_t stream) {
     const unsigned char* dp = static_cast<const unsigned char*>(d_plan);
     const RoomDev* d_rooms = reinterpret_cast<const RoomDev*>(dp + S.off_rooms);
-    if ((stage & 1) && !S.P.items.empty()) {
+    if ((stage & 1) && !S.P.items.empty() && S.basis) {
+        AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
+        const void* fn_once = reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB>);
+        const void* fn_pers = reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB>);
+        int n_items = (int)S.P.items.size();
+        int resident = 0;
+        if (S.counters) {
+            if (int rc = allow_lds(fn_pers, S.lds)) return rc;
+            if (int rc = resident_workgroups(fn_pers, S.lds, &resident, WGB)) return rc;
+        }
+        if (S.counters && n_items > resident) {
+            unsigned* cnt = S.counters;
+            void* args[] = {&a, &n_items, &cnt};
+            GPURIR_HIP(hipLaunchKernel(fn_pers, dim3((unsigned)resident), dim3(WGB), args, S.lds, stream));
+        } else {
+            if (int rc = allow_lds(fn_once, S.lds)) return rc;
+            void* args[] = {&a};
+            GPURIR_HIP(hipLaunchKernel(fn_once, dim3((unsigned)n_items), dim3(WGB), args, S.lds, stream));
+        }
+    } else if ((stage & 1) && !S.P.items.empty()) {
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
         const dim3 grid((unsigned)S.P.items.size());
         const void* fn = nullptr;

@@ -36,8 +36,11 @@ def _stream(dev):
     return torch.cuda.current_stream(dev).cuda_stream
 
 
+METHODS = {"auto": _lib.METHOD_AUTO, "taps": _lib.METHOD_TAPS, "basis": _lib.METHOD_BASIS}
+
+
 def options(precision="fp32", noise="logistic", seed=1234, call_offset=0, room_index=0, level=1.0,
-            chunk_images=0, tile_samples=0, persistent=False, crossfade=0):
+            chunk_images=0, tile_samples=0, persistent=False, crossfade=0, method="auto"):
     o = _lib.Options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.noise = NOISES[noise] if isinstance(noise, str) else int(noise)
@@ -49,6 +52,7 @@ def options(precision="fp32", noise="logistic", seed=1234, call_offset=0, room_i
     o.tile_samples = int(tile_samples)
     o.persistent = int(bool(persistent))
     o.crossfade = int(crossfade)
+    o.method = METHODS[method] if isinstance(method, str) else int(method)
     return o
 
 
@@ -59,11 +63,13 @@ def n_samples(T, fs):
 def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, spkr_pattern="omni",
                  mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0, precision="fp32", noise="logistic",
                  seed=1234, call_offset=0, room_index=0, level=1.0, chunk_images=0, tile_samples=0, out=None,
-                 device=None, crossfade=0):
+                 device=None, crossfade=0, method="auto"):
     """simulateRIR on device tensors; returns float32 [n_src, n_rcv, nSamples] on the GPU.
 
     crossfade: samples of linear cross-fade from the early part into the diffuse model before
-    Tdiff (0, the default, is the reference's hard switch)."""
+    Tdiff (0, the default, is the reference's hard switch).  method (FP32 only): "basis" (the
+    default: Chebyshev expansion of the taps in the fractional delay, matrix-core filters) or
+    "taps" (every image scatters its windowed-sinc taps)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     ps = _dev_f32(pos_src, dev).reshape(-1, 3)
     pr = _dev_f32(pos_rcv, dev).reshape(-1, 3)
@@ -80,7 +86,7 @@ def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, 
         out = torch.empty(shape, dtype=torch.float32, device=dev)
     assert out.shape == shape and out.dtype == torch.float32 and out.is_contiguous()
     o = options(precision, noise, seed, call_offset, room_index, level, chunk_images, tile_samples,
-                crossfade=crossfade)
+                crossfade=crossfade, method=method)
     _lib.check(_lib.load().gpurir_simulate_rir_device(
         _lib.fptr(room), _lib.fptr(b), ps.data_ptr(), ps.shape[0], pr.data_ptr(), pr.shape[0], os_.data_ptr(),
         or_.data_ptr(), _pat(spkr_pattern), _pat(mic_pattern), _lib.iptr(nb), float(Tdiff), float(Tmax), float(fs),
@@ -160,7 +166,7 @@ class _BatchArgs:
 
 def simulate_batch(rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
                    noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
-                   device=None, crossfade=0):
+                   device=None, crossfade=0, method="auto"):
     """rooms: list of dicts with room_sz, beta, pos_src, pos_rcv, nb_img, Tmax [, Tdiff, orV_src, orV_rcv, level].
 
     Returns (flat float32 output on the GPU, list of per-room [n_src, n_rcv, nSamples] views)."""
@@ -168,7 +174,7 @@ def simulate_batch(rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, 
     B = _BatchArgs(rooms, fs, dev)
     out = torch.empty(B.total, dtype=torch.float32, device=dev)
     o = options(precision, noise, seed, call_offset, room_index, chunk_images=chunk_images,
-                tile_samples=tile_samples, crossfade=crossfade)
+                tile_samples=tile_samples, crossfade=crossfade, method=method)
     _lib.check(_lib.load().gpurir_simulate_batch_device(
         *B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),
         out.data_ptr(), B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _stream(dev)))
@@ -180,14 +186,14 @@ class BatchPlan:
 
     def __init__(self, rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, precision="fp32",
                  noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
-                 persistent=False, device=None, crossfade=0):
+                 persistent=False, device=None, crossfade=0, method="auto"):
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dev = dev
         self.B = _BatchArgs(rooms, fs, dev)
         self.out = torch.empty(self.B.total, dtype=torch.float32, device=dev)
         self.rirs = self.B.split(self.out)
         o = options(precision, noise, seed, call_offset, room_index, chunk_images=chunk_images,
-                    tile_samples=tile_samples, persistent=persistent, crossfade=crossfade)
+                    tile_samples=tile_samples, persistent=persistent, crossfade=crossfade, method=method)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().gpurir_batch_plan_create(
             *self.B.call_args(), _pat(spkr_pattern), _pat(mic_pattern), float(fs), float(c), ctypes.byref(o),

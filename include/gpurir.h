@@ -37,6 +37,12 @@ extern "C" {
 #define GPURIR_PREC_LUT 1  /* 1/16-sample table + linear interpolation (lookup_table) */
 #define GPURIR_PREC_FP16 2 /* packed FP16 windowed sinc, FP32 accumulation (mixed_precision) */
 
+/* Accumulation of the FP32 exact sinc (other precisions always scatter taps) */
+#define GPURIR_METHOD_AUTO 0  /* library default: GPURIR_METHOD_BASIS                    */
+#define GPURIR_METHOD_TAPS 1  /* every image scatters its Tw windowed-sinc taps           */
+#define GPURIR_METHOD_BASIS 2 /* 8-term Chebyshev expansion of the taps in the fractional */
+                              /* delay; matrix-core FIR (<= 5e-7 per tap of 1)             */
+
 /* Diffuse-tail noise */
 #define GPURIR_NOISE_LOGISTIC 0 /* unit-variance logistic, as src/gpuRIR_cuda.cu:454-455 */
 #define GPURIR_NOISE_GAUSSIAN 1 /* standard Gaussian (Box-Muller)                      */
@@ -53,6 +59,7 @@ typedef struct gpurir_options {
     int persistent;             /* batch plans: 1 = persistent accumulation launch       */
     int crossfade;              /* samples of linear cross-fade into the diffuse model   */
                                 /* before Tdiff (0 = the reference's hard switch)        */
+    int method;                 /* GPURIR_METHOD_* (FP32 precision only)                 */
 } gpurir_options_t;
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -18,7 +18,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (packed); non-packed issue caps at half of it
+FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix
 FLOP_PER_TAP = 12               # sub, x2pi, /Tw, cos, +1, x0.5, xpi, sin, /, xamp, xsinc, +acc
+BASIS_TILE = 512                # samples per tile of the basis kernel
+BASIS_MFMA_PER_TILE = 576       # v_mfma_f32_16x16x4_f32 per tile at Tw = 128: 8 rows x 36 K steps x 2 blocks
+MFMA_16X16X4_FLOP = 2048
 
 
 def log(msg):
@@ -177,8 +181,8 @@ def main():
     log(f"[bench] generating {args.rooms} uneven rooms")
     rooms = W.uneven_rooms(n_rooms=args.rooms, seed=0)
 
-    def measure_batch(precision):
-        plan = T.BatchPlan(rooms, fs, precision=precision, seed=1234, call_offset=0)
+    def measure_batch(precision, method="auto"):
+        plan = T.BatchPlan(rooms, fs, precision=precision, seed=1234, call_offset=0, method=method)
         n_rirs = plan.n_rirs
         taps = plan.count_taps()
         tune = plan.autotune() if not args.no_autotune else None
@@ -205,7 +209,8 @@ def main():
             o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
                                  Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
             errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
-        res = {"precision": precision, "autotune": tune, "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
+        res = {"precision": precision, "method": method if precision == "fp32" else "taps", "autotune": tune,
+               "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
                "rirs_per_s": n_rirs / (ms_step / 1e3), "accumulate_ms": ms_acc, "finalize_ms": ms_fin,
                "tflops_algorithmic": FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12, "rel_l2_vs_fp64": max(errs),
                "out_gb": plan.out.numel() * 4 / 1e9}
@@ -216,8 +221,8 @@ def main():
         torch.cuda.empty_cache()
         return res
 
-    modes = [args.precision] if args.only_batch else ["fp32", "fp16", "lut"]
-    batch = {m: measure_batch(m) for m in modes}
+    modes = [args.precision] if args.only_batch else ["fp32", "fp32_taps", "fp16", "lut"]
+    batch = {m: (measure_batch("fp32", "taps") if m == "fp32_taps" else measure_batch(m)) for m in modes}
     main_mode = args.precision if args.only_batch else "fp32"
     B = batch[main_mode]
     n_rirs, taps, ms_step, rirs_per_s = B["rirs"], B["taps"], B["ms_per_step"], B["rirs_per_s"]
@@ -289,7 +294,11 @@ def main():
                          f"windowed scatter, {nthreads} OpenMP threads, {dt:.1f} s"}
         log(f"[bench] CPU oracle: {cpu['value']:.3e} RIRs/s on {nthreads} threads")
 
-    traffic = read_traffic("traffic_ism_accumulate_uneven_rooms.txt")
+    traffic = read_traffic("traffic_ism_basis_uneven_rooms.txt")
+    # work the basis kernel executes: its matrix-core filters per 512-sample tile
+    tiles = sum(len(r["pos_src"]) * len(r["pos_rcv"]) * -(-T.n_samples(r["Tdiff"], fs) // BASIS_TILE)
+                for r in rooms)
+    mfma_tflops = tiles * BASIS_MFMA_PER_TILE * MFMA_16X16X4_FLOP / (ms_acc / 1e3) / 1e12
     line = {
         "metric": "RIRs/s batched vs plain at <=1e-3 rel-L2 vs FP64; fwd+bwd grads/s",
         "value": rirs_per_s,
@@ -308,8 +317,12 @@ def main():
                                "autotuned work items, hipGraph replay", "rirs": n_rirs, "rooms": args.rooms, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "ism_accumulate_kernel", "kernel_ms": ms_acc,
-                     "work": f"{taps} in-window taps x {FLOP_PER_TAP} flop"},
+                     "kernel": "ism_basis_persistent_kernel" if (B["autotune"] or {}).get("persistent", True)
+                     else "ism_basis_kernel", "kernel_ms": ms_acc,
+                     "work": f"{taps} in-window taps x {FLOP_PER_TAP} flop (the tap formula's work; the basis "
+                             "method executes 8 fixed-point sums per image and matrix-core filters instead)",
+                     "executed": {"tiles": tiles, "mfma_tflops": mfma_tflops,
+                                  "mfma_frac": mfma_tflops / FP32_MFMA_PEAK_TFLOPS}},
         "cpu_baseline": cpu,
         "accuracy": {"batch_rel_l2_max_3_rooms": max(errs), "plain_rel_l2": err_plain, "mid_room_rel_l2": err_mid},
         "plain_rirs_per_s": plain_rirs_s,

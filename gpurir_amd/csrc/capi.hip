@@ -334,8 +334,9 @@ std::string tune_key(const gpurir_plan* p) {
         nism_max = std::max(nism_max, n_samples_rule(h.Tdiff, p->so.fs));
     }
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s|p%d|fs%g|r%zu|m%lld|i%lld|t%d|d%d,%d", arch.c_str(), p->so.precision,
-                  (double)p->so.fs, p->rooms.size(), rirs, images, nism_max, p->so.spkr, p->so.mic);
+    std::snprintf(buf, sizeof(buf), "%s|p%d|b%d|fs%g|r%zu|m%lld|i%lld|t%d|d%d,%d", arch.c_str(), p->so.precision,
+                  p->sim && sim_is_basis(*p->sim) ? 1 : 0, (double)p->so.fs, p->rooms.size(), rirs, images, nism_max,
+                  p->so.spkr, p->so.mic);
     return buf;
 }
 }  // namespace
@@ -715,12 +716,14 @@ int gpurir_batch_plan_autotune(gpurir_plan_t p, void* stream, int* chunk_images,
             return 0;
         };
         // samples per tile (planner default first) and images per item with the persistent
-        // launch, then the one-workgroup-per-item launch of the best shape
+        // launch, then the one-workgroup-per-item launch of the best shape; basis plans
+        // have no image chunks, and their tile_samples is the samples per work item
         for (int tile : {0, 1024, 2048, 3072, 4096, 6144, 8192})
             if (int rc = consider(0, tile, 1)) return rc;
         const int tile = best.tile;
-        for (int chunk : {4096, 8192, 16384, 32768, 65536})
-            if (int rc = consider(chunk, tile, 1)) return rc;
+        if (!sim_is_basis(*p->sim))
+            for (int chunk : {4096, 8192, 16384, 32768, 65536})
+                if (int rc = consider(chunk, tile, 1)) return rc;
         if (int rc = consider(best.chunk, best.tile, 0)) return rc;
         GPURIR_CHECK(best.ms < 1e29f, "no plan candidate could be run");
         std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -119,6 +119,8 @@ void sim_set_stream_opts(SimLaunch& S, unsigned long long seed, unsigned call_of
 // left zero by every launch) switches the launch to the persistent kernel; NULL switches back.
 void sim_set_counters(SimLaunch& S, unsigned* d_counters);
 const SimOptions& sim_options_of(const SimLaunch& S);
+// true if the launch accumulates by the basis method (no image chunks)
+bool sim_is_basis(const SimLaunch& S);
 int sim_enqueue(const SimLaunch& S, const void* d_plan, float* d_part, const float* d_pos_src, const float* d_pos_rcv,
                 const float* d_orv_src, const float* d_orv_rcv, float* d_out, hipStream_t stream);
 int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* d_part, const float* d_pos_src,

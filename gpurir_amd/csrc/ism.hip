@@ -670,12 +670,13 @@ __global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a
 // followed by NB fixed Tw-tap filters, y[t] = sum_m sum_j C[m][j] D_m[t - j], which are
 // Toeplitz products: they run on the matrix cores (v_mfma_f32_16x16x4_f32).
 //
-// A work item is one RIR over a range of BT-sample tiles, one 512-thread workgroup.
-//   * The axis tables are sorted by |offset| once per item.  For a tile only images
-//     whose n0 can fall in [t_lo - J1, t_hi - 1 - J0] are visited: a row (x, y) is
-//     enumerated if x^2 + y^2 is inside the tile's outer radius, and its z entries in
-//     the distance shell are one contiguous range of the sorted z table (two binary
-//     searches).  Rows are taken 512 at a time (one per thread), their images written
+// A work item is one RIR over a range of BT-sample tiles, one WGB-thread workgroup.
+//   * For a tile only images whose n0 can fall in [t_lo - J1, t_hi - 1 - J0] are
+//     visited.  Along an axis the image offsets are 2 m L + c_q (q the parity), so the
+//     entries inside a distance bound are contiguous runs of m found in closed form:
+//     the rows (x, y) with x^2 + y^2 inside the tile's outer radius are one run of y
+//     per parity for each x, and a row's z entries in the shell at most two runs per
+//     parity.  Rows are taken WGB at a time (one per thread), their images written
 //     to an LDS queue by a block-wide prefix sum, and the queue is evaluated one image
 //     per lane with the same FP64 operations as image_eval (identical n0 and f).
 //   * Each live image adds amp scale T_m(u), m < NB, into D_m (64-bit fixed point,
@@ -684,37 +685,17 @@ __global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a
 //     16-byte operand reads of the 16 columns of a product fall in distinct banks)
 //     and the tile is y = sum_m A_m X_m with A_m[r][k] = C[m][J1 - k + r] (Toeplitz)
 //     and X_m[k][b] = D_m[16 b + k] (Hankel): 16 samples by 16 blocks per product,
-//     two products per 512-sample tile, NB/4 basis rows per wave, and the four
+//     BT / 256 products per tile, NB/4 basis rows per wave, and the four
 //     partial sums of a sample added in a fixed order.
 constexpr int BT = 512;     // samples per tile
 constexpr int WGB = 512;    // threads per workgroup
 constexpr int NWB = WGB / 64;
 constexpr int QB = 2048;    // queue entries per round
+constexpr int NOT = BT / 256;  // 256-sample output blocks of a tile (16 x 16 products)
+static_assert(NWB == 4 * NOT, "four basis quarters per output block");
 constexpr int BASIS_NB = 8; // Chebyshev terms
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-// number of a[0..n) <= key (a ascending)
-__device__ inline int count_le(const double* a, int n, double key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] <= key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// number of a[0..n) < key (a ascending)
-__device__ inline int count_lt(const double* a, int n, double key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
 
 // Exclusive prefix sum of one int per thread over the WGB threads; *total gets the sum.
 // Every thread must call it (two barriers); tmp holds NWB ints.
@@ -740,6 +721,44 @@ __device__ inline int block_scan_excl(int v, int* tmp, int* total) {
     return before + x - v;
 }
 
+// Image entries of one axis in closed form: entry n has k = n - N/2 and offset
+// 2 m L + c_q - (rounding) with m = ceil(k / 2), q = k & 1, c_0 = s - r, c_1 = -s - r
+// (axis_entry), increasing in m for each parity.  The entries with
+// lo <= |offset| <= hi (lo <= 0: no lower bound) are at most two runs of m per parity.
+struct AxisRuns {
+    double c[2];
+    double inv2L;
+    int mlo[2], mhi[2];  // valid m per parity
+    int half;            // N / 2
+};
+
+__device__ inline int floor_div2(int k) { return k >> 1; }  // arithmetic shift: floor(k / 2)
+
+__device__ inline AxisRuns axis_runs(int N, double L, float sd, float rd) {
+    AxisRuns A;
+    A.half = N / 2;
+    const int kmin = -A.half, kmax = N - 1 - A.half;
+    A.c[0] = (double)sd - (double)rd;
+    A.c[1] = -(double)sd - (double)rd;
+    A.inv2L = 1.0 / (2.0 * L);
+    A.mlo[0] = -floor_div2(-kmin);  // k = 2 m
+    A.mhi[0] = floor_div2(kmax);
+    A.mlo[1] = -floor_div2(-(kmin + 1));  // k = 2 m - 1
+    A.mhi[1] = floor_div2(kmax + 1);
+    return A;
+}
+
+// Entry range (first n, count; n steps by 2) of parity q with offset in [u, v] (u <= v),
+// widened by 1e-7 of 2L on each side.
+__device__ inline void axis_range(const AxisRuns& A, int q, double u, double v, int& n_first, int& count, int cap_hi) {
+    int m0 = (int)ceil((u - A.c[q]) * A.inv2L - 1e-7);
+    int m1 = (int)floor((v - A.c[q]) * A.inv2L + 1e-7);
+    m0 = max(m0, A.mlo[q]);
+    m1 = min(min(m1, A.mhi[q]), cap_hi);
+    count = max(m1 - m0 + 1, 0);
+    n_first = 2 * m0 - q + A.half;
+}
+
 template <int NB>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert(NB % 4 == 0, "NB/4 basis rows per wave");
@@ -750,11 +769,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     const int ntab = a.ntab_max, DL = a.bdl, CL = a.bcl, KS = a.bks;
     const int DLP = DL + DL / 4;
 
-    double* offs = reinterpret_cast<double*>(smem_raw);  // [ntab] axis offsets (original order)
-    double* sq = offs + ntab;                            // [ntab] squared offsets, sorted per axis
-    float* att = reinterpret_cast<float*>(sq + ntab);    // [ntab]
-    int* perm = reinterpret_cast<int*>(att + ntab);      // [ntab] sorted rank -> index in the axis
-    int* rowStart = perm + ntab;                         // [ntab + 4] rows before sorted x entry ix
+    double* offs = reinterpret_cast<double*>(smem_raw);  // [ntab] axis offsets
+    float* att = reinterpret_cast<float*>(offs + ntab);  // [ntab]
+    int* xrun = reinterpret_cast<int*>(att + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
+    int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4] rows before x entry n
     long long* D = reinterpret_cast<long long*>(rowStart + ntab + 4);  // [NB][DL] fixed point
     float* ctab = reinterpret_cast<float*>(D + NB * DL);              // [NB][CL]
     double* rb2 = reinterpret_cast<double*>(ctab + NB * CL);          // [WGB] row x^2 + y^2
@@ -778,22 +796,12 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     __syncthreads();
 
     const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz, ntot = Nx + Ny + Nz;
-    // sort every axis by |offset| (rank by counting, ties by index) and take the
-    // per-axis maxima of |att| for the fixed-point scale
     for (int e = tid; e < ntot; e += WGB) {
         const int ax = e < Nx ? 0 : (e < Nx + Ny ? 1 : 2);
-        const int lo = ax == 0 ? 0 : (ax == 1 ? Nx : Nx + Ny);
-        const int hi = ax == 0 ? Nx : (ax == 1 ? Nx + Ny : ntot);
-        const double v = fabs(offs[e]);
-        int rank = 0;
-        for (int k = lo; k < hi; k++) {
-            const double w = fabs(offs[k]);
-            rank += (w < v || (w == v && k < e)) ? 1 : 0;
-        }
-        sq[lo + rank] = offs[e] * offs[e];
-        perm[lo + rank] = e - lo;
         atomicMax(misc + 9 + ax, __float_as_int(fabsf(att[e])));
     }
+    const AxisRuns AY = axis_runs(Ny, R.L1, ps[1], pr[1]);
+    const AxisRuns AZ = axis_runs(Nz, R.L2, ps[2], pr[2]);
     __syncthreads();
     float scale, inv_scale;
     {
@@ -807,15 +815,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         inv_scale = ldexpf(1.0f, -E);
     }
     const double fsc = R.fs_over_c;
-    const double* sqX = sq;
-    const double* sqY = sq + Nx;
-    const double* sqZ = sq + Nx + Ny;
-    const int* pX = perm;
-    const int* pY = perm + Nx;
-    const int* pZ = perm + Nx + Ny;
     const int J0 = a.J0, J1 = a.J0 + a.nJ - 1;
     const bool pattern = a.spkr != 0 || a.mic != 0;
     float* orow = a.out + R.out_off + (long long)it.rir * R.nSamples;
+    constexpr int BIG = 1 << 30;
 
     for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
         const int th = min(tl + BT, it.t_hi);
@@ -824,18 +827,28 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         // n0 test below decides)
         const double rlo = (double)(lo_n0 - 1) / fsc, rhi = (double)(hi_n0 + 2) / fsc;
         const double d2lo = lo_n0 >= 1 ? rlo * rlo : -1.0, d2hi = rhi * rhi;
-        // rows: sorted y entries per sorted x entry inside the outer radius
+        // rows: per x entry, the y entries with x^2 + y^2 <= d2hi (one run per parity)
         int carry = 0;
         for (int ix0 = 0; ix0 < Nx; ix0 += WGB) {
-            const int ix = ix0 + tid;
+            const int nx = ix0 + tid;
             int c = 0;
-            if (ix < Nx) {
-                const double rem = d2hi - sqX[ix];
-                c = rem >= 0.0 ? count_le(sqY, Ny, rem) : 0;
+            if (nx < Nx) {
+                const double X = offs[nx];
+                const double rem = d2hi - X * X;
+                int f0 = 0, c0 = 0, f1 = 0, c1 = 0;
+                if (rem >= 0.0) {
+                    const double Y = sqrt(rem);
+                    axis_range(AY, 0, -Y, Y, f0, c0, BIG);
+                    axis_range(AY, 1, -Y, Y, f1, c1, BIG);
+                }
+                xrun[nx] = f0;
+                xrun[(ntab + 4) + nx] = c0;
+                xrun[2 * (ntab + 4) + nx] = f1;
+                c = c0 + c1;
             }
             int tot;
             const int ex = block_scan_excl(c, misc, &tot);
-            if (ix < Nx) rowStart[ix + 1] = carry + ex + c;
+            if (nx < Nx) rowStart[nx + 1] = carry + ex + c;
             carry += tot;
         }
         if (tid == 0) rowStart[0] = 0;
@@ -843,7 +856,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         const int nrows = carry;
         for (int r0 = 0; r0 < nrows; r0 += WGB) {
             const int row = r0 + tid;
-            int cnt = 0, zlo = 0;
+            int cnt = 0;
+            int zf[4] = {0, 0, 0, 0}, zc[4] = {0, 0, 0, 0};
             if (row < nrows) {
                 int lo = 0, hi = Nx;  // rowStart[lo] <= row < rowStart[hi]
                 while (hi - lo > 1) {
@@ -851,13 +865,30 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     if (rowStart[mid] <= row) lo = mid;
                     else hi = mid;
                 }
-                const int nx = pX[lo], ny = pY[row - rowStart[lo]];
+                const int nx = lo, j = row - rowStart[lo];
+                const int c0 = xrun[(ntab + 4) + nx];
+                const int ny = j < c0 ? xrun[nx] + 2 * j : xrun[2 * (ntab + 4) + nx] + 2 * (j - c0);
                 const double vx = offs[nx], vy = offs[Nx + ny];
                 double b2 = vx * vx;
                 b2 = b2 + vy * vy;
-                const int zhi = count_le(sqZ, Nz, d2hi - b2);
-                zlo = count_lt(sqZ, Nz, d2lo - b2);
-                cnt = max(zhi - zlo, 0);
+                // z entries with d2lo - b2 <= z^2 <= d2hi - b2
+                const double V2 = d2hi - b2, U2 = d2lo - b2;
+                if (V2 >= 0.0) {
+                    const double V = sqrt(V2);
+                    if (U2 <= 0.0) {
+                        axis_range(AZ, 0, -V, V, zf[0], zc[0], BIG);
+                        axis_range(AZ, 1, -V, V, zf[2], zc[2], BIG);
+                    } else {
+                        const double U = sqrt(U2);
+                        // positive run first, then the negative run capped below it
+                        // (disjoint even where the widened bounds meet near 0)
+                        axis_range(AZ, 0, U, V, zf[1], zc[1], BIG);
+                        axis_range(AZ, 0, -V, -U, zf[0], zc[0], (zf[1] - AZ.half) / 2 - 1);
+                        axis_range(AZ, 1, U, V, zf[3], zc[3], BIG);
+                        axis_range(AZ, 1, -V, -U, zf[2], zc[2], (zf[3] - AZ.half + 1) / 2 - 1);
+                    }
+                }
+                cnt = zc[0] + zc[1] + zc[2] + zc[3];
                 rb2[tid] = b2;
                 ratt[tid] = att[nx] * att[Nx + ny];
                 rxy[tid] = nx | (ny << 16);
@@ -865,8 +896,13 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             int tot;
             const int ex = block_scan_excl(cnt, misc, &tot);
             for (int q0 = 0; q0 < tot; q0 += QB) {
-                const int e0 = max(ex, q0), e1 = min(ex + cnt, q0 + QB);
-                for (int e = e0; e < e1; e++) queue[e - q0] = (tid << 12) | pZ[zlo + (e - ex)];
+                int e = ex;
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const int e0 = max(e, q0), e1 = min(e + zc[g], q0 + QB);
+                    for (int p = e0; p < e1; p++) queue[p - q0] = (tid << 12) | (zf[g] + 2 * (p - e));
+                    e += zc[g];
+                }
                 __syncthreads();
                 const int nq = min(QB, tot - q0);
                 for (int k = tid; k < nq; k += WGB) {
@@ -934,11 +970,11 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             __syncthreads();
         }
 
-        // Toeplitz products: wave w takes samples [256 (w & 1), +256) and basis rows
-        // (w >> 1) NB/4 .. +NB/4.  Lane (c, g) = (lane & 15, lane >> 4) supplies
+        // Toeplitz products: wave w takes samples [256 (w % NOT), +256) and basis rows
+        // (w / NOT) NB/4 .. +NB/4.  Lane (c, g) = (lane & 15, lane >> 4) supplies
         // A[c][k] and X[k][c] for k = g KS + i, i < KS (one k per lane group and step)
         {
-            const int ot = wave & 1, mq = wave >> 1;
+            const int ot = wave % NOT, mq = wave / NOT;
             const int c = lane & 15, g = lane >> 4;
             floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -1836,6 +1872,7 @@ struct Plan {
 constexpr int MIN_CHUNK = 512;
 constexpr int MAX_CHUNK = 32768;      // images per work item by default at most (splits the largest rooms)
 constexpr int TARGET_ITEMS = 2048;
+constexpr long long AUTO_BASIS_MIN_TILES = 256;  // METHOD_AUTO: fewer basis tiles -> tap scatter
 constexpr size_t LDS_PER_WG = 53 * 1024;  // three workgroups per CU (160 KiB of LDS)
 constexpr int MIN_TILE = 512;
 
@@ -1951,7 +1988,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 struct BasisTables {
     int ks = 0;   // K steps per lane group: 4 ks >= 16 + nJ - 1, ks a multiple of 4
     int cl = 0;   // row length of ctab: 4 ks + 16
-    int dl = 0;   // basis row of a tile: >= BT + nJ - 1 and >= 496 + 4 ks, a multiple of 16
+    int dl = 0;   // basis row of a tile: >= BT + nJ - 1 and >= BT - 16 + 4 ks, a multiple of 16
     std::vector<float> ctab;
 };
 
@@ -1966,7 +2003,7 @@ BasisTables make_basis_tables(const TapTables& T, int Tw) {
     BasisTables B;
     B.ks = 4 * ((T.nJ + 15 + 15) / 16);
     B.cl = 4 * B.ks + 16;
-    B.dl = (int)align_up((size_t)std::max(BT + T.nJ - 1, 496 + 4 * B.ks), 16);
+    B.dl = (int)align_up((size_t)std::max(BT + T.nJ - 1, BT - 16 + 4 * B.ks), 16);
     B.ctab.assign((size_t)NB * B.cl, 0.0f);
     const int J1 = T.J0 + T.nJ - 1;
     double fn[NB], val[NB];
@@ -1985,18 +2022,24 @@ BasisTables make_basis_tables(const TapTables& T, int Tw) {
 }
 
 size_t basis_lds_bytes(int ntab, const BasisTables& B) {
-    return 8 * (size_t)ntab * 2 + 4 * (size_t)ntab * 2 + 4 * ((size_t)ntab + 4) + 8 * (size_t)BASIS_NB * B.dl +
+    return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + 8 * (size_t)BASIS_NB * B.dl +
            4 * (size_t)BASIS_NB * B.cl + (8 + 4 + 4) * (size_t)WGB + 4 * (size_t)QB + 4 * 16;
 }
 
-// Basis plan: one item per RIR and range of BT-sample tiles (the whole early part
-// by default, or tile_samples rounded up to BT), longest expected work first: the
+// Basis plan: one item per RIR and range of BT-sample tiles (tile_samples rounded up
+// to BT; by default as many tiles as keep about TARGET_ITEMS items, so that a small
+// launch still spreads its tiles over the device), longest expected work first: the
 // images in a shell grow with the cube of its outer radius and with 1 / volume.
 Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt) {
     Plan P;
     int ntab = 0;
-    for (const RoomHost& h : rh) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
+    long long tiles_all = 0;
+    for (const RoomHost& h : rh) {
+        ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
+        tiles_all += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + BT - 1) / BT);
+    }
     P.ntab_max = ntab;
+    const long long per_default = (long long)BT * std::max<long long>(1, tiles_all / TARGET_ITEMS);
     std::vector<std::pair<double, Item>> work;
     long long rir_off = 0;
     for (size_t k = 0; k < rh.size(); k++) {
@@ -2024,7 +2067,8 @@ Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, con
         R.n_chunks = 1;
         const int N = h.nb[0] * h.nb[1] * h.nb[2];
         const int M = h.n_src * h.n_rcv;
-        int per = opt.tile_samples > 0 ? (int)align_up((size_t)opt.tile_samples, BT) : (int)align_up((size_t)R.nISM, BT);
+        int per = opt.tile_samples > 0 ? (int)align_up((size_t)opt.tile_samples, BT)
+                                       : (int)std::min<long long>(per_default, 1 << 30);
         per = std::max(per, BT);
         const double vol = (double)h.L[0] * h.L[1] * h.L[2];
         for (int m = 0; m < M; m++) {
@@ -2095,8 +2139,26 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.T = make_tap_tables(S.prec, S.Tw);
     GPURIR_CHECK(opt.method >= METHOD_AUTO && opt.method <= METHOD_BASIS, "unknown accumulation method");
     S.basis = S.prec == PREC_FP32 && opt.method != METHOD_TAPS;
+    if (S.basis && opt.method == METHOD_AUTO) {
+        // a launch of fewer tiles than the device has CUs (e.g. one room with two
+        // receivers) is faster as a tap scatter, whose image chunks fill the device
+        long long tiles = 0;
+        for (const RoomHost& h : rooms)
+            tiles += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + BT - 1) / BT);
+        if (tiles < AUTO_BASIS_MIN_TILES) S.basis = false;
+    }
     if (S.basis) {
         S.Bt = make_basis_tables(S.T, S.Tw);
+        int ntab = 0;
+        for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt) > 160 * 1024) {
+            // the tile's basis rows do not fit (windows of more than ~1000 taps, beyond the
+            // 768 of 96 kHz, or very long axis tables): the tap scatter takes the launch
+            GPURIR_CHECK(opt.method == METHOD_AUTO, "basis method: LDS budget exceeded (window too long)");
+            S.basis = false;
+        }
+    }
+    if (S.basis) {
         S.P = make_plan_basis(rooms, opt, S.T);
     } else {
         S.Bt = BasisTables();
@@ -2346,6 +2408,7 @@ int sim_enqueue(const SimLaunch& S, const void* d_plan, float* d_part, const flo
 SimLaunch* sim_new() { return new SimLaunch(); }
 void sim_set_counters(SimLaunch& S, unsigned* d_counters) { S.counters = d_counters; }
 const SimOptions& sim_options_of(const SimLaunch& S) { return S.opt; }
+bool sim_is_basis(const SimLaunch& S) { return S.basis; }
 void sim_delete(SimLaunch* s) { delete s; }
 const void* sim_blob(const SimLaunch& S) { return S.blob.data(); }
 void sim_set_stream_opts(SimLaunch& S, unsigned long long seed, unsigned call_offset) {

@@ -104,19 +104,28 @@ def test_sampling_rates_odd_and_even_windows(fs):
 
 
 def test_basis_and_taps_methods():
-    """FP32: the Chebyshev-basis accumulation (default) and the tap scatter both meet the
-    1e-5 bar on a directional room with several sources; the basis error is dominated by
-    its interpolation (<= 5e-7 per tap of 1), so the two agree to a few 1e-6."""
-    cfg = W.mid_room(n_src=3, n_rcv=8, seed=5)
+    """FP32: the Chebyshev-basis accumulation and the tap scatter both meet the 1e-5 bar on a
+    directional room with several sources; the basis error is dominated by its
+    interpolation (<= 5e-7 per tap of 1), so the two agree to a few 1e-6.  The automatic
+    choice takes the basis method for this launch (48 RIRs x 7 tiles of 512 samples) and
+    the tap scatter for a launch of fewer than 256 tiles (one room, two receivers)."""
+    cfg = W.mid_room(n_src=3, n_rcv=16, seed=5)
     g_b, o = run_both(cfg, method="basis")
     g_t, _ = run_both(cfg, method="taps")
     g_a, _ = run_both(cfg)
     nISM = orc.n_samples(cfg["Tdiff"], cfg["fs"])
+    assert 3 * 16 * -(-nISM // 512) >= 256
     e_b = check_parity(g_b, o, nISM)
     e_t = check_parity(g_t, o, nISM)
     assert np.array_equal(g_a, g_b)
     assert e_b < 5e-6 and e_t < 1e-6
     assert rel_l2(g_b[..., :nISM], g_t[..., :nISM]) < 5e-6
+    small = W.plain_room()
+    g_s, o_s = run_both(small)
+    g_st, _ = run_both(small, method="taps")
+    g_sb, _ = run_both(small, method="basis")
+    assert np.array_equal(g_s, g_st)
+    check_parity(g_sb, o_s, orc.n_samples(small["Tdiff"], small["fs"]))
 
 
 @pytest.mark.parametrize("pattern", ["homni", "card", "hypcard", "subcard", "bidir"])

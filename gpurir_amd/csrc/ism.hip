@@ -694,6 +694,7 @@ constexpr int QB = 2048;    // queue entries per round
 constexpr int NOT = BT / 256;  // 256-sample output blocks of a tile (16 x 16 products)
 static_assert(NWB == 4 * NOT, "four basis quarters per output block");
 constexpr int BASIS_NB = 8; // Chebyshev terms
+constexpr int BCS = 2;      // columns per thread of the in-place conversion: DL <= BCS * WGB
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -947,26 +948,26 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         }
         __syncthreads();
 
-        // fixed point -> FP32 in output units, in place: element e = m DL + d goes to
-        // m DLP + d + 4 (d >> 4) < 5 e / 4 + 4 floats, below the 8-byte words that later
-        // rounds read (all reads of a round precede its writes)
-        constexpr int CH = 8;
-        for (int base = 0; base < NB * DL; base += WGB * CH) {
-            long long v[CH];
+        // fixed point -> FP32 in output units, in place: thread tid converts columns
+        // d = tid + s WGB (s < BCS) of every row into registers, and after the barrier
+        // writes them to m DLP + d + 4 (d >> 4)
+        {
+            float fv[NB][BCS];
 #pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const int e = base + c * WGB + tid;
-                v[c] = e < NB * DL ? D[e] : 0;
-            }
+            for (int m = 0; m < NB; m++)
+#pragma unroll
+                for (int sl = 0; sl < BCS; sl++) {
+                    const int d = tid + sl * WGB;
+                    fv[m][sl] = d < DL ? (float)((double)D[m * DL + d] * (double)inv_scale) : 0.0f;
+                }
             __syncthreads();
 #pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const int e = base + c * WGB + tid;
-                if (e < NB * DL) {
-                    const int m = e / DL, d = e - m * DL;
-                    Df[m * DLP + d + 4 * (d >> 4)] = (float)((double)v[c] * (double)inv_scale);
+            for (int m = 0; m < NB; m++)
+#pragma unroll
+                for (int sl = 0; sl < BCS; sl++) {
+                    const int d = tid + sl * WGB;
+                    if (d < DL) Df[m * DLP + d + 4 * (d >> 4)] = fv[m][sl];
                 }
-            }
             __syncthreads();
         }
 
@@ -983,14 +984,25 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 const float* Ap = ctab + m * CL + g * KS + 16 - c;
                 const float* Bp = Df + m * DLP;
                 const int kb = 256 * ot + 16 * c + g * KS;
+                // operands of step t4 + 1 are read before the products of step t4; the
+                // read after the last step (4 floats past this lane's K range) stays
+                // inside the LDS of D and the rows and is not used
+                float4 xv = *reinterpret_cast<const float4*>(Bp + kb + 4 * (kb >> 4));
+                float a0 = Ap[0], a1 = Ap[1], a2 = Ap[2], a3 = Ap[3];
                 for (int t4 = 0; t4 < KS / 4; t4++) {
-                    const int dl = kb + 4 * t4;
-                    const float4 xv = *reinterpret_cast<const float4*>(Bp + dl + 4 * (dl >> 4));
-                    const float* ap = Ap + 4 * t4;
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[0], xv.x, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[1], xv.y, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[2], xv.z, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[3], xv.w, acc, 0, 0, 0);
+                    const int dn = kb + 4 * (t4 + 1);
+                    const float4 xn = *reinterpret_cast<const float4*>(Bp + dn + 4 * (dn >> 4));
+                    const float* an = Ap + 4 * (t4 + 1);
+                    const float n0 = an[0], n1 = an[1], n2 = an[2], n3 = an[3];
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, xv.x, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, xv.y, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, xv.z, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, xv.w, acc, 0, 0, 0);
+                    xv = xn;
+                    a0 = n0;
+                    a1 = n1;
+                    a2 = n2;
+                    a3 = n3;
                 }
             }
             // result element (row 4 g + i, column c) is sample 16 c + 4 g + i of the half
@@ -2151,9 +2163,10 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         S.Bt = make_basis_tables(S.T, S.Tw);
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
-        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt) > 160 * 1024) {
-            // the tile's basis rows do not fit (windows of more than ~1000 taps, beyond the
-            // 768 of 96 kHz, or very long axis tables): the tap scatter takes the launch
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
+            // the tile's basis rows do not fit the conversion or the LDS (windows of more
+            // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
+            // scatter takes the launch
             GPURIR_CHECK(opt.method == METHOD_AUTO, "basis method: LDS budget exceeded (window too long)");
             S.basis = false;
         }

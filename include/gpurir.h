@@ -40,7 +40,8 @@ extern "C" {
 /* Accumulation of the FP32 exact sinc (other precisions always scatter taps) */
 #define GPURIR_METHOD_AUTO 0  /* library default: BASIS, or TAPS for launches of fewer    */
                               /* than 256 tiles of 512 early samples (too few to fill    */
-                              /* the device) and for windows too long for the basis tile */
+                              /* the device) and for windows of more than 513 taps     */
+                              /* (fs above 64 kHz)                                       */
 #define GPURIR_METHOD_TAPS 1  /* every image scatters its Tw windowed-sinc taps           */
 #define GPURIR_METHOD_BASIS 2 /* 8-term Chebyshev expansion of the taps in the fractional */
                               /* delay; matrix-core FIR (<= 5e-7 per tap of 1)             */

@@ -698,16 +698,25 @@ constexpr int BCS = 2;      // columns per thread of the in-place conversion: DL
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves (VALU only):
+// row_shr 1, 2, 4, 8 within each row of 16 lanes (lanes shifted in from outside the
+// row add 0), then row_bcast:15 adds the last lane of rows 0 and 2 to rows 1 and 3,
+// and row_bcast:31 the last lane of row 1 to rows 2 and 3.
+__device__ inline int wave_scan_incl(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 // Exclusive prefix sum of one int per thread over the WGB threads; *total gets the sum.
 // Every thread must call it (two barriers); tmp holds NWB ints.
 __device__ inline int block_scan_excl(int v, int* tmp, int* total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    const int x = wave_scan_incl(v);
     if (lane == 63) tmp[w] = x;
     __syncthreads();
     int before = 0, all = 0;

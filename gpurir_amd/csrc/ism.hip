@@ -2157,7 +2157,16 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.Tw = tw_rule(opt.fs);
     GPURIR_CHECK(S.Tw >= 2, "sampling frequency too low for the 8 ms window");
     S.prec = opt.precision;
-    S.T = make_tap_tables(S.prec, S.Tw);
+    {
+        // per (precision, Tw): the LUT mode's table takes ~4e3 transcendental calls
+        static std::mutex mu;
+        static std::map<std::pair<int, int>, TapTables> cache;
+        std::lock_guard<std::mutex> lk(mu);
+        const auto key = std::make_pair(S.prec, S.Tw);
+        auto f = cache.find(key);
+        if (f == cache.end()) f = cache.emplace(key, make_tap_tables(S.prec, S.Tw)).first;
+        S.T = f->second;
+    }
     GPURIR_CHECK(opt.method >= METHOD_AUTO && opt.method <= METHOD_BASIS, "unknown accumulation method");
     S.basis = S.prec == PREC_FP32 && opt.method != METHOD_TAPS;
     if (S.basis && opt.method == METHOD_AUTO) {
@@ -2169,7 +2178,15 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         if (tiles < AUTO_BASIS_MIN_TILES) S.basis = false;
     }
     if (S.basis) {
-        S.Bt = make_basis_tables(S.T, S.Tw);
+        // the tables depend on Tw only (about 10^4 transcendental calls): built once
+        static std::mutex mu;
+        static std::map<int, BasisTables> cache;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto f = cache.find(S.Tw);
+            if (f == cache.end()) f = cache.emplace(S.Tw, make_basis_tables(S.T, S.Tw)).first;
+            S.Bt = f->second;
+        }
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
@@ -2454,7 +2471,9 @@ int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src
     }
     unsigned* counters = nullptr;  // persistent launch: items taken longest first from a counter
     if (int rc = device_counters(&counters)) return rc;
-    sim_set_counters(S, counters);
+    // basis launches take one workgroup per item (their persistent form holds more
+    // registers and fits fewer workgroups per CU; plans choose it only when timed faster)
+    sim_set_counters(S, S.basis ? nullptr : counters);
     return sim_enqueue(S, dplan, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
 }
 
@@ -2484,7 +2503,9 @@ int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_
     }
     unsigned* counters = nullptr;
     if (int rc = device_counters(&counters)) return rc;
-    sim_set_counters(S, counters);
+    // basis launches take one workgroup per item (their persistent form holds more
+    // registers and fits fewer workgroups per CU; plans choose it only when timed faster)
+    sim_set_counters(S, S.basis ? nullptr : counters);
     const unsigned char* d = static_cast<const unsigned char*>(dplan);
     return sim_enqueue(S, dplan, dpart, reinterpret_cast<const float*>(d + o_ps), reinterpret_cast<const float*>(d + o_pr),
                        reinterpret_cast<const float*>(d + o_os), reinterpret_cast<const float*>(d + o_or), d_out, stream);

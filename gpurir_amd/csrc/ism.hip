@@ -769,14 +769,16 @@ __device__ inline void axis_range(const AxisRuns& A, int q, double u, double v, 
     n_first = 2 * m0 - q + A.half;
 }
 
-template <int NB>
+// DLC: the basis row length as a compile-time constant (the LDS offsets of an image's
+// eight atomics are then immediates), or 0 for the launch's a.bdl.
+template <int NB, int DLC>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert(NB % 4 == 0, "NB/4 basis rows per wave");
     const Item it = a.items[item];
     const RoomK R = load_room(a.rooms + it.room);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int ntab = a.ntab_max, DL = a.bdl, CL = a.bcl, KS = a.bks;
+    const int ntab = a.ntab_max, DL = DLC ? DLC : a.bdl, CL = a.bcl, KS = a.bks;
     const int DLP = DL + DL / 4;
 
     double* offs = reinterpret_cast<double*>(smem_raw);  // [ntab] axis offsets
@@ -1027,14 +1029,14 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     }
 }
 
-template <int NB>
+template <int NB, int DLC>
 __global__ __launch_bounds__(WGB) void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    basis_item<NB>(a, blockIdx.x, smem_raw);
+    basis_item<NB, DLC>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
-template <int NB>
+template <int NB, int DLC>
 __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -1044,7 +1046,7 @@ __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, in
         const int item = next;
         __syncthreads();
         if (item >= n_items) break;
-        basis_item<NB>(a, item, smem_raw);
+        basis_item<NB, DLC>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -2326,8 +2328,12 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     const RoomDev* d_rooms = reinterpret_cast<const RoomDev*>(dp + S.off_rooms);
     if ((stage & 1) && !S.P.items.empty() && S.basis) {
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
-        const void* fn_once = reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB>);
-        const void* fn_pers = reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB>);
+        // the 16 kHz window (128 taps) has its row length compiled in
+        const bool dl640 = S.Bt.dl == 640;
+        const void* fn_once = dl640 ? reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 640>)
+                                    : reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 0>);
+        const void* fn_pers = dl640 ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 640>)
+                                    : reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 0>);
         int n_items = (int)S.P.items.size();
         int resident = 0;
         if (S.counters) {

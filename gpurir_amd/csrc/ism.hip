@@ -1342,6 +1342,9 @@ struct AdjArgs {
     int J0, nJ;
     float halfTw, inv_tw2, pi_tw;  // Tw/2, 2/Tw, pi/Tw
     int ntab_max, gp_len;
+    // basis adjoint: coefficients [ADJ_NB][nJ] (offset k = j - J0), row length of E, length of g'
+    const float* actab;
+    int adl, agl;
 };
 
 constexpr int AQ = 2 * WG;  // adjoint image queue (ring of two rounds)
@@ -1423,6 +1426,61 @@ __device__ __forceinline__ void adj_tap_near(float4 kc, float g, float f, float 
     const float wps = sg * fmaf(kc.y, ps, -kc.z * pc);             // sigma_k w'
     G0 = fmaf(g * ws, u, G0);
     G1 = fmaf(g, fmaf(wps, u, -ws * v), G1);
+}
+
+// Chain rule of one image (indices nx, ny, nz) from G0 = sum g' h and G1 = sum g' h'
+// through amp(beta, v), the directivities and tau(v), in FP64, added to acc:
+// dbeta[6], dsrc[3], drcv[3].
+__device__ __forceinline__ void adj_chain(const RoomK& R, int nx, int ny, int nz, const double* offs, const float* F,
+                                          const float* Flo, const float* Fhi, double orx, double ory, double orz,
+                                          double osx, double osy, double osz, int spkr, int mic, float G0, float G1,
+                                          double acc[12]) {
+    const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+    const int ey = Nx + ny, ez = Nx + Ny + nz;
+    const double vx = offs[nx], vy = offs[ey], vz = offs[ez];
+    const double d = sqrt(vx * vx + vy * vy + vz * vz);
+    const double id = 1.0 / d;
+    const double ux = vx * id, uy = vy * id, uz = vz * id;
+    const double Fx = F[nx], Fy = F[ey], Fz = F[ez];
+    const double att = Fx * Fy * Fz;
+    const double a0 = id / 12.566370614359172;
+    const double sx = ((nx - Nx / 2) & 1) ? -1.0 : 1.0;
+    const double sy = ((ny - Ny / 2) & 1) ? -1.0 : 1.0;
+    const double sz = ((nz - Nz / 2) & 1) ? -1.0 : 1.0;
+    double Dr = 1.0, Drp = 0.0, Ds = 1.0, Dsp = 0.0, cr = 0.0, cs = 0.0;
+    if (mic != 0) {
+        cr = ux * orx + uy * ory + uz * orz;
+        pattern_gain_d(cr, mic, &Dr, &Drp);
+    }
+    const double qx = sx * osx, qy = sy * osy, qz = sz * osz;  // mirrored source orientation
+    if (spkr != 0) {
+        cs = -(ux * qx + uy * qy + uz * qz);
+        pattern_gain_d(cs, spkr, &Ds, &Dsp);
+    }
+    const double geo = Ds * Dr * a0;
+    const double amp = att * geo;
+    const double g0 = (double)G0, g1 = (double)G1;
+    const double gbeta = g0 * geo;
+    acc[0] += gbeta * (double)Flo[nx] * Fy * Fz;
+    acc[1] += gbeta * (double)Fhi[nx] * Fy * Fz;
+    acc[2] += gbeta * Fx * (double)Flo[ey] * Fz;
+    acc[3] += gbeta * Fx * (double)Fhi[ey] * Fz;
+    acc[4] += gbeta * Fx * Fy * (double)Flo[ez];
+    acc[5] += gbeta * Fx * Fy * (double)Fhi[ez];
+    // d amp / d v = att a0 (-Ds Dr u/d + Ds' Dr dcs/dv + Ds Dr' dcr/dv),
+    // dcr/dv = (o_r - cr u)/d, dcs/dv = (-o_s' - cs u)/d;  d tau / d v = (fs/c) u
+    const double kr = att * a0 * Ds * Drp * id, ks = att * a0 * Dsp * Dr * id;
+    const double kd = -att * a0 * Ds * Dr * id;
+    const double ku = g0 * (kd - kr * cr - ks * cs) + (-amp * g1) * R.fs_over_c;
+    const double gvx = ku * ux + g0 * (kr * orx - ks * qx);
+    const double gvy = ku * uy + g0 * (kr * ory - ks * qy);
+    const double gvz = ku * uz + g0 * (kr * orz - ks * qz);
+    acc[6] += sx * gvx;
+    acc[7] += sy * gvy;
+    acc[8] += sz * gvz;
+    acc[9] -= gvx;
+    acc[10] -= gvy;
+    acc[11] -= gvz;
 }
 
 struct AdjImg {
@@ -1532,51 +1590,7 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
         const int rem = i - nx * (Ny * Nz);
         const int ny = (int)(((double)rem + 0.5) * R.inv_nz);
         const int nz = rem - ny * Nz;
-        const int ey = Nx + ny, ez = Nx + Ny + nz;
-        const double vx = offs[nx], vy = offs[ey], vz = offs[ez];
-        const double d = sqrt(vx * vx + vy * vy + vz * vz);
-        const double id = 1.0 / d;
-        const double ux = vx * id, uy = vy * id, uz = vz * id;
-        const double Fx = F[nx], Fy = F[ey], Fz = F[ez];
-        const double att = Fx * Fy * Fz;
-        const double a0 = id / 12.566370614359172;
-        const double sx = ((nx - Nx / 2) & 1) ? -1.0 : 1.0;
-        const double sy = ((ny - Ny / 2) & 1) ? -1.0 : 1.0;
-        const double sz = ((nz - Nz / 2) & 1) ? -1.0 : 1.0;
-        double Dr = 1.0, Drp = 0.0, Ds = 1.0, Dsp = 0.0, cr = 0.0, cs = 0.0;
-        if (a.mic != 0) {
-            cr = ux * orx + uy * ory + uz * orz;
-            pattern_gain_d(cr, a.mic, &Dr, &Drp);
-        }
-        const double qx = sx * osx, qy = sy * osy, qz = sz * osz;  // mirrored source orientation
-        if (a.spkr != 0) {
-            cs = -(ux * qx + uy * qy + uz * qz);
-            pattern_gain_d(cs, a.spkr, &Ds, &Dsp);
-        }
-        const double geo = Ds * Dr * a0;
-        const double amp = att * geo;
-        const double g0 = (double)G0, g1 = (double)G1;
-        const double gbeta = g0 * geo;
-        acc[0] += gbeta * (double)Flo[nx] * Fy * Fz;
-        acc[1] += gbeta * (double)Fhi[nx] * Fy * Fz;
-        acc[2] += gbeta * Fx * (double)Flo[ey] * Fz;
-        acc[3] += gbeta * Fx * (double)Fhi[ey] * Fz;
-        acc[4] += gbeta * Fx * Fy * (double)Flo[ez];
-        acc[5] += gbeta * Fx * Fy * (double)Fhi[ez];
-        // d amp / d v = att a0 (-Ds Dr u/d + Ds' Dr dcs/dv + Ds Dr' dcr/dv),
-        // dcr/dv = (o_r - cr u)/d, dcs/dv = (-o_s' - cs u)/d;  d tau / d v = (fs/c) u
-        const double kr = att * a0 * Ds * Drp * id, ks = att * a0 * Dsp * Dr * id;
-        const double kd = -att * a0 * Ds * Dr * id;
-        const double ku = g0 * (kd - kr * cr - ks * cs) + (-amp * g1) * R.fs_over_c;
-        const double gvx = ku * ux + g0 * (kr * orx - ks * qx);
-        const double gvy = ku * uy + g0 * (kr * ory - ks * qy);
-        const double gvz = ku * uz + g0 * (kr * orz - ks * qz);
-        acc[6] += sx * gvx;
-        acc[7] += sy * gvy;
-        acc[8] += sz * gvz;
-        acc[9] -= gvx;
-        acc[10] -= gvy;
-        acc[11] -= gvz;
+        adj_chain(R, nx, ny, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, a.spkr, a.mic, G0, G1, acc);
     };
 
     // image rounds of WG: phase A lists the images whose taps reach the tile into a
@@ -1643,6 +1657,250 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
     if (tid < 12) {
         double t = 0.0;
         for (int w = 0; w < NWAVE; w++) t += red[w][tid];
+        a.item_part[(long long)bid * 12 + tid] = t;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// adjoint, basis method
+// ---------------------------------------------------------------------------
+// With h_j(f) ~ sum_m C[m][j] T_m(u), u = 2 f - 1 (ADJ_NB = 10 terms: the f-derivative of
+// the 8-term expansion of the forward kernel is only within 6e-5 of h', of 10 terms within
+// 5e-7), an image at tau = n0 + f has
+//   G0 = sum_t g'(t) h(t - tau)  = sum_m T_m(u) E_m[n0],   E_m[n] = sum_j C[m][j] g'(n + j),
+//   G1 = sum_t g'(t) h'(t - tau) = -2 sum_m m U_{m-1}(u) E_m[n0]
+// (h'(x) = -d h_j / df at x = j - f, du/df = 2, T_m' = m U_{m-1}).  Per 512-sample tile the
+// workgroup builds g' on the tile (zero outside it, so that every tile adds only its own
+// taps), correlates it with the coefficient rows (E, FP32 FMAs with the coefficients as
+// scalar operands), enumerates the images of the tile's delay range as the forward basis
+// kernel does, and sends each live image's G0, G1 through adj_chain into per-lane FP64 sums,
+// reduced per item in a fixed order.
+constexpr int ADJ_NB = 10;
+
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void ism_adjoint_basis_kernel(AdjArgs a) {
+    constexpr int NB = ADJ_NB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NWB][12];
+    const int bid = a.order[blockIdx.x];
+    const Item it = a.items[bid];
+    const RoomK R = load_room(a.rooms + it.room);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ntab = a.ntab_max, DL = a.adl, GL = a.agl, nJ = a.nJ, J0 = a.J0, J1 = a.J0 + a.nJ - 1;
+
+    double* offs = reinterpret_cast<double*>(smem_raw);  // [ntab]
+    float* F = reinterpret_cast<float*>(offs + ntab);    // [ntab] attenuation and its beta derivatives
+    float* Flo = F + ntab;
+    float* Fhi = Flo + ntab;
+    int* xrun = reinterpret_cast<int*>(Fhi + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
+    int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4]
+    float* gp = reinterpret_cast<float*>(rowStart + ntab + 4);  // [GL] g'(lo_n0 + J0 + u)
+    float* E = gp + GL;                                  // [NB][DL] E_m[lo_n0 + d]
+    double* rb2 = reinterpret_cast<double*>(E + NB * DL);  // [WGB] row x^2 + y^2
+    int* rxy = reinterpret_cast<int*>(rb2 + WGB);        // [WGB] row nx | ny << 16
+    int* queue = rxy + WGB;                              // [QB] slot << 12 | nz
+    int* misc = queue + QB;                              // [16] scan scratch
+    float* cf = reinterpret_cast<float*>(misc + 16);     // [nJ][NB] coefficients of offset J0 + k
+
+    for (int e = tid; e < NB * nJ; e += WGB) {
+        const int m = e / nJ, k = e - m * nJ;
+        cf[k * NB + m] = a.actab[e];
+    }
+    const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
+    const int g_rir = R.rir_off + it.rir;
+    const float* ps = a.pos_src + 3 * (R.src_off + s);
+    const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
+    for (int n = tid; n < R.Nx; n += WGB)
+        axis_entry_adj(n, R.Nx, R.L0, ps[0], pr[0], R.b0, R.b1, offs, F, Flo, Fhi);
+    for (int n = tid; n < R.Ny; n += WGB)
+        axis_entry_adj(n, R.Ny, R.L1, ps[1], pr[1], R.b2, R.b3, offs + R.Nx, F + R.Nx, Flo + R.Nx, Fhi + R.Nx);
+    for (int n = tid; n < R.Nz; n += WGB) {
+        const int o = R.Nx + R.Ny;
+        axis_entry_adj(n, R.Nz, R.L2, ps[2], pr[2], R.b4, R.b5, offs + o, F + o, Flo + o, Fhi + o);
+    }
+    double orx = 0, ory = 0, orz = 0, osx = 0, osy = 0, osz = 0;
+    if (a.mic != 0) {
+        const float* o = a.orv_rcv + 3 * (R.rcv_off + r);
+        const double n = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+        orx = o[0] / n;
+        ory = o[1] / n;
+        orz = o[2] / n;
+    }
+    if (a.spkr != 0) {
+        const float* o = a.orv_src + 3 * (R.src_off + s);
+        const double n = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+        osx = o[0] / n;
+        osy = o[1] / n;
+        osz = o[2] / n;
+    }
+    const AxisRuns AY = axis_runs(R.Ny, R.L1, ps[1], pr[1]);
+    const AxisRuns AZ = axis_runs(R.Nz, R.L2, ps[2], pr[2]);
+    const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+    const double fsc = R.fs_over_c;
+    const long long row_off = R.out_off + (long long)it.rir * R.nSamples;
+    const float cw = a.rir_cw[g_rir];
+    const int ws = a.rir_ws[g_rir];
+    constexpr int BIG = 1 << 30;
+    double acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = 0.0;
+    __syncthreads();
+
+    for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
+        const int th = min(tl + BT, it.t_hi);
+        const int lo_n0 = tl - J1, hi_n0 = th - 1 - J0;
+        // g' on the tile, zero outside it: dL/dout plus the envelope-window term
+        for (int u = tid; u < GL; u += WGB) {
+            const int t = lo_n0 + J0 + u;
+            float g = 0.0f;
+            if (t >= tl && t < th) {
+                g = a.gy[row_off + t];
+                if (t >= ws) g = fmaf(cw, a.y[row_off + t], g);
+            }
+            gp[u] = g;
+        }
+        __syncthreads();
+        // E_m[lo_n0 + d] = sum_k C[m][k] g'(lo_n0 + d + J0 + k)
+        for (int d = tid; d < DL; d += WGB) {
+            float e[NB];
+#pragma unroll
+            for (int m = 0; m < NB; m++) e[m] = 0.0f;
+            for (int k = 0; k < nJ; k++) {
+                const float g = gp[d + k];
+                const float* ck = cf + k * NB;  // the same address in every lane: broadcast reads
+#pragma unroll
+                for (int m = 0; m < NB; m++) e[m] = fmaf(ck[m], g, e[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < NB; m++) E[m * DL + d] = e[m];
+        }
+        // rows: per x entry, the y entries with x^2 + y^2 <= d2hi (one run per parity)
+        const double rlo = (double)(lo_n0 - 1) / fsc, rhi = (double)(hi_n0 + 2) / fsc;
+        const double d2lo = lo_n0 >= 1 ? rlo * rlo : -1.0, d2hi = rhi * rhi;
+        int carry = 0;
+        for (int ix0 = 0; ix0 < Nx; ix0 += WGB) {
+            const int nx = ix0 + tid;
+            int c = 0;
+            if (nx < Nx) {
+                const double X = offs[nx];
+                const double rem = d2hi - X * X;
+                int f0 = 0, c0 = 0, f1 = 0, c1 = 0;
+                if (rem >= 0.0) {
+                    const double Y = sqrt(rem);
+                    axis_range(AY, 0, -Y, Y, f0, c0, BIG);
+                    axis_range(AY, 1, -Y, Y, f1, c1, BIG);
+                }
+                xrun[nx] = f0;
+                xrun[(ntab + 4) + nx] = c0;
+                xrun[2 * (ntab + 4) + nx] = f1;
+                c = c0 + c1;
+            }
+            int tot;
+            const int ex = block_scan_excl(c, misc, &tot);
+            if (nx < Nx) rowStart[nx + 1] = carry + ex + c;
+            carry += tot;
+        }
+        if (tid == 0) rowStart[0] = 0;
+        __syncthreads();
+        const int nrows = carry;
+        for (int r0 = 0; r0 < nrows; r0 += WGB) {
+            const int row = r0 + tid;
+            int cnt = 0;
+            int zf[4] = {0, 0, 0, 0}, zc[4] = {0, 0, 0, 0};
+            if (row < nrows) {
+                int lo = 0, hi = Nx;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rowStart[mid] <= row) lo = mid;
+                    else hi = mid;
+                }
+                const int nx = lo, j = row - rowStart[lo];
+                const int c0 = xrun[(ntab + 4) + nx];
+                const int ny = j < c0 ? xrun[nx] + 2 * j : xrun[2 * (ntab + 4) + nx] + 2 * (j - c0);
+                const double vx = offs[nx], vy = offs[Nx + ny];
+                double b2 = vx * vx;
+                b2 = b2 + vy * vy;
+                const double V2 = d2hi - b2, U2 = d2lo - b2;
+                if (V2 >= 0.0) {
+                    const double V = sqrt(V2);
+                    if (U2 <= 0.0) {
+                        axis_range(AZ, 0, -V, V, zf[0], zc[0], BIG);
+                        axis_range(AZ, 1, -V, V, zf[2], zc[2], BIG);
+                    } else {
+                        const double U = sqrt(U2);
+                        axis_range(AZ, 0, U, V, zf[1], zc[1], BIG);
+                        axis_range(AZ, 0, -V, -U, zf[0], zc[0], (zf[1] - AZ.half) / 2 - 1);
+                        axis_range(AZ, 1, U, V, zf[3], zc[3], BIG);
+                        axis_range(AZ, 1, -V, -U, zf[2], zc[2], (zf[3] - AZ.half + 1) / 2 - 1);
+                    }
+                }
+                cnt = zc[0] + zc[1] + zc[2] + zc[3];
+                rb2[tid] = b2;
+                rxy[tid] = nx | (ny << 16);
+            }
+            int tot;
+            const int ex = block_scan_excl(cnt, misc, &tot);
+            for (int q0 = 0; q0 < tot; q0 += QB) {
+                int e = ex;
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const int e0 = max(e, q0), e1 = min(e + zc[g], q0 + QB);
+                    for (int p = e0; p < e1; p++) queue[p - q0] = (tid << 12) | (zf[g] + 2 * (p - e));
+                    e += zc[g];
+                }
+                __syncthreads();
+                const int nq = min(QB, tot - q0);
+                for (int k = tid; k < nq; k += WGB) {
+                    const int qe = queue[k];
+                    const int slot = qe >> 12, nz = qe & 4095;
+                    const double vz = offs[Nx + Ny + nz];
+                    double d2 = rb2[slot];
+                    d2 = d2 + vz * vz;
+                    const double tau = sqrt_normal(d2) * fsc;
+                    const double fl = floor(tau);
+                    const int n0 = (int)fl;
+                    if (n0 < lo_n0 || n0 > hi_n0) continue;
+                    const float u = fmaf(2.0f, (float)(tau - fl), -1.0f);
+                    const float u2 = u + u;
+                    const float* ep = E + (n0 - lo_n0);
+                    // T_m (first kind) and m U_{m-1} (derivative) by their recurrences
+                    float t0 = 1.0f, t1 = u, w0 = 1.0f, w1 = u2;  // T_0, T_1, U_0, U_1
+                    float G0 = fmaf(t1, ep[DL], ep[0]);
+                    float D1 = ep[DL];  // 1 * U_0 * E_1
+#pragma unroll
+                    for (int m = 2; m < NB; m++) {
+                        const float t2 = fmaf(u2, t1, -t0);
+                        const float em = ep[m * DL];
+                        G0 = fmaf(t2, em, G0);
+                        D1 = fmaf((float)m * w1, em, D1);
+                        const float w2 = fmaf(u2, w1, -w0);
+                        t0 = t1;
+                        t1 = t2;
+                        w0 = w1;
+                        w1 = w2;
+                    }
+                    const int xy = rxy[slot];
+                    adj_chain(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, a.spkr,
+                              a.mic, G0, -2.0f * D1, acc);
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+
+    // fixed-order reduction: lanes, then waves
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (tid < 12) {
+        double t = 0.0;
+        for (int w = 0; w < NWB; w++) t += red[w][tid];
         a.item_part[(long long)bid * 12 + tid] = t;
     }
 }
@@ -2044,6 +2302,24 @@ BasisTables make_basis_tables(const TapTables& T, int Tw) {
     return B;
 }
 
+// Chebyshev interpolation coefficients of every tap offset for the basis adjoint:
+// C[m][k] for k = j - J0, NB terms (FP64 on the host).
+std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB) {
+    std::vector<float> c((size_t)NB * T.nJ, 0.0f);
+    std::vector<double> fn(NB), val(NB);
+    for (int i = 0; i < NB; i++) fn[i] = 0.5 + 0.5 * std::cos(M_PI * (i + 0.5) / NB);
+    for (int k = 0; k < T.nJ; k++) {
+        const int j = T.J0 + k;
+        for (int i = 0; i < NB; i++) val[i] = windowed_sinc((double)j - fn[i], Tw);
+        for (int m = 0; m < NB; m++) {
+            double v = 0.0;
+            for (int i = 0; i < NB; i++) v += val[i] * std::cos(m * M_PI * (i + 0.5) / NB);
+            c[(size_t)m * T.nJ + k] = (float)(v * (m == 0 ? 1.0 : 2.0) / NB);
+        }
+    }
+    return c;
+}
+
 size_t basis_lds_bytes(int ntab, const BasisTables& B) {
     return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + 8 * (size_t)BASIS_NB * B.dl +
            4 * (size_t)BASIS_NB * B.cl + (8 + 4 + 4) * (size_t)WGB + 4 * (size_t)QB + 4 * 16;
@@ -2053,7 +2329,8 @@ size_t basis_lds_bytes(int ntab, const BasisTables& B) {
 // to BT; by default as many tiles as keep about TARGET_ITEMS items, so that a small
 // launch still spreads its tiles over the device), longest expected work first: the
 // images in a shell grow with the cube of its outer radius and with 1 / volume.
-Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt) {
+Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt,
+                     bool sort_items = true) {
     Plan P;
     int ntab = 0;
     long long tiles_all = 0;
@@ -2114,8 +2391,10 @@ Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, con
         rir_off += M;
         P.rooms.push_back(R);
     }
-    std::stable_sort(work.begin(), work.end(),
-                     [](const std::pair<double, Item>& x, const std::pair<double, Item>& y) { return x.first > y.first; });
+    if (sort_items)
+        std::stable_sort(work.begin(), work.end(), [](const std::pair<double, Item>& x, const std::pair<double, Item>& y) {
+            return x.first > y.first;
+        });
     P.items.reserve(work.size());
     for (const auto& w : work) P.items.push_back(w.second);
     P.tt_max = BT;
@@ -2557,7 +2836,10 @@ struct AdjLaunch {
     std::vector<float4> ktab;
     std::vector<int> rir_item_begin;
     std::vector<int> order;
-    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, off_ord = 0, plan_bytes = 0;
+    bool basis = false;      // basis adjoint (ism_adjoint_basis_kernel)
+    std::vector<float> actab;
+    int adl = 0, agl = 0;
+    size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, off_ord = 0, off_act = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
     size_t lds = 0;
     int ntab = 0, gp_len = 0, Tw = 0;
@@ -2592,23 +2874,52 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         nism_max = std::max(nism_max, n_samples_rule(h.Tdiff, opt_in.fs));
     }
     SimOptions opt = opt_in;
-    if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
-    if (opt.chunk_images <= 0) {
-        // about 8192 items (a few per resident workgroup) so that the launch ends evenly
-        double work = 0.0;
-        for (const RoomHost& h : rooms) {
-            const int nISM = n_samples_rule(h.Tdiff, opt.fs);
-            const int tiles = nISM > 0 ? (nISM + opt.tile_samples - 1) / opt.tile_samples : 0;
-            work += (double)h.n_src * h.n_rcv * h.nb[0] * h.nb[1] * (double)h.nb[2] * tiles;
-        }
-        opt.chunk_images = (int)std::min(65536.0, std::max(1024.0, std::ceil(work / 8192.0)));
-    }
-    A.opt = opt;
     A.Tw = tw_rule(opt.fs);
     GPURIR_CHECK(A.Tw >= 2, "sampling frequency too low for the 8 ms window");
     A.T = make_tap_tables(PREC_FP32, A.Tw);
-    A.mask = (A.Tw % 2) != 0;
-    A.P = make_plan(rooms, opt, A.T, false);
+    // basis adjoint unless asked for taps, or (automatic) fewer tiles than CUs, or its LDS
+    // does not fit
+    A.basis = opt.method != METHOD_TAPS;
+    if (A.basis) {
+        long long tiles = 0;
+        int ntab = 0;
+        for (const RoomHost& h : rooms) {
+            tiles += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + BT - 1) / BT);
+            ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
+        }
+        A.adl = (int)align_up((size_t)(BT + A.T.nJ - 1), 16);
+        A.agl = (int)align_up((size_t)(A.adl + A.T.nJ), 4);
+        const size_t nt = align_up((size_t)ntab, 4);
+        A.lds = 20 * nt + 16 * (nt + 4) + 4 * (size_t)A.agl + 4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB +
+                4 * (size_t)QB + 64 + 4 * (size_t)ADJ_NB * A.T.nJ;
+        if ((opt.method == METHOD_AUTO && tiles < AUTO_BASIS_MIN_TILES) || A.lds > 150 * 1024) {
+            GPURIR_CHECK(opt.method == METHOD_AUTO, "basis adjoint: LDS budget exceeded");
+            A.basis = false;
+        }
+    }
+    if (A.basis) {
+        SimOptions ob = opt;
+        ob.tile_samples = 0;  // the planner's default tiles per item
+        A.opt = opt;
+        A.mask = (A.Tw % 2) != 0;
+        A.P = make_plan_basis(rooms, ob, A.T, false);
+        A.actab = make_adjoint_coefficients(A.T, A.Tw, ADJ_NB);
+    } else {
+        if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
+        if (opt.chunk_images <= 0) {
+            // about 8192 items (a few per resident workgroup) so that the launch ends evenly
+            double work = 0.0;
+            for (const RoomHost& h : rooms) {
+                const int nISM = n_samples_rule(h.Tdiff, opt.fs);
+                const int tiles = nISM > 0 ? (nISM + opt.tile_samples - 1) / opt.tile_samples : 0;
+                work += (double)h.n_src * h.n_rcv * h.nb[0] * h.nb[1] * (double)h.nb[2] * tiles;
+            }
+            opt.chunk_images = (int)std::min(65536.0, std::max(1024.0, std::ceil(work / 8192.0)));
+        }
+        A.opt = opt;
+        A.mask = (A.Tw % 2) != 0;
+        A.P = make_plan(rooms, opt, A.T, false);
+    }
     const Plan& P = A.P;
     const int J1 = A.T.J0 + A.T.nJ - 1;
     GPURIR_CHECK(A.T.J0 <= 0 && J1 >= 1, "window too short for the adjoint");
@@ -2627,19 +2938,32 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         for (const Item& it : P.items) cnt[P.rooms[it.room].rir_off + it.rir]++;
         for (int g = 0; g < n_rirs; g++) A.rir_item_begin[g + 1] = A.rir_item_begin[g] + cnt[g];
     }
-    // launch order: longest items first (the partial sums keep the item's own index)
+    // launch order: longest items first (the partial sums keep the item's own index); basis
+    // items by their shell volume over the room volume
     A.order.resize(P.items.size());
     for (size_t e = 0; e < P.items.size(); e++) A.order[e] = (int)e;
-    std::stable_sort(A.order.begin(), A.order.end(), [&](int x, int y) {
-        return (P.items[x].i1 - P.items[x].i0) > (P.items[y].i1 - P.items[y].i0);
-    });
+    if (A.basis) {
+        std::vector<double> wk(P.items.size());
+        for (size_t e = 0; e < P.items.size(); e++) {
+            const Item& it = P.items[e];
+            const RoomDev& R = P.rooms[it.room];
+            const double r1 = (double)(it.t_hi + A.T.nJ), r0 = (double)std::max(it.t_lo - A.T.nJ, 0);
+            wk[e] = (r1 * r1 * r1 - r0 * r0 * r0) / (R.L[0] * R.L[1] * R.L[2]);
+        }
+        std::stable_sort(A.order.begin(), A.order.end(), [&](int x, int y) { return wk[x] > wk[y]; });
+    } else {
+        std::stable_sort(A.order.begin(), A.order.end(), [&](int x, int y) {
+            return (P.items[x].i1 - P.items[x].i0) > (P.items[y].i1 - P.items[y].i0);
+        });
+    }
     A.off_rooms = 0;
     A.off_items = align_up(sizeof(RoomDev) * P.rooms.size(), 256);
     A.off_rr = align_up(A.off_items + sizeof(Item) * P.items.size(), 256);
     A.off_kt = align_up(A.off_rr + sizeof(int) * P.rir_room.size(), 256);
     A.off_rib = align_up(A.off_kt + sizeof(float4) * A.ktab.size(), 256);
     A.off_ord = align_up(A.off_rib + sizeof(int) * A.rir_item_begin.size(), 256);
-    A.plan_bytes = align_up(A.off_ord + sizeof(int) * (A.order.size() + 1), 256);
+    A.off_act = align_up(A.off_ord + sizeof(int) * (A.order.size() + 1), 256);
+    A.plan_bytes = align_up(A.off_act + sizeof(float) * (A.actab.size() + 1), 256);
     A.blob.assign(A.plan_bytes, 0);
     std::memcpy(A.blob.data() + A.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(A.blob.data() + A.off_items, P.items.data(), sizeof(Item) * P.items.size());
@@ -2647,9 +2971,11 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
     std::memcpy(A.blob.data() + A.off_kt, A.ktab.data(), sizeof(float4) * A.ktab.size());
     std::memcpy(A.blob.data() + A.off_rib, A.rir_item_begin.data(), sizeof(int) * A.rir_item_begin.size());
     if (!A.order.empty()) std::memcpy(A.blob.data() + A.off_ord, A.order.data(), sizeof(int) * A.order.size());
+    if (!A.actab.empty()) std::memcpy(A.blob.data() + A.off_act, A.actab.data(), sizeof(float) * A.actab.size());
     A.ntab = (P.ntab_max + 3) / 4 * 4;
     A.gp_len = (int)align_up((size_t)P.tt_max + 2 * (size_t)(A.T.nJ - 1) + 4, 4);
-    A.lds = (sizeof(double) + 3 * sizeof(float)) * A.ntab + sizeof(float) * A.gp_len + (size_t)AQ * 12;
+    if (!A.basis)
+        A.lds = (sizeof(double) + 3 * sizeof(float)) * A.ntab + sizeof(float) * A.gp_len + (size_t)AQ * 12;
     GPURIR_CHECK(A.lds <= 150 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
 }
@@ -2715,11 +3041,16 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.pi_tw = (float)(M_PI / (double)A.Tw);
         a.ntab_max = A.ntab;
         a.gp_len = A.gp_len;
-        const void* fn = A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
-                                : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);
+        a.actab = reinterpret_cast<const float*>(dp + A.off_act);
+        a.adl = A.adl;
+        a.agl = A.agl;
+        const void* fn = A.basis  ? reinterpret_cast<const void*>(&ism_adjoint_basis_kernel)
+                         : A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
+                                  : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);
         if (int rc = allow_lds(fn, A.lds)) return rc;
         void* args[] = {&a};
-        GPURIR_HIP(hipLaunchKernel(fn, dim3((unsigned)P.items.size()), dim3(WG), args, A.lds, stream));
+        GPURIR_HIP(hipLaunchKernel(fn, dim3((unsigned)P.items.size()), dim3(A.basis ? WGB : WG), args, A.lds,
+                                   stream));
     }
     hipLaunchKernelGGL(grad_rir_kernel, dim3((unsigned)((n_rirs + WG - 1) / WG)), dim3(WG), 0, stream, item_part,
                        reinterpret_cast<const int*>(dp + A.off_rib), rir_tail, rir_grad, n_rirs);

@@ -295,14 +295,15 @@ class _BatchFn(torch.autograd.Function):
 
 def simulate_batch_diff(rooms, beta, pos_src, pos_rcv, level, fs, spkr_pattern="omni", mic_pattern="omni",
                         c=343.0, noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0,
-                        tile_samples=0, device=None):
+                        tile_samples=0, device=None, method="auto"):
     """Differentiable batch simulation.
 
     rooms: list of dicts with room_sz, nb_img, Tmax [, Tdiff, orV_src, orV_rcv] and the number of
     sources / receivers given by pos_src / pos_rcv rows (keys "n_src", "n_rcv" or the shapes of
     "pos_src" / "pos_rcv").  beta [R,6], pos_src [sum n_src,3], pos_rcv [sum n_rcv,3], level [R] are
     tensors (they may require grad).  The forward pass is the exact FP32 simulation; the backward
-    pass is the hand-written adjoint.  Returns (flat output, per-room views)."""
+    pass is the hand-written adjoint.  method ("auto", "taps" or "basis") selects the
+    accumulation of both passes.  Returns (flat output, per-room views)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     R = len(rooms)
     desc = []
@@ -319,7 +320,7 @@ def simulate_batch_diff(rooms, beta, pos_src, pos_rcv, level, fs, spkr_pattern="
     assert tuple(pos_src.shape) == (int(B.n_src.sum()), 3) and tuple(pos_rcv.shape) == (int(B.n_rcv.sum()), 3)
     cfg = {"dev": dev, "B": B, "spkr": _pat(spkr_pattern), "mic": _pat(mic_pattern), "fs": float(fs),
            "c": float(c), "opt": options("fp32", noise, seed, call_offset, room_index, chunk_images=chunk_images,
-                                         tile_samples=tile_samples)}
+                                         tile_samples=tile_samples, method=method)}
     out = _BatchFn.apply(beta, pos_src, pos_rcv, level, cfg)
     return out, B.split(out)
 
@@ -327,7 +328,7 @@ def simulate_batch_diff(rooms, beta, pos_src, pos_rcv, level, fs, spkr_pattern="
 def simulate_rir_diff(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, level=None,
                       spkr_pattern="omni", mic_pattern="omni", orV_src=None, orV_rcv=None, c=343.0,
                       noise="logistic", seed=1234, call_offset=0, room_index=0, chunk_images=0, tile_samples=0,
-                      device=None):
+                      device=None, method="auto"):
     """Differentiable simulateRIR of one room: beta [6], pos_src [S,3], pos_rcv [R,3] and level
     (scalar tensor or None = 1) may require grad.  Returns [S, R, nSamples] float32 on the GPU."""
     if level is None:
@@ -336,7 +337,7 @@ def simulate_rir_diff(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=N
                 n_src=pos_src.shape[0], n_rcv=pos_rcv.shape[0])
     out, views = simulate_batch_diff([room], beta.reshape(1, 6), pos_src, pos_rcv, level.reshape(1), fs,
                                      spkr_pattern, mic_pattern, c, noise, seed, call_offset, room_index,
-                                     chunk_images, tile_samples, device)
+                                     chunk_images, tile_samples, device, method)
     return views[0]
 
 

@@ -55,11 +55,14 @@ def rounded_case(seed=0):
     return f32(L), f32(beta), f32(src), f32(rcv), g
 
 
+@pytest.mark.parametrize("method", ["taps", "basis"])
 @pytest.mark.parametrize("patterns", [dict(), DIRECTIONAL], ids=["omni", "card-hypcard"])
-def test_adjoint_matches_fd_and_autograd(patterns):
+def test_adjoint_matches_fd_and_autograd(patterns, method):
+    """Both accumulation methods: the tap scatter and the 10-term basis adjoint (the forward of
+    the basis method is within 1e-6 of the exact sinc, its adjoint's h' within 5e-7 per tap)."""
     L, beta, src, rcv, g = rounded_case()
     kw = {k: (f32(v) if k.startswith("orV") else v) for k, v in patterns.items()}
-    got = gpu_grads(L, beta, src, rcv, g, **kw)
+    got = gpu_grads(L, beta, src, rcv, g, method=method, **kw)
     assert_close(got, fd_grads(L, beta, src, rcv, g, **kw), what="vs finite differences")
     assert_close(got, autograd_grads(L, beta, src, rcv, g, **kw), what="vs autograd")
 
@@ -91,14 +94,15 @@ def test_adjoint_without_tail():
         assert rel(got[k], want[k]) <= TOL
 
 
-def test_adjoint_odd_window_44k1():
-    """fs = 44.1 kHz: Tw = 353 (odd), the masked adjoint kernel."""
+@pytest.mark.parametrize("method", ["taps", "basis"])
+def test_adjoint_odd_window_44k1(method):
+    """fs = 44.1 kHz: Tw = 353 (odd): the masked tap adjoint, and the basis adjoint."""
     L, beta, src, rcv, _ = rounded_case(4)
     fs = 44100.0
     nS = orc.n_samples(0.02, fs)
     g = np.random.default_rng(5).standard_normal((1, 2, nS))
     kw = dict(nb=[3, 3, 3], Tmax=0.02, Tdiff=0.012)
-    got = gpu_grads(L, beta, src, rcv, g, fs=fs, **kw)
+    got = gpu_grads(L, beta, src, rcv, g, fs=fs, method=method, **kw)
     b = torch.tensor(beta, requires_grad=True)
     s = torch.tensor(src, requires_grad=True)
     r = torch.tensor(rcv, requires_grad=True)
@@ -109,10 +113,11 @@ def test_adjoint_odd_window_44k1():
     assert_close(got, want, what="44.1 kHz vs autograd")
 
 
-def test_adjoint_is_deterministic():
+@pytest.mark.parametrize("method", ["taps", "basis"])
+def test_adjoint_is_deterministic(method):
     L, beta, src, rcv, g = rounded_case(6)
-    a = gpu_grads(L, beta, src, rcv, g, chunk_images=64)
-    b = gpu_grads(L, beta, src, rcv, g, chunk_images=64)
+    a = gpu_grads(L, beta, src, rcv, g, chunk_images=64, method=method)
+    b = gpu_grads(L, beta, src, rcv, g, chunk_images=64, method=method)
     for k in GROUPS:
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
 
@@ -132,14 +137,15 @@ def batch_case(n=4, seed=7):
     return rooms, betas, ps, pr, lv
 
 
-def test_batch_adjoint_uneven_rooms():
+@pytest.mark.parametrize("method", ["taps", "basis"])
+def test_batch_adjoint_uneven_rooms(method):
     rooms, betas, ps, pr, lv = batch_case()
     dev = torch.device("cuda", 0)
     b = torch.tensor(np.stack(betas), dtype=torch.float32, device=dev, requires_grad=True)
     s = torch.tensor(np.concatenate(ps), dtype=torch.float32, device=dev, requires_grad=True)
     r = torch.tensor(np.concatenate(pr), dtype=torch.float32, device=dev, requires_grad=True)
     l = torch.tensor(lv, dtype=torch.float32, device=dev, requires_grad=True)
-    out, views = T.simulate_batch_diff(rooms, b, s, r, l, FS, room_index=3)
+    out, views = T.simulate_batch_diff(rooms, b, s, r, l, FS, room_index=3, method=method)
     rng = np.random.default_rng(8)
     gs = [rng.standard_normal(tuple(v.shape)) for v in views]
     loss = sum((v * torch.as_tensor(gk, dtype=torch.float32, device=dev)).sum() for v, gk in zip(views, gs))
@@ -163,11 +169,12 @@ def test_batch_adjoint_uneven_rooms():
         ro += nr
 
 
-def test_batch_plan_adjoint_matches_autograd_path():
+@pytest.mark.parametrize("method", ["taps", "basis"])
+def test_batch_plan_adjoint_matches_autograd_path(method):
     rooms, betas, ps, pr, lv = batch_case(seed=9)
     dev = torch.device("cuda", 0)
     plan_rooms = [dict(r, beta=betas[k], pos_src=ps[k], pos_rcv=pr[k], level=lv[k]) for k, r in enumerate(rooms)]
-    plan = T.BatchPlan(plan_rooms, FS)
+    plan = T.BatchPlan(plan_rooms, FS, method=method)
     y = plan.run()
     g = torch.randn(y.shape, generator=torch.Generator().manual_seed(1)).to(dev)
     gb, gs, gr, gl = plan.adjoint(g)
@@ -175,7 +182,7 @@ def test_batch_plan_adjoint_matches_autograd_path():
     s = torch.tensor(np.concatenate(ps), dtype=torch.float32, device=dev, requires_grad=True)
     r = torch.tensor(np.concatenate(pr), dtype=torch.float32, device=dev, requires_grad=True)
     l = torch.tensor(lv, dtype=torch.float32, device=dev, requires_grad=True)
-    out, _ = T.simulate_batch_diff(rooms, b, s, r, l, FS)
+    out, _ = T.simulate_batch_diff(rooms, b, s, r, l, FS, method=method)
     assert torch.equal(out, y)
     (out * g).sum().backward()
     torch.cuda.synchronize()

@@ -1342,9 +1342,11 @@ struct AdjArgs {
     int J0, nJ;
     float halfTw, inv_tw2, pi_tw;  // Tw/2, 2/Tw, pi/Tw
     int ntab_max, gp_len;
-    // basis adjoint: coefficients [ADJ_NB][nJ] (offset k = j - J0), row length of E, length of g'
+    // basis adjoint: coefficients [ADJ_NB][acl] (16 zeros, then offset k = j - J0 at 16 + k),
+    // row length of E, logical length of g' (stored with 4 floats of padding per 16), K steps
+    // per lane group of the correlation products
     const float* actab;
-    int adl, agl;
+    int adl, agl, aks, acl;
 };
 
 constexpr int AQ = 2 * WG;  // adjoint image queue (ring of two rounds)
@@ -1688,6 +1690,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ntab = a.ntab_max, DL = a.adl, GL = a.agl, nJ = a.nJ, J0 = a.J0, J1 = a.J0 + a.nJ - 1;
+    const int KS = a.aks, CL = a.acl;
 
     double* offs = reinterpret_cast<double*>(smem_raw);  // [ntab]
     float* F = reinterpret_cast<float*>(offs + ntab);    // [ntab] attenuation and its beta derivatives
@@ -1695,18 +1698,14 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     float* Fhi = Flo + ntab;
     int* xrun = reinterpret_cast<int*>(Fhi + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
     int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4]
-    float* gp = reinterpret_cast<float*>(rowStart + ntab + 4);  // [GL] g'(lo_n0 + J0 + u)
-    float* E = gp + GL;                                  // [NB][DL] E_m[lo_n0 + d]
+    float* gp = reinterpret_cast<float*>(rowStart + ntab + 4);  // [GL + GL/4] g'(lo_n0 + J0 + u) at u + 4 (u >> 4)
+    float* cpad = gp + GL + GL / 4;                      // [NB][CL] coefficients
+    float* E = cpad + NB * CL;                           // [NB][DL] E_m[lo_n0 + d]
     double* rb2 = reinterpret_cast<double*>(E + NB * DL);  // [WGB] row x^2 + y^2
     int* rxy = reinterpret_cast<int*>(rb2 + WGB);        // [WGB] row nx | ny << 16
     int* queue = rxy + WGB;                              // [QB] slot << 12 | nz
     int* misc = queue + QB;                              // [16] scan scratch
-    float* cf = reinterpret_cast<float*>(misc + 16);     // [nJ][NB] coefficients of offset J0 + k
-
-    for (int e = tid; e < NB * nJ; e += WGB) {
-        const int m = e / nJ, k = e - m * nJ;
-        cf[k * NB + m] = a.actab[e];
-    }
+    for (int e = tid; e < NB * CL; e += WGB) cpad[e] = a.actab[e];
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const int g_rir = R.rir_off + it.rir;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
@@ -1758,22 +1757,47 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 g = a.gy[row_off + t];
                 if (t >= ws) g = fmaf(cw, a.y[row_off + t], g);
             }
-            gp[u] = g;
+            gp[u + 4 * (u >> 4)] = g;
         }
         __syncthreads();
-        // E_m[lo_n0 + d] = sum_k C[m][k] g'(lo_n0 + d + J0 + k)
-        for (int d = tid; d < DL; d += WGB) {
-            float e[NB];
+        // E_m[lo_n0 + d] = sum_k C[m][k] g'(lo_n0 + d + J0 + k): E[16 b + r] = sum_x A[r][x] X[x][b]
+        // with A[r][x] = C[x - r] (Toeplitz) and X[x][b] = g'[16 b + x] (Hankel), the forward
+        // basis kernel's products with the coefficients unreversed; one task per (row m,
+        // 256 positions of E), the waves taking tasks in turn
+        {
+            const int c = lane & 15, g = lane >> 4;
+            const int nota = (DL + 255) / 256;
+            for (int task = wave; task < NB * nota; task += NWB) {
+                const int m = task / nota, ot = task - m * nota;
+                floatx4 acc4 = {0.0f, 0.0f, 0.0f, 0.0f};
+                const float* Ap = cpad + m * CL + g * KS + 16 - c;
+                const int kb = 256 * ot + 16 * c + g * KS;
+                // operands of step t4 + 1 are read before the products of step t4; the read
+                // after the last step stays inside g' (its logical length covers it) and the
+                // arrays after the coefficients, and is not used
+                float4 xv = *reinterpret_cast<const float4*>(gp + kb + 4 * (kb >> 4));
+                float a0 = Ap[0], a1 = Ap[1], a2 = Ap[2], a3 = Ap[3];
+                for (int t4 = 0; t4 < KS / 4; t4++) {
+                    const int dn = kb + 4 * (t4 + 1);
+                    const float4 xn = *reinterpret_cast<const float4*>(gp + dn + 4 * (dn >> 4));
+                    const float* an = Ap + 4 * (t4 + 1);
+                    const float n0 = an[0], n1 = an[1], n2 = an[2], n3 = an[3];
+                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, xv.x, acc4, 0, 0, 0);
+                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, xv.y, acc4, 0, 0, 0);
+                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, xv.z, acc4, 0, 0, 0);
+                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, xv.w, acc4, 0, 0, 0);
+                    xv = xn;
+                    a0 = n0;
+                    a1 = n1;
+                    a2 = n2;
+                    a3 = n3;
+                }
+                // element (row 4 g + i, column c) is position 256 ot + 16 c + 4 g + i
+                const int d0 = 256 * ot + 16 * c + 4 * g;
 #pragma unroll
-            for (int m = 0; m < NB; m++) e[m] = 0.0f;
-            for (int k = 0; k < nJ; k++) {
-                const float g = gp[d + k];
-                const float* ck = cf + k * NB;  // the same address in every lane: broadcast reads
-#pragma unroll
-                for (int m = 0; m < NB; m++) e[m] = fmaf(ck[m], g, e[m]);
+                for (int i = 0; i < 4; i++)
+                    if (d0 + i < DL) E[m * DL + d0 + i] = acc4[i];
             }
-#pragma unroll
-            for (int m = 0; m < NB; m++) E[m * DL + d] = e[m];
         }
         // rows: per x entry, the y entries with x^2 + y^2 <= d2hi (one run per parity)
         const double rlo = (double)(lo_n0 - 1) / fsc, rhi = (double)(hi_n0 + 2) / fsc;
@@ -2302,10 +2326,11 @@ BasisTables make_basis_tables(const TapTables& T, int Tw) {
     return B;
 }
 
-// Chebyshev interpolation coefficients of every tap offset for the basis adjoint:
-// C[m][k] for k = j - J0, NB terms (FP64 on the host).
-std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB) {
-    std::vector<float> c((size_t)NB * T.nJ, 0.0f);
+// Chebyshev interpolation coefficients of every tap offset for the basis adjoint: row m
+// holds 16 zeros, then C[m][k] for k = j - J0 at 16 + k, then zeros up to cl (FP64 on the
+// host), the Toeplitz operand of its correlation products.
+std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB, int cl) {
+    std::vector<float> c((size_t)NB * cl, 0.0f);
     std::vector<double> fn(NB), val(NB);
     for (int i = 0; i < NB; i++) fn[i] = 0.5 + 0.5 * std::cos(M_PI * (i + 0.5) / NB);
     for (int k = 0; k < T.nJ; k++) {
@@ -2314,7 +2339,7 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB)
         for (int m = 0; m < NB; m++) {
             double v = 0.0;
             for (int i = 0; i < NB; i++) v += val[i] * std::cos(m * M_PI * (i + 0.5) / NB);
-            c[(size_t)m * T.nJ + k] = (float)(v * (m == 0 ? 1.0 : 2.0) / NB);
+            c[(size_t)m * cl + 16 + k] = (float)(v * (m == 0 ? 1.0 : 2.0) / NB);
         }
     }
     return c;
@@ -2838,7 +2863,7 @@ struct AdjLaunch {
     std::vector<int> order;
     bool basis = false;      // basis adjoint (ism_adjoint_basis_kernel)
     std::vector<float> actab;
-    int adl = 0, agl = 0;
+    int adl = 0, agl = 0, aks = 0, acl = 0;
     size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, off_ord = 0, off_act = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
     size_t lds = 0;
@@ -2888,10 +2913,13 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
             ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         }
         A.adl = (int)align_up((size_t)(BT + A.T.nJ - 1), 16);
-        A.agl = (int)align_up((size_t)(A.adl + A.T.nJ), 4);
+        A.aks = 4 * ((A.T.nJ + 15 + 15) / 16);
+        A.acl = 4 * A.aks + 16;
+        const int nota = (A.adl + 255) / 256;
+        A.agl = (int)align_up((size_t)std::max(A.adl + A.T.nJ - 1, 256 * nota - 16 + 4 * A.aks + 4), 16);
         const size_t nt = align_up((size_t)ntab, 4);
-        A.lds = 20 * nt + 16 * (nt + 4) + 4 * (size_t)A.agl + 4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB +
-                4 * (size_t)QB + 64 + 4 * (size_t)ADJ_NB * A.T.nJ;
+        A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 4) + 4 * (size_t)ADJ_NB * A.acl +
+                4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB + 4 * (size_t)QB + 64;
         if ((opt.method == METHOD_AUTO && tiles < AUTO_BASIS_MIN_TILES) || A.lds > 150 * 1024) {
             GPURIR_CHECK(opt.method == METHOD_AUTO, "basis adjoint: LDS budget exceeded");
             A.basis = false;
@@ -2903,7 +2931,7 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         A.opt = opt;
         A.mask = (A.Tw % 2) != 0;
         A.P = make_plan_basis(rooms, ob, A.T, false);
-        A.actab = make_adjoint_coefficients(A.T, A.Tw, ADJ_NB);
+        A.actab = make_adjoint_coefficients(A.T, A.Tw, ADJ_NB, A.acl);
     } else {
         if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
         if (opt.chunk_images <= 0) {
@@ -3044,6 +3072,8 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.actab = reinterpret_cast<const float*>(dp + A.off_act);
         a.adl = A.adl;
         a.agl = A.agl;
+        a.aks = A.aks;
+        a.acl = A.acl;
         const void* fn = A.basis  ? reinterpret_cast<const void*>(&ism_adjoint_basis_kernel)
                          : A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
                                   : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);

@@ -693,10 +693,12 @@ constexpr int NWB = WGB / 64;
 constexpr int QB = 2048;    // queue entries per round
 constexpr int NOT = BT / 256;  // 256-sample output blocks of a tile (16 x 16 products)
 static_assert(NWB == 4 * NOT, "four basis quarters per output block");
-constexpr int BASIS_NB = 8; // Chebyshev terms
+constexpr int BASIS_NB = 8; // Chebyshev terms (FP32: within 5e-7 of each tap)
+constexpr int BASIS_NB_FP16 = 6;  // mixed precision: 6 terms, within 6e-5 (1e-4 relative L2)
 constexpr int BCS = 2;      // columns per thread of the in-place conversion: DL <= BCS * WGB
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
 // Inclusive prefix sum over the 64 lanes of a wave with DPP moves (VALU only):
 // row_shr 1, 2, 4, 8 within each row of 16 lanes (lanes shifted in from outside the
@@ -770,10 +772,12 @@ __device__ inline void axis_range(const AxisRuns& A, int q, double u, double v, 
 }
 
 // DLC: the basis row length as a compile-time constant (the LDS offsets of an image's
-// eight atomics are then immediates), or 0 for the launch's a.bdl.
-template <int NB, int DLC>
+// eight atomics are then immediates), or 0 for the launch's a.bdl.  H16 (mixed precision):
+// the filters take FP16 inputs (D scaled per tile by a power of two, the coefficients
+// rounded) with FP32 accumulation, v_mfma_f32_16x16x16_f16.
+template <int NB, int DLC, bool H16>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
-    static_assert(NB % 4 == 0, "NB/4 basis rows per wave");
+    static_assert(NB >= 4 && NB <= 8, "basis rows mq, mq + 4 of wave quarter mq");
     const Item it = a.items[item];
     const RoomK R = load_room(a.rooms + it.room);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -786,14 +790,19 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     int* xrun = reinterpret_cast<int*>(att + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
     int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4] rows before x entry n
     long long* D = reinterpret_cast<long long*>(rowStart + ntab + 4);  // [NB][DL] fixed point
-    float* ctab = reinterpret_cast<float*>(D + NB * DL);              // [NB][CL]
-    double* rb2 = reinterpret_cast<double*>(ctab + NB * CL);          // [WGB] row x^2 + y^2
+    float* ctab = reinterpret_cast<float*>(D + NB * DL);              // [NB][CL] (H16: see chs)
+    // H16: four copies of the coefficients in FP16, copy rho shifted by rho entries, so that a
+    // lane's four A operands start at a multiple of 4 halves: [4][NB][CL + 4]
+    _Float16* chs = reinterpret_cast<_Float16*>(ctab);
+    const int crw = H16 ? 2 * NB * (CL + 4) : NB * CL;                 // floats of the region
+    double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
     int* queue = rxy + WGB;                                           // [QB] slot << 12 | nz
     int* misc = queue + QB;                                           // [16] scan scratch, att maxima
     float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
     float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
+    _Float16* Dh = reinterpret_cast<_Float16*>(D);                    // H16: [NB][DLP] (aliases D)
 
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
@@ -801,7 +810,16 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     if (tid < 4) misc[8 + tid] = 0;
     build_axis_tables(R, ps, pr, offs, att, tid, WGB);
     for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
-    for (int e = tid; e < NB * CL; e += WGB) ctab[e] = a.ctab[e];
+    if (H16) {
+        for (int e = tid; e < 4 * NB * (CL + 4); e += WGB) {
+            const int rho = e / (NB * (CL + 4)), rm = e - rho * (NB * (CL + 4));
+            const int m = rm / (CL + 4), z = rm - m * (CL + 4), src = z - rho;
+            chs[e] = (src >= 0 && src < CL) ? (_Float16)a.ctab[m * CL + src] : (_Float16)0.0f;
+        }
+        if (tid == 0) misc[12] = 0;
+    } else {
+        for (int e = tid; e < NB * CL; e += WGB) ctab[e] = a.ctab[e];
+    }
     float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
     if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
     if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
@@ -962,22 +980,37 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         // fixed point -> FP32 in output units, in place: thread tid converts columns
         // d = tid + s WGB (s < BCS) of every row into registers, and after the barrier
         // writes them to m DLP + d + 4 (d >> 4)
+        float out_scale = 1.0f;  // H16: the power of two that undoes the tile's FP16 scale
         {
             float fv[NB][BCS];
+            float mx = 0.0f;
 #pragma unroll
             for (int m = 0; m < NB; m++)
 #pragma unroll
                 for (int sl = 0; sl < BCS; sl++) {
                     const int d = tid + sl * WGB;
                     fv[m][sl] = d < DL ? (float)((double)D[m * DL + d] * (double)inv_scale) : 0.0f;
+                    mx = fmaxf(mx, fabsf(fv[m][sl]));
                 }
+            if (H16) atomicMax(misc + 12, __float_as_int(mx));  // non-negative floats order as ints
             __syncthreads();
+            float sc = 1.0f;
+            if (H16) {
+                // the tile's largest |D| maps into [2^14, 2^15): inside the FP16 range
+                const float big = __int_as_float(misc[12]);
+                const int e = (big > 0.0f && big < 3.0e38f) ? 14 - ilogbf(big) : 0;
+                sc = ldexpf(1.0f, min(max(e, -120), 120));
+                out_scale = ldexpf(1.0f, -min(max(e, -120), 120));
+            }
 #pragma unroll
             for (int m = 0; m < NB; m++)
 #pragma unroll
                 for (int sl = 0; sl < BCS; sl++) {
                     const int d = tid + sl * WGB;
-                    if (d < DL) Df[m * DLP + d + 4 * (d >> 4)] = fv[m][sl];
+                    if (d < DL) {
+                        if (H16) Dh[m * DLP + d + 4 * (d >> 4)] = (_Float16)(fv[m][sl] * sc);
+                        else Df[m * DLP + d + 4 * (d >> 4)] = fv[m][sl];
+                    }
                 }
             __syncthreads();
         }
@@ -989,9 +1022,30 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             const int ot = wave % NOT, mq = wave / NOT;
             const int c = lane & 15, g = lane >> 4;
             floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (H16) {
+                // four k per lane and instruction: K steps of 4 per lane group, KS / 4 products
+                const int rho = c & 3;
 #pragma unroll
-            for (int mm = 0; mm < NB / 4; mm++) {
-                const int m = mq * (NB / 4) + mm;
+                for (int mm = 0; mm < (NB + 3) / 4; mm++) {
+                    // rows 2 mq, 2 mq + 1 (NB = 8) or mq, mq + 4 (NB = 6: waves 2 and 3 take one)
+                    const int m = NB % 4 == 0 ? mq * (NB / 4) + mm : mq + 4 * mm;
+                    if (NB % 4 != 0 && m >= NB) break;
+                    const _Float16* Ap = chs + (rho * NB + m) * (CL + 4) + g * KS + 16 - c + rho;
+                    const _Float16* Bp = Dh + m * DLP;
+                    const int kb = 256 * ot + 16 * c + g * KS;
+                    for (int t4 = 0; t4 < KS / 4; t4++) {
+                        const int dl = kb + 4 * t4;
+                        const half4_t xv = *reinterpret_cast<const half4_t*>(Bp + dl + 4 * (dl >> 4));
+                        const half4_t av = *reinterpret_cast<const half4_t*>(Ap + 4 * t4);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, xv, acc, 0, 0, 0);
+                    }
+                }
+                acc = acc * out_scale;
+            } else
+#pragma unroll
+            for (int mm = 0; mm < (NB + 3) / 4; mm++) {
+                const int m = NB % 4 == 0 ? mq * (NB / 4) + mm : mq + 4 * mm;
+                if (NB % 4 != 0 && m >= NB) break;
                 const float* Ap = ctab + m * CL + g * KS + 16 - c;
                 const float* Bp = Df + m * DLP;
                 const int kb = 256 * ot + 16 * c + g * KS;
@@ -1025,18 +1079,19 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             orow[tl + t] = ((ybuf[t] + ybuf[BT + t]) + ybuf[2 * BT + t]) + ybuf[3 * BT + t];
         __syncthreads();
         for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
+        if (H16 && tid == 0) misc[12] = 0;  // read by every thread before the barrier above
         // the row scan of the next tile has barriers before its first atomic
     }
 }
 
-template <int NB, int DLC>
+template <int NB, int DLC, bool H16>
 __global__ __launch_bounds__(WGB) void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    basis_item<NB, DLC>(a, blockIdx.x, smem_raw);
+    basis_item<NB, DLC, H16>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
-template <int NB, int DLC>
+template <int NB, int DLC, bool H16>
 __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -1046,7 +1101,7 @@ __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, in
         const int item = next;
         __syncthreads();
         if (item >= n_items) break;
-        basis_item<NB, DLC>(a, item, smem_raw);
+        basis_item<NB, DLC, H16>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -2303,15 +2358,14 @@ double windowed_sinc(double x, int Tw) {
     return x == 0.0 ? w : w * std::sin(M_PI * x) / (M_PI * x);
 }
 
-BasisTables make_basis_tables(const TapTables& T, int Tw) {
-    constexpr int NB = BASIS_NB;
+BasisTables make_basis_tables(const TapTables& T, int Tw, int NB) {
     BasisTables B;
     B.ks = 4 * ((T.nJ + 15 + 15) / 16);
     B.cl = 4 * B.ks + 16;
     B.dl = (int)align_up((size_t)std::max(BT + T.nJ - 1, BT - 16 + 4 * B.ks), 16);
     B.ctab.assign((size_t)NB * B.cl, 0.0f);
     const int J1 = T.J0 + T.nJ - 1;
-    double fn[NB], val[NB];
+    std::vector<double> fn(NB), val(NB);
     for (int i = 0; i < NB; i++) fn[i] = 0.5 + 0.5 * std::cos(M_PI * (i + 0.5) / NB);  // u_i = 2 f_i - 1
     for (int x = 0; x < T.nJ; x++) {
         const int j = J1 - x;
@@ -2345,9 +2399,10 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB,
     return c;
 }
 
-size_t basis_lds_bytes(int ntab, const BasisTables& B) {
-    return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + 8 * (size_t)BASIS_NB * B.dl +
-           4 * (size_t)BASIS_NB * B.cl + (8 + 4 + 4) * (size_t)WGB + 4 * (size_t)QB + 4 * 16;
+size_t basis_lds_bytes(int ntab, const BasisTables& B, bool h16, int nb) {
+    const size_t coef = h16 ? 8 * (size_t)nb * (B.cl + 4) : 4 * (size_t)nb * B.cl;
+    return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + 8 * (size_t)nb * B.dl + coef +
+           (8 + 4 + 4) * (size_t)WGB + 4 * (size_t)QB + 4 * 16;
 }
 
 // Basis plan: one item per RIR and range of BT-sample tiles (tile_samples rounded up
@@ -2435,7 +2490,9 @@ struct SimLaunch {
     Plan P;
     TapTables T;
     BasisTables Bt;
-    bool basis = false;  // FP32 accumulation by the basis method
+    bool basis = false;  // accumulation by the basis method (FP32, or FP16 filter inputs)
+    bool h16 = false;    // mixed precision: FP16 inputs of the basis filters
+    int nb = 8;          // Chebyshev terms of the basis method
     int prec = PREC_FP32;
     int Tw = 0;
     size_t off_rooms = 0, off_items = 0, off_rr = 0, off_jt = 0, off_lut = 0, off_sb = 0, off_ct = 0, plan_bytes = 0;
@@ -2474,7 +2531,9 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         S.T = f->second;
     }
     GPURIR_CHECK(opt.method >= METHOD_AUTO && opt.method <= METHOD_BASIS, "unknown accumulation method");
-    S.basis = S.prec == PREC_FP32 && opt.method != METHOD_TAPS;
+    S.basis = (S.prec == PREC_FP32 || S.prec == PREC_FP16) && opt.method != METHOD_TAPS;
+    S.h16 = S.prec == PREC_FP16;
+    S.nb = S.h16 ? BASIS_NB_FP16 : BASIS_NB;
     if (S.basis && opt.method == METHOD_AUTO) {
         // a launch of fewer tiles than the device has CUs (e.g. one room with two
         // receivers) is faster as a tap scatter, whose image chunks fill the device
@@ -2486,16 +2545,17 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     if (S.basis) {
         // the tables depend on Tw only (about 10^4 transcendental calls): built once
         static std::mutex mu;
-        static std::map<int, BasisTables> cache;
+        static std::map<std::pair<int, int>, BasisTables> cache;
         {
             std::lock_guard<std::mutex> lk(mu);
-            auto f = cache.find(S.Tw);
-            if (f == cache.end()) f = cache.emplace(S.Tw, make_basis_tables(S.T, S.Tw)).first;
+            const auto key = std::make_pair(S.Tw, S.nb);
+            auto f = cache.find(key);
+            if (f == cache.end()) f = cache.emplace(key, make_basis_tables(S.T, S.Tw, S.nb)).first;
             S.Bt = f->second;
         }
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
-        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
             // scatter takes the launch
@@ -2534,7 +2594,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
             (size_t)(QCAP + QPAD) * 5 * 4 + 16 + sizeof(float2) * T.lut.size();
-    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt);
+    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb);
     GPURIR_CHECK(T.n_pass <= 12, "sampling frequency too high: at most 768 window taps are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -2634,10 +2694,13 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
         // the 16 kHz window (128 taps) has its row length compiled in
         const bool dl640 = S.Bt.dl == 640;
-        const void* fn_once = dl640 ? reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 640>)
-                                    : reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 0>);
-        const void* fn_pers = dl640 ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 640>)
-                                    : reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 0>);
+#define GPURIR_BASIS_FN(K, N, H) \
+    (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H>) : reinterpret_cast<const void*>(&K<N, 0, H>))
+        const void* fn_once = S.h16 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, true)
+                                    : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, false);
+        const void* fn_pers = S.h16 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, true)
+                                    : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, false);
+#undef GPURIR_BASIS_FN
         int n_items = (int)S.P.items.size();
         int resident = 0;
         if (S.counters) {

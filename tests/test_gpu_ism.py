@@ -240,6 +240,9 @@ def test_mid_room_fp16_packed():
 
 @pytest.mark.parametrize("fs", ALL_FS)
 def test_fp16_sampling_rates(fs):
+    """Mixed precision, both accumulations: the packed-FP16 tap scatter (what the automatic
+    choice takes for this small launch) and the basis method with FP16 filter inputs (up to
+    64 kHz; above, a basis request is rejected)."""
     cfg = W.plain_room()
     cfg["fs"] = fs
     cfg["Tmax"] = 0.12
@@ -247,7 +250,23 @@ def test_fp16_sampling_rates(fs):
     cfg["nb_img"] = gpuRIR.t2n(0.06, cfg["room_sz"])
     g, o = run_both(cfg, precision="fp16")
     e = check_parity(g, o, orc.n_samples(0.06, fs), tol=FP16_TOL)
-    print(f"fs {fs:.0f} FP16 rel-L2 {e:.3e}")
+    print(f"fs {fs:.0f} FP16 taps rel-L2 {e:.3e}")
+    if fs <= 64000.0:
+        g, o = run_both(cfg, precision="fp16", method="basis")
+        e = check_parity(g, o, orc.n_samples(0.06, fs), tol=FP16_TOL)
+        print(f"fs {fs:.0f} FP16 basis rel-L2 {e:.3e}")
+
+
+def test_fp16_basis_is_the_automatic_choice_for_large_launches():
+    """The mid room (128 RIRs x 7 tiles) takes the FP16 basis path by default: the same bits as
+    an explicit basis request, within 1e-3 of FP64, and within 1e-3 of the FP32 basis result."""
+    cfg = W.mid_room()
+    g_a, o = run_both(cfg, precision="fp16")
+    g_b, _ = run_both(cfg, precision="fp16", method="basis")
+    g_f, _ = run_both(cfg, precision="fp32")
+    assert np.array_equal(g_a, g_b)
+    check_parity(g_a, o, 3200, tol=FP16_TOL)
+    assert rel_l2(g_a[..., :3200], g_f[..., :3200]) < FP16_TOL
 
 
 def test_fp16_drop_in_toggle(capfd):

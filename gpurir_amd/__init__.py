@@ -8,9 +8,9 @@ the C ABI of libgpurir.so (include/gpurir.h); there is no CPU fallback.
 Differences that follow from the design (see the repository documentation):
   * the diffuse tail draws its noise from Philox4x32-10 with the library's seed
     (1234, as the reference seeds cuRAND) and a per-call counter, not from cuRAND;
-  * activateMixedPrecision selects packed-FP16 evaluation of the windowed sinc with
-    sums kept in 64-bit fixed point (at least FP32 accuracy) instead of the reference's
-    half2 accumulation; the reference's messages and the mutual exclusion with the
+  * activateMixedPrecision selects FP16 arithmetic with FP32 accumulation (packed-FP16
+    taps on small launches, FP16 filter inputs on the matrix cores on large ones)
+    instead of the reference's half2 accumulation; the reference's messages and the mutual exclusion with the
     lookup table are kept.
 """
 from __future__ import division

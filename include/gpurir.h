@@ -35,16 +35,19 @@ extern "C" {
 /* Evaluation of the windowed sinc (src/gpuRIR_cuda.h:29-33 flags mixed_precision / lookup_table) */
 #define GPURIR_PREC_FP32 0 /* exact FP32 sinc (reference: both flags off)            */
 #define GPURIR_PREC_LUT 1  /* 1/16-sample table + linear interpolation (lookup_table) */
-#define GPURIR_PREC_FP16 2 /* packed FP16 windowed sinc, FP32 accumulation (mixed_precision) */
+#define GPURIR_PREC_FP16 2 /* FP16 arithmetic, FP32 accumulation (mixed_precision): FP16 */
+                           /* filter inputs on the matrix cores (basis) or packed FP16  */
+                           /* windowed-sinc taps (tap scatter)                          */
 
-/* Accumulation of the FP32 exact sinc (other precisions always scatter taps) */
+/* Accumulation of the exact sinc, FP32 or mixed precision (the lookup table always scatters taps) */
 #define GPURIR_METHOD_AUTO 0  /* library default: BASIS, or TAPS for launches of fewer    */
                               /* than 256 tiles of 512 early samples (too few to fill    */
                               /* the device) and for windows of more than 513 taps     */
                               /* (fs above 64 kHz)                                       */
 #define GPURIR_METHOD_TAPS 1  /* every image scatters its Tw windowed-sinc taps           */
 #define GPURIR_METHOD_BASIS 2 /* 8-term Chebyshev expansion of the taps in the fractional */
-                              /* delay; matrix-core FIR (<= 5e-7 per tap of 1)             */
+                              /* delay; matrix-core FIR (<= 5e-7 per tap of 1); 6 terms  */
+                              /* and FP16 filter inputs in mixed precision (<= 6e-5)      */
 
 /* Diffuse-tail noise */
 #define GPURIR_NOISE_LOGISTIC 0 /* unit-variance logistic, as src/gpuRIR_cuda.cu:454-455 */
@@ -79,8 +82,8 @@ int gpurir_n_samples(float T, float fs);
  * state of the flag.  Replace gpuRIR_cuda::activate_mixed_precision / activate_lut
  * (src/gpuRIR_cuda.cu:1050-1073) bound as activate_mixed_precision_bind /
  * activate_lut_bind (src/python_bind.cpp:121-128,140-142).  Mixed precision selects
- * GPURIR_PREC_FP16 for the plain entry point; its FP16 arithmetic is the packed
- * instructions of gfx950 (the reference's Pascal-architecture check has no equivalent). */
+ * GPURIR_PREC_FP16 for the plain entry point; its FP16 arithmetic is the packed FP16
+ * and FP16 matrix-core instructions of gfx950 (the reference's Pascal-architecture check has no equivalent). */
 int gpurir_activate_mixed_precision(int activate);
 int gpurir_activate_lut(int activate);
 

@@ -1277,20 +1277,49 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
         const float r = ((float)k + 0.5f) / (float)X;
         o[t] = fmaf(r, gain * env * z, (1.0f - r) * o[t]);
     }
+    // Tail: noise block B (samples 4B..4B+3) is one Philox call.  The stores go out as aligned
+    // 16-byte groups: group q holds samples off + 4q .. off + 4q + 3 (off puts tail + off on a
+    // 16-byte boundary), i.e. the last 4 - off values of block q and the first off of block q + 1.
+    // Lane l of a wave draws block B = b0 + l and writes group B - 1, taking block B - 1 from
+    // lane l - 1; lane 0 draws the block that the previous range's lane 63 drew, so the ranges
+    // of successive waves advance by 63 blocks and every group is written once.
     float* tail = o + nISM;
     const int nblk = (nDiff + 3) >> 2;
-    for (int blk = tid; blk < nblk; blk += WG) {
-        unsigned w[4];
-        philox4x32_10((unsigned)blk, (unsigned)m, R.room_id, a.call_offset, k0, k1, w);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int off = (4 - (int)((reinterpret_cast<unsigned long long>(tail) >> 2) & 3)) & 3;
+    for (int b0 = 63 * wave; b0 <= nblk; b0 += 63 * NWAVE) {
+        const int B = b0 + lane;
+        float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (B < nblk) {
+            unsigned w[4];
+            philox4x32_10((unsigned)B, (unsigned)m, R.room_id, a.call_offset, k0, k1, w);
+#pragma unroll
+            for (int u = 0; u < 4; u++) z[u] = 4 * B + u < nDiff ? tail_noise(w, u, a.noise) : 0.0f;
+        }
+        float p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = __shfl_up(z[u], 1);
+        if (lane == 0 && b0 > 0) continue;
+        float v[4];
+        switch (off) {  // uniform over the workgroup
+            case 0: v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3]; break;
+            case 1: v[0] = p[1]; v[1] = p[2]; v[2] = p[3]; v[3] = z[0]; break;
+            case 2: v[0] = p[2]; v[1] = p[3]; v[2] = z[0]; v[3] = z[1]; break;
+            default: v[0] = p[3]; v[1] = z[0]; v[2] = z[1]; v[3] = z[2]; break;
+        }
+        const int sd0 = off + 4 * (B - 1);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int sd = 4 * blk + u;
-            if (sd < nDiff) {
-                const float z = tail_noise(w, u, a.noise);
-                // exp(x) = 2^(x log2 e) with the hardware exp2 (relative error ~1e-7)
-                const float env = __builtin_amdgcn_exp2f(fmaf((float)sd, ha, base) * 1.4426950408889634f);
-                tail[sd] = gain * env * z;
-            }
+            // exp(x) = 2^(x log2 e) with the hardware exp2 (relative error ~1e-7)
+            const float env = __builtin_amdgcn_exp2f(fmaf((float)(sd0 + u), ha, base) * 1.4426950408889634f);
+            v[u] = gain * env * v[u];
+        }
+        if (sd0 >= 0 && sd0 + 3 < nDiff) {
+            *reinterpret_cast<float4*>(tail + sd0) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (sd0 + u >= 0 && sd0 + u < nDiff) tail[sd0 + u] = v[u];
         }
     }
 }

@@ -209,7 +209,7 @@ def main():
             o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
                                  Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
             errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
-        res = {"precision": precision, "method": method if precision == "fp32" else "taps", "autotune": tune,
+        res = {"precision": precision, "method": method if precision in ("fp32", "fp16") else "taps", "autotune": tune,
                "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
                "rirs_per_s": n_rirs / (ms_step / 1e3), "accumulate_ms": ms_acc, "finalize_ms": ms_fin,
                "tflops_algorithmic": FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12, "rel_l2_vs_fp64": max(errs),

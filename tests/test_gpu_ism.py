@@ -160,6 +160,25 @@ def test_full_ism_no_tail_and_all_tail():
     assert np.allclose(np.nan_to_num(g), np.nan_to_num(o), atol=1e-6)
 
 
+@pytest.mark.parametrize("n_s,n_ism", [(4002, 1602), (4002, 1600), (4004, 1602), (1604, 1602),
+                                       (1606, 1602), (1608, 1602), (1854, 1600), (1852, 1600)])
+def test_tail_alignment_and_short_tails(n_s, n_ism):
+    """Sample counts are even (the reference's rounding), so a row length of 2 mod 4 puts alternate
+    receivers' tails 8 bytes off a 16-byte boundary, and an early part of 2 mod 4 does so for all;
+    tails of 2, 4 and 6 samples, and of 4 * 63 (+2) samples (a 64-lane range of noise blocks
+    advances by 63), cover the ends of the aligned group stores."""
+    cfg = W.plain_room()
+    rng = np.random.default_rng(5)
+    cfg["pos_rcv"] = rng.uniform(0.3, 2.2, size=(5, 3))
+    cfg["Tmax"] = (n_s - 0.5) / cfg["fs"]
+    cfg["Tdiff"] = (n_ism - 0.5) / cfg["fs"]
+    g, o = run_both(cfg)
+    assert g.shape == (1, 5, n_s) and o.shape == (1, 5, n_s)
+    check_parity(g, o, n_ism)
+    # every tail sample is written: the noise is non-zero almost surely
+    assert np.count_nonzero(g[..., n_ism:]) == 5 * (n_s - n_ism)
+
+
 def test_tiny_room_single_image():
     cfg = W.plain_room()
     cfg["nb_img"] = [1, 1, 1]

@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's default work-item shape")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "lut"],
                     help="sinc evaluation of the batch measured by --only-batch")
+    ap.add_argument("--method", default="auto", choices=["auto", "taps", "basis"],
+                    help="accumulation of the batch measured by --only-batch (tap scatter or basis expansion)")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rooms", type=int, default=4096)
@@ -222,7 +224,10 @@ def main():
         return res
 
     modes = [args.precision] if args.only_batch else ["fp32", "fp32_taps", "fp16", "lut"]
-    batch = {m: (measure_batch("fp32", "taps") if m == "fp32_taps" else measure_batch(m)) for m in modes}
+    if args.only_batch:
+        batch = {args.precision: measure_batch(args.precision, args.method)}
+    else:
+        batch = {m: (measure_batch("fp32", "taps") if m == "fp32_taps" else measure_batch(m)) for m in modes}
     main_mode = args.precision if args.only_batch else "fp32"
     B = batch[main_mode]
     n_rirs, taps, ms_step, rirs_per_s = B["rirs"], B["taps"], B["ms_per_step"], B["rirs_per_s"]

@@ -401,3 +401,40 @@ def test_crossfade_option():
         o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
                              Tdiff=r["Tdiff"], room_index=k, crossfade=64)
         check_parity(views[k].cpu().numpy(), o, orc.n_samples(r["Tdiff"], W.FS))
+
+
+def test_basis_tails_independent_of_item_split():
+    """Basis method: the RIRs (early part and tail) match the oracle and are bit-identical however
+    the early part is split into work items (tiles keep their 512-sample grid and the envelope
+    window is summed in a fixed order); a cross-fade matches the oracle; a batch room with no early
+    part (Tdiff below every direct path) gives the reference's degenerate envelope."""
+    cfg = W.mid_room(n_src=2, n_rcv=8, seed=6)
+    nISM = orc.n_samples(cfg["Tdiff"], cfg["fs"])
+    g0, o = run_both(cfg, method="basis")
+    check_parity(g0, o, nISM)
+    for ts in (512, 1536, 4096):
+        g1, _ = run_both(cfg, method="basis", tile_samples=ts)
+        assert np.array_equal(g1, g0), ts
+    args = dict(room_sz=cfg["room_sz"], beta=cfg["beta"], pos_src=cfg["pos_src"], pos_rcv=cfg["pos_rcv"],
+                nb_img=cfg["nb_img"], Tmax=cfg["Tmax"], fs=cfg["fs"], Tdiff=cfg["Tdiff"],
+                spkr_pattern=cfg["spkr_pattern"], mic_pattern=cfg["mic_pattern"], orV_src=cfg["orV_src"],
+                orV_rcv=cfg["orV_rcv"])
+    g2 = T.simulate_rir(**args, crossfade=160, method="basis").cpu().numpy()
+    o2 = orc.simulate_rir(**args, crossfade=160)
+    check_parity(g2, o2, nISM)
+    # a batch whose rooms include one with no early part (Tdiff below every direct path) leaves that
+    # room's envelope to the finalize kernel
+    rooms = W.uneven_rooms(n_rooms=40, seed=9)
+    rooms[3]["Tdiff"] = 0.0
+    _, views = T.simulate_batch(rooms, W.FS, method="basis")
+    torch.cuda.synchronize()
+    for k in (0, 3, 39):
+        r = rooms[k]
+        ob = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], W.FS,
+                              Tdiff=r["Tdiff"], room_index=k)
+        gb = views[k].cpu().numpy()
+        if k == 3:
+            assert np.array_equal(np.isnan(gb), np.isnan(ob))
+            assert np.allclose(np.nan_to_num(gb), np.nan_to_num(ob), atol=1e-6)
+        else:
+            check_parity(gb, ob, orc.n_samples(r["Tdiff"], W.FS))

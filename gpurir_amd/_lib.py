@@ -64,6 +64,7 @@ _SIG = {
     "gpurir_batch_plan_out_elems": (ctypes.c_longlong, [ctypes.c_void_p]),
     "gpurir_batch_plan_n_rirs": (ctypes.c_longlong, [ctypes.c_void_p]),
     "gpurir_batch_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "gpurir_batch_plan_bind_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gpurir_batch_plan_count_taps": (ctypes.c_int, [ctypes.c_void_p, c_ll_p, ctypes.c_void_p]),
     "gpurir_batch_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int_p, c_int_p, c_int_p,
                                                   c_float_p]),

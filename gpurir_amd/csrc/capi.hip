@@ -208,6 +208,8 @@ struct gpurir_plan {
     void* d_adj_plan = nullptr;
     void* d_adj_ws = nullptr;
     unsigned* d_counters = nullptr;  // persistent accumulation (chosen by the autotuner)
+    const float* d_beta = nullptr;   // bound by gpurir_batch_plan_bind_params (null: planned values)
+    const float* d_level = nullptr;
 };
 
 // ---- autotuner: work-item shape (images per item, samples per tile) and launch form
@@ -222,8 +224,10 @@ std::map<std::string, TuneChoice> g_tune_cache;
 void drop_graph(gpurir_plan* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     p->exec = nullptr;
     p->graph = nullptr;
+    p->cap_stream = nullptr;
 }
 
 // Replace the plan's launch by one prepared with so (and the persistent form if asked).
@@ -261,6 +265,7 @@ int plan_rebuild(gpurir_plan* p, const SimOptions& so, bool persistent) {
         }
     }
     sim_set_counters(*s, persistent ? p->d_counters : nullptr);
+    sim_set_params(*s, p->d_beta, p->d_level);
     drop_graph(p);
     if (p->d_plan) (void)hipFree(p->d_plan);
     if (p->d_part) (void)hipFree(p->d_part);
@@ -296,6 +301,7 @@ int time_config(gpurir_plan* p, const SimOptions& so, bool persistent, hipStream
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = -2;
     if (!rc) {
         sim_set_counters(*s, persistent ? static_cast<unsigned*>(dc) : nullptr);
+        sim_set_params(*s, p->d_beta, p->d_level);
         float best = 1e30f;
         for (int k = 0; k < 3 && !rc; k++) {
             if (hipEventRecord(e0, st) != hipSuccess) rc = -2;
@@ -566,16 +572,25 @@ int gpurir_batch_plan_run(gpurir_plan_t p, void* stream) {
     GPURIR_CHECK(p != nullptr, "null plan");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (!p->exec) {
+        drop_graph(p);  // a graph whose instantiation failed, and the previous capture stream
         GPURIR_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         GPURIR_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
         int rc = sim_enqueue(*p->sim, p->d_plan, p->d_part, p->pos_src, p->pos_rcv, p->orv_src, p->orv_rcv, p->out,
                              p->cap_stream);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
-        if (rc) return rc;
-        GPURIR_HIP(e);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            GPURIR_HIP(e);
+        }
         p->graph = g;
-        GPURIR_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+        hipError_t ie = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            p->exec = nullptr;
+            drop_graph(p);
+            GPURIR_HIP(ie);
+        }
     }
     GPURIR_HIP(hipGraphLaunch(p->exec, st));
     return 0;
@@ -653,6 +668,7 @@ int gpurir_batch_plan_adjoint(gpurir_plan_t p, const float* d_grad_out, float* d
             set_error("adjoint plan upload failed");
             return -2;
         }
+        adj_set_params(*a, p->d_beta, p->d_level);
         p->adj = a;
         p->d_adj_plan = dp;
         p->d_adj_ws = dw;
@@ -740,8 +756,20 @@ int gpurir_batch_plan_autotune(gpurir_plan_t p, void* stream, int* chunk_images,
     return 0;
 }
 
+int gpurir_batch_plan_bind_params(gpurir_plan_t p, const float* d_beta, const float* d_level) {
+    GPURIR_CHECK(p != nullptr, "null plan");
+    p->d_beta = d_beta;
+    p->d_level = d_level;
+    sim_set_params(*p->sim, d_beta, d_level);
+    if (p->adj) adj_set_params(*p->adj, d_beta, d_level);
+    drop_graph(p);  // the next run captures launches that read the bound arrays
+    return 0;
+}
+
 int gpurir_batch_plan_destroy(gpurir_plan_t p) {
     if (!p) return 0;
+    // replays and eager launches of the plan may still be running on any stream
+    (void)hipDeviceSynchronize();
     if (p->d_counters) (void)hipFree(p->d_counters);
     if (p->d_adj_plan) (void)hipFree(p->d_adj_plan);
     if (p->d_adj_ws) (void)hipFree(p->d_adj_ws);

@@ -118,6 +118,10 @@ void sim_set_stream_opts(SimLaunch& S, unsigned long long seed, unsigned call_of
 // Persistent accumulation: d_counters (two unsigned, zero before the first launch and
 // left zero by every launch) switches the launch to the persistent kernel; NULL switches back.
 void sim_set_counters(SimLaunch& S, unsigned* d_counters);
+// Betas [n_rooms][6] and diffuse levels [n_rooms] read from device memory at every launch
+// (either may be null: the values planned from RoomHost).  A captured launch reads the
+// arrays' current contents at each replay.
+void sim_set_params(SimLaunch& S, const float* d_beta, const float* d_level);
 const SimOptions& sim_options_of(const SimLaunch& S);
 // true if the launch accumulates by the basis method (no image chunks)
 bool sim_is_basis(const SimLaunch& S);
@@ -138,6 +142,7 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, AdjLa
 size_t adj_plan_bytes(const AdjLaunch& A);
 const void* adj_blob(const AdjLaunch& A);
 size_t adj_ws_bytes(const AdjLaunch& A);
+void adj_set_params(AdjLaunch& A, const float* d_beta, const float* d_level);  // as sim_set_params
 int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float* d_pos_src, const float* d_pos_rcv,
                 const float* d_orv_src, const float* d_orv_rcv, const float* d_y, const float* d_gy, float* g_beta,
                 float* g_src, float* g_rcv, float* g_level, hipStream_t stream);

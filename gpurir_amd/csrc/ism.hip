@@ -65,6 +65,15 @@ __device__ inline float h2_bits(half2_t v) { return __builtin_bit_cast(float, v)
 __device__ inline half2_t bits_h2(float v) { return __builtin_bit_cast(half2_t, v); }
 __device__ inline half2_t h2(float lo, float hi) { return half2_t{(_Float16)lo, (_Float16)hi}; }
 
+// Reflection coefficients and diffuse-tail levels read at run time: when set, room k's
+// six betas are beta[6k .. 6k+5] and its level is level[k] (device arrays that a caller may
+// update between replays of a captured plan, e.g. an optimiser step); when null, the values
+// planned into RoomDev.
+struct RoomParams {
+    const float* beta;
+    const float* level;
+};
+
 struct AccArgs {
     const RoomDev* rooms;
     const Item* items;
@@ -89,6 +98,7 @@ struct AccArgs {
     // of the Toeplitz product, and the length of one basis row of the tile
     const float* ctab;
     int bks, bdl, bcl;
+    RoomParams prm;
 };
 
 struct FinArgs {
@@ -103,6 +113,7 @@ struct FinArgs {
     unsigned call_offset;
     int noise;
     int crossfade;
+    RoomParams prm;
 };
 
 // Integer power by squaring (e >= 0); 0^0 = 1 as powf(0,0) in the reference.
@@ -205,7 +216,8 @@ struct RoomK {
     unsigned room_id;
 };
 
-__device__ inline RoomK load_room(const RoomDev* __restrict__ p) {
+__device__ inline RoomK load_room(const RoomDev* __restrict__ rooms, int k, const RoomParams& prm) {
+    const RoomDev* __restrict__ p = rooms + k;
     RoomK r;
     r.L0 = p->L[0];
     r.L1 = p->L[1];
@@ -233,6 +245,16 @@ __device__ inline RoomK load_room(const RoomDev* __restrict__ p) {
     r.out_off = p->out_off;
     r.part_off = p->part_off;
     r.room_id = p->room_id;
+    if (prm.beta) {
+        const float* b = prm.beta + 6 * k;
+        r.b0 = b[0];
+        r.b1 = b[1];
+        r.b2 = b[2];
+        r.b3 = b[3];
+        r.b4 = b[4];
+        r.b5 = b[5];
+    }
+    if (prm.level) r.level = prm.level[k];
     return r;
 }
 
@@ -402,7 +424,7 @@ __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, con
 template <int MODE, int PMAX, bool MASK>
 __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     const Item it = a.items[item];
-    const RoomK R = load_room(a.rooms + it.room);
+    const RoomK R = load_room(a.rooms, it.room, a.prm);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
@@ -412,13 +434,14 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
     // queue of live images: q4[e] = (acc index bits, f, B/2 or amp, B/2 cos), qS[e] = B/2 sin
     float4* q4 = reinterpret_cast<float4*>(att + a.ntab_max);
     float* qS = reinterpret_cast<float*>(q4 + QCAP + QPAD);
-    int* qcount = reinterpret_cast<int*>(qS + QCAP + QPAD);     // [0] queue length, [1..3] per-axis max |att| bits
-    float2* lut = reinterpret_cast<float2*>(qcount + 4);  // LUT mode: [16 * lut_row] polyphase table
+    // [0] queue length, [1..3] per-axis max |att| bits, [4..6] per-axis min |offset| bits
+    int* qcount = reinterpret_cast<int*>(qS + QCAP + QPAD);
+    float2* lut = reinterpret_cast<float2*>(qcount + 8);  // LUT mode: [16 * lut_row] polyphase table
 
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
     const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
-    if (tid < 4) qcount[tid] = 0;
+    if (tid < 8) qcount[tid] = tid < 4 ? 0 : 0x7f800000;  // maxima from 0, minima from +inf
     __syncthreads();
     build_axis_tables(R, ps, pr, offs, att, tid, WG);
     for (int e = tid; e < a.acc_len; e += WG) acc[e] = 0;
@@ -450,17 +473,19 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
     if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
     if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
 
-    // fixed-point scale: |tap| <= max|att| / (4 pi d_direct) (no image is closer than the
-    // direct path; directivity gains and the windowed sinc are bounded by 1)
+    // fixed-point scale: |tap| <= max|att| / (4 pi d_min), d_min^2 = sum over the axes of the
+    // smallest squared offset (no image is closer; directivity gains and the windowed sinc are
+    // bounded by 1).  With the source and receiver inside the room d_min is the direct path.
     __syncthreads();  // table entries are written by other threads than the ones reading them here
     for (int e = tid; e < R.Nx + R.Ny + R.Nz; e += WG) {
         const int d = (e < R.Nx) ? 0 : (e < R.Nx + R.Ny ? 1 : 2);
         atomicMax(qcount + 1 + d, __float_as_int(fabsf(att[e])));
+        atomicMin(qcount + 4 + d, __float_as_int((float)fabs(offs[e])));
     }
     __syncthreads();
     float scale, inv_scale;
     {
-        const float dx = ps[0] - pr[0], dy = ps[1] - pr[1], dz = ps[2] - pr[2];
+        const float dx = __int_as_float(qcount[4]), dy = __int_as_float(qcount[5]), dz = __int_as_float(qcount[6]);
         const float bound = __int_as_float(qcount[1]) * __int_as_float(qcount[2]) * __int_as_float(qcount[3]) *
                             __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
         // |tap| * 2^E < 2^30 (64-bit sums), or < 2^21 in the 32-bit mixed-precision sums,
@@ -525,9 +550,10 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
                         q4[q] = make_float4(__int_as_float(n0 + acc_shift), B, h2_bits(h2(fh, fh)),
                                             h2_bits(h2(0.5f * cph, 0.5f * sph)));
                     } else {
-                        // f == 0 would make the tap at j = 0 a 0 * inf; moving tau by
-                        // 2^-30 samples changes no tap by more than ~1e-9 relative.
-                        const float fc = fmaxf(f, 0x1p-30f);
+                        // f == 0 would make the tap at j = 0 a 0 * inf, and so would f == 1 at j = 1
+                        // (a fraction within 2^-25 of 1 rounds up to 1.0f); moving tau by less
+                        // than 2^-24 samples changes no tap by more than ~1e-7 relative.
+                        const float fc = fminf(fmaxf(f, 0x1p-30f), 0x1.fffffep-1f);
                         const float B = amp * scale * sinpi01(fc) * 0.31830988618379067f;
                         const float ph = fc * a.inv_tw2;  // phi / pi = 2 f / Tw
                         float sph, cph;
@@ -779,7 +805,7 @@ template <int NB, int DLC, bool H16>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert(NB >= 4 && NB <= 8, "basis rows mq, mq + 4 of wave quarter mq");
     const Item it = a.items[item];
-    const RoomK R = load_room(a.rooms + it.room);
+    const RoomK R = load_room(a.rooms, it.room, a.prm);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ntab = a.ntab_max, DL = DLC ? DLC : a.bdl, CL = a.bcl, KS = a.bks;
@@ -808,6 +834,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     const float* ps = a.pos_src + 3 * (R.src_off + s);
     const float* pr = a.pos_rcv + 3 * (R.rcv_off + r);
     if (tid < 4) misc[8 + tid] = 0;
+    if (tid >= 4 && tid < 7) misc[9 + tid] = 0x7f800000;  // misc[13..15]: per-axis min |offset| bits
     build_axis_tables(R, ps, pr, offs, att, tid, WGB);
     for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
     if (H16) {
@@ -829,14 +856,17 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     for (int e = tid; e < ntot; e += WGB) {
         const int ax = e < Nx ? 0 : (e < Nx + Ny ? 1 : 2);
         atomicMax(misc + 9 + ax, __float_as_int(fabsf(att[e])));
+        atomicMin(misc + 13 + ax, __float_as_int((float)fabs(offs[e])));
     }
     const AxisRuns AY = axis_runs(Ny, R.L1, ps[1], pr[1]);
     const AxisRuns AZ = axis_runs(Nz, R.L2, ps[2], pr[2]);
     __syncthreads();
     float scale, inv_scale;
     {
-        // |amp T_m| <= max|att| / (4 pi d_direct): < 2^30 after scaling, 64-bit sums
-        const float dx = ps[0] - pr[0], dy = ps[1] - pr[1], dz = ps[2] - pr[2];
+        // |amp T_m| <= max|att| / (4 pi d_min), d_min from the smallest offset per axis (the
+        // direct path when the source and receiver are inside the room): < 2^30 after
+        // scaling, 64-bit sums
+        const float dx = __int_as_float(misc[13]), dy = __int_as_float(misc[14]), dz = __int_as_float(misc[15]);
         const float bound = __int_as_float(misc[9]) * __int_as_float(misc[10]) * __int_as_float(misc[11]) *
                             __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
         int E = (bound > 0.0f && bound < 3.0e38f) ? 29 - ilogbf(bound) : 0;
@@ -1221,7 +1251,7 @@ __global__ __launch_bounds__(WG) void partial_sum_kernel(const RoomDev* rooms, c
 __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     __shared__ double red[NWAVE];
     const int g = blockIdx.x;
-    const RoomK R = load_room(a.rooms + a.rir_room[g]);
+    const RoomK R = load_room(a.rooms, a.rir_room[g], a.prm);
     const int m = g - R.rir_off;
     const int s = m / R.n_rcv, r = m - s * R.n_rcv;
     const int tid = threadIdx.x;
@@ -1330,7 +1360,7 @@ __global__ __launch_bounds__(WG) void image_list_kernel(const RoomDev* rooms, co
                                                         const float* orv_rcv, int spkr, int mic, int ntab_max,
                                                         int* n0_out, float* f_out, float* amp_out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const RoomK R = load_room(rooms);
+    const RoomK R = load_room(rooms, 0, RoomParams{nullptr, nullptr});
     const int m = blockIdx.y;
     const int s = m / R.n_rcv, r = m - s * R.n_rcv;
     double* offs = reinterpret_cast<double*>(smem_raw);
@@ -1359,7 +1389,7 @@ __global__ __launch_bounds__(WG) void count_taps_kernel(AccArgs a, unsigned long
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ unsigned long long red[NWAVE];
     const Item it = a.items[blockIdx.x];
-    const RoomK R = load_room(a.rooms + it.room);
+    const RoomK R = load_room(a.rooms, it.room, a.prm);
     double* offs = reinterpret_cast<double*>(smem_raw);
     float* att = reinterpret_cast<float*>(offs + a.ntab_max);
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
@@ -1431,6 +1461,7 @@ struct AdjArgs {
     // per lane group of the correlation products
     const float* actab;
     int adl, agl, aks, acl;
+    RoomParams prm;
 };
 
 constexpr int AQ = 2 * WG;  // adjoint image queue (ring of two rounds)
@@ -1580,7 +1611,7 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
     __shared__ int wcnt[NWAVE];
     const int bid = a.order[blockIdx.x];
     const Item it = a.items[bid];
-    const RoomK R = load_room(a.rooms + it.room);
+    const RoomK R = load_room(a.rooms, it.room, a.prm);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ntab = a.ntab_max;
@@ -1770,7 +1801,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     __shared__ double red[NWB][12];
     const int bid = a.order[blockIdx.x];
     const Item it = a.items[bid];
-    const RoomK R = load_room(a.rooms + it.room);
+    const RoomK R = load_room(a.rooms, it.room, a.prm);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ntab = a.ntab_max, DL = a.adl, GL = a.agl, nJ = a.nJ, J0 = a.J0, J1 = a.J0 + a.nJ - 1;
@@ -2028,12 +2059,13 @@ struct TailAdjArgs {
     unsigned long long seed;
     unsigned call_offset;
     int noise;
+    RoomParams prm;
 };
 
 __global__ __launch_bounds__(WG) void tail_adjoint_kernel(TailAdjArgs a) {
     __shared__ double red[NWAVE];
     const int g = blockIdx.x;
-    const RoomK R = load_room(a.rooms + a.rir_room[g]);
+    const RoomK R = load_room(a.rooms, a.rir_room[g], a.prm);
     const int m = g - R.rir_off;
     const int s = m / R.n_rcv, r = m - s * R.n_rcv;
     const int tid = threadIdx.x;
@@ -2269,7 +2301,7 @@ constexpr int MIN_TILE = 512;
 size_t acc_lds_bytes(int tt, int ntab, const TapTables& T) {
     const size_t nt = (size_t)(ntab + 3) / 4 * 4;
     const size_t acc = ((size_t)tt + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64 + 3) / 4 * 4;
-    return 12 * nt + 8 * acc + (size_t)(QCAP + QPAD) * 20 + 16 + sizeof(float2) * T.lut.size();
+    return 12 * nt + 8 * acc + (size_t)(QCAP + QPAD) * 20 + 32 + sizeof(float2) * T.lut.size();
 }
 
 Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt, bool sort_items) {
@@ -2530,6 +2562,7 @@ struct SimLaunch {
     int acc_len = 0, ntab = 0;
     SimOptions opt;
     unsigned* counters = nullptr;  // persistent launch: device item counter pair (zero between launches)
+    RoomParams prm{nullptr, nullptr};  // betas / levels read at run time (null: the planned values)
 };
 
 int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLaunch& S) {
@@ -2664,6 +2697,7 @@ static AccArgs acc_args(const SimLaunch& S, const void* d_plan, float* d_part, c
     a.bks = S.Bt.ks;
     a.bdl = S.Bt.dl;
     a.bcl = S.Bt.cl;
+    a.prm = S.prm;
     return a;
 }
 
@@ -2836,6 +2870,7 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     f.call_offset = S.opt.call_offset;
     f.noise = S.opt.noise;
     f.crossfade = S.opt.crossfade;
+    f.prm = S.prm;
     hipLaunchKernelGGL(ism_finalize_kernel, dim3((unsigned)S.P.rir_room.size()), dim3(WG), 0, stream, f);
     GPURIR_HIP(hipGetLastError());
     return 0;
@@ -2848,6 +2883,7 @@ int sim_enqueue(const SimLaunch& S, const void* d_plan, float* d_part, const flo
 
 SimLaunch* sim_new() { return new SimLaunch(); }
 void sim_set_counters(SimLaunch& S, unsigned* d_counters) { S.counters = d_counters; }
+void sim_set_params(SimLaunch& S, const float* d_beta, const float* d_level) { S.prm = RoomParams{d_beta, d_level}; }
 const SimOptions& sim_options_of(const SimLaunch& S) { return S.opt; }
 bool sim_is_basis(const SimLaunch& S) { return S.basis; }
 void sim_delete(SimLaunch* s) { delete s; }
@@ -2962,10 +2998,12 @@ struct AdjLaunch {
     int ntab = 0, gp_len = 0, Tw = 0;
     bool mask = false;
     SimOptions opt;
+    RoomParams prm{nullptr, nullptr};  // betas / levels read at run time (null: the planned values)
 };
 
 AdjLaunch* adj_new() { return new AdjLaunch(); }
 void adj_delete(AdjLaunch* a) { delete a; }
+void adj_set_params(AdjLaunch& A, const float* d_beta, const float* d_level) { A.prm = RoomParams{d_beta, d_level}; }
 size_t adj_plan_bytes(const AdjLaunch& A) { return A.plan_bytes; }
 const void* adj_blob(const AdjLaunch& A) { return A.blob.data(); }
 
@@ -3134,6 +3172,7 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
     t.seed = A.opt.seed;
     t.call_offset = A.opt.call_offset;
     t.noise = A.opt.noise;
+    t.prm = A.prm;
     hipLaunchKernelGGL(tail_adjoint_kernel, dim3((unsigned)n_rirs), dim3(WG), 0, stream, t);
     GPURIR_HIP(hipGetLastError());
 
@@ -3166,6 +3205,7 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.agl = A.agl;
         a.aks = A.aks;
         a.acl = A.acl;
+        a.prm = A.prm;
         const void* fn = A.basis  ? reinterpret_cast<const void*>(&ism_adjoint_basis_kernel)
                          : A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
                                   : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);

@@ -5,6 +5,8 @@ current HIP stream, nothing copied to the host.
   image_list         stage (a) per (RIR, image): n0, fraction, amplitude
   simulate_batch     many rooms of uneven sizes in one launch
   BatchPlan          the same, planned once and replayed as a hipGraph
+  DiffPlan           a differentiable batch planned once: parameters in device buffers
+                     read by the captured graph at every replay (optimisation loops)
   simulate_batch_diff / simulate_rir_diff
                      differentiable versions (torch.autograd) with respect to beta,
                      source and receiver positions and the diffuse-tail level
@@ -253,46 +255,116 @@ class BatchPlan:
             self._h = None
 
 
-class _BatchFn(torch.autograd.Function):
-    """out = simulate(beta, pos_src, pos_rcv, level) for a fixed batch description."""
+class _PlanFn(torch.autograd.Function):
+    """out = simulate(beta, pos_src, pos_rcv, level) through a DiffPlan's captured graph."""
 
     @staticmethod
-    def forward(ctx, beta, pos_src, pos_rcv, level, cfg):
-        dev = cfg["dev"]
-        B = cfg["B"]
-        B.beta = np.ascontiguousarray(beta.detach().to("cpu", torch.float32).numpy().reshape(B.R, 6))
-        B.level = np.ascontiguousarray(level.detach().to("cpu", torch.float32).numpy().reshape(B.R))
-        B.d_ps = pos_src.detach().to(device=dev, dtype=torch.float32).contiguous()
-        B.d_pr = pos_rcv.detach().to(device=dev, dtype=torch.float32).contiguous()
-        out = torch.empty(B.total, dtype=torch.float32, device=dev)
-        o = cfg["opt"]
-        _lib.check(_lib.load().gpurir_simulate_batch_device(
-            *B.call_args(), cfg["spkr"], cfg["mic"], cfg["fs"], cfg["c"], ctypes.byref(o), out.data_ptr(),
-            B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _stream(dev)))
-        ctx.cfg = cfg
-        ctx.args = (B.beta.copy(), B.level.copy(), B.d_ps, B.d_pr)
+    def forward(ctx, beta, pos_src, pos_rcv, level, dp, keep):
+        dp._load(beta, pos_src, pos_rcv, level)
+        dp.plan.run()
+        dp._token += 1
+        ctx.dp, ctx.token, ctx.keep = dp, dp._token, keep
         ctx.in_meta = [(t.dtype, t.device) for t in (beta, pos_src, pos_rcv, level)]
-        ctx.save_for_backward(out)
-        return out
+        if keep:
+            # the call's own output and parameters, so that its backward is right whatever
+            # other forward passes run before it
+            out = dp.plan.out.clone()
+            ctx.saved = (dp.beta.clone(), dp.pos_src.clone(), dp.pos_rcv.clone(), dp.level.clone())
+            ctx.save_for_backward(out)
+            return out
+        return dp.plan.out.detach()
 
     @staticmethod
     def backward(ctx, grad_out):
-        (out,) = ctx.saved_tensors
-        cfg = ctx.cfg
-        dev, B = cfg["dev"], cfg["B"]
-        B.beta, B.level, B.d_ps, B.d_pr = ctx.args
-        g = grad_out.detach().to(device=dev, dtype=torch.float32).contiguous()
-        gb = torch.empty((B.R, 6), dtype=torch.float32, device=dev)
-        gs = torch.empty_like(B.d_ps)
-        gr = torch.empty_like(B.d_pr)
-        gl = torch.empty(B.R, dtype=torch.float32, device=dev)
-        o = cfg["opt"]
-        _lib.check(_lib.load().gpurir_simulate_batch_adjoint_device(
-            *B.call_args(), cfg["spkr"], cfg["mic"], cfg["fs"], cfg["c"], ctypes.byref(o), out.data_ptr(),
-            g.data_ptr(), B.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), gb.data_ptr(), gs.data_ptr(),
-            gr.data_ptr(), gl.data_ptr(), _stream(dev)))
+        dp = ctx.dp
+        if ctx.token != dp._token:
+            if not ctx.keep:
+                raise RuntimeError("DiffPlan: another forward pass ran before this backward; its output buffer "
+                                   "now holds that pass (use keep=True to keep each call's output)")
+            (out,) = ctx.saved_tensors
+            for dst, src in zip((dp.beta, dp.pos_src, dp.pos_rcv, dp.level), ctx.saved):
+                dst.copy_(src)
+            dp.plan.out.copy_(out)
+            dp._token = ctx.token
+        gb, gs, gr, gl = dp.plan.adjoint(grad_out)
         res = [t.to(device=dv, dtype=dt) for t, (dt, dv) in zip((gb, gs, gr, gl), ctx.in_meta)]
-        return res[0], res[1], res[2], res[3], None
+        return res[0], res[1], res[2], res[3], None, None
+
+
+def _diff_desc(rooms):
+    """Rooms without their parameters (betas and positions are given as tensors)."""
+    desc = []
+    for r in rooms:
+        ns = int(r["n_src"]) if "n_src" in r else int(np.asarray(r["pos_src"]).reshape(-1, 3).shape[0])
+        nr = int(r["n_rcv"]) if "n_rcv" in r else int(np.asarray(r["pos_rcv"]).reshape(-1, 3).shape[0])
+        d = dict(r)
+        d["beta"] = np.asarray(r["beta"], np.float32) if "beta" in r else np.zeros(6, np.float32)
+        d["pos_src"] = np.asarray(r["pos_src"], np.float32) if "pos_src" in r else np.full((ns, 3), 0.5, np.float32)
+        d["pos_rcv"] = np.asarray(r["pos_rcv"], np.float32) if "pos_rcv" in r else np.full((nr, 3), 0.5, np.float32)
+        desc.append(d)
+    return desc
+
+
+class DiffPlan:
+    """A differentiable batch of rooms planned once.
+
+    The betas [R, 6], diffuse levels [R] and source / receiver positions live in device
+    buffers that the plan's captured hipGraph reads at every replay
+    (gpurir_batch_plan_bind_params), so an optimisation step copies the new parameters
+    into them on the device and replays: no host copy, no synchronisation and no
+    re-planning per step.  The backward pass is the hand-written adjoint of the plan.
+
+        plan = DiffPlan(rooms, fs)            # rooms: room_sz, nb_img, Tmax [, Tdiff,
+                                              # orV_src, orV_rcv, n_src / n_rcv]
+        y = plan(beta, pos_src, pos_rcv, level)   # autograd-aware, flat output
+
+    keep=False (the default of __call__) returns the plan's output buffer itself; its
+    backward must then run before the next forward pass of the same plan (an
+    optimisation loop does exactly that; a violation raises).  keep=True returns a copy
+    and restores the call's own output and parameters if its backward comes later.
+    The forward pass is the exact FP32 simulation."""
+
+    def __init__(self, rooms, fs, spkr_pattern="omni", mic_pattern="omni", c=343.0, noise="logistic", seed=1234,
+                 call_offset=0, room_index=0, chunk_images=0, tile_samples=0, device=None, method="auto",
+                 autotune=False):
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.dev = dev
+        desc = _diff_desc(rooms)
+        self.plan = BatchPlan(desc, fs, spkr_pattern, mic_pattern, c, "fp32", noise, seed, call_offset, room_index,
+                              chunk_images, tile_samples, device=dev, method=method)
+        B = self.plan.B
+        self.R = B.R
+        self.beta = torch.as_tensor(B.beta, device=dev).clone()
+        self.level = torch.as_tensor(B.level, device=dev).clone()
+        self.pos_src = B.d_ps  # the position buffers the plan was created with
+        self.pos_rcv = B.d_pr
+        _lib.check(_lib.load().gpurir_batch_plan_bind_params(self.plan._h, self.beta.data_ptr(),
+                                                             self.level.data_ptr()))
+        self._token = 0
+        self.tune = self.plan.autotune() if autotune else None
+
+    @property
+    def out(self):
+        return self.plan.out
+
+    def split(self, out):
+        return self.plan.B.split(out)
+
+    def _load(self, beta, pos_src, pos_rcv, level):
+        assert tuple(beta.shape) == (self.R, 6) and tuple(level.shape) == (self.R,)
+        assert tuple(pos_src.shape) == tuple(self.pos_src.shape) and tuple(pos_rcv.shape) == tuple(self.pos_rcv.shape)
+        with torch.no_grad():
+            self.beta.copy_(beta.detach())
+            self.pos_src.copy_(pos_src.detach())
+            self.pos_rcv.copy_(pos_rcv.detach())
+            self.level.copy_(level.detach())
+
+    def __call__(self, beta, pos_src, pos_rcv, level, keep=False):
+        return _PlanFn.apply(beta, pos_src, pos_rcv, level, self, keep)
+
+
+_DIFF_PLANS = {}  # key -> DiffPlan, the most recent few (simulate_batch_diff)
+_DIFF_PLANS_MAX = 2
 
 
 def simulate_batch_diff(rooms, beta, pos_src, pos_rcv, level, fs, spkr_pattern="omni", mic_pattern="omni",
@@ -305,26 +377,28 @@ def simulate_batch_diff(rooms, beta, pos_src, pos_rcv, level, fs, spkr_pattern="
     "pos_src" / "pos_rcv").  beta [R,6], pos_src [sum n_src,3], pos_rcv [sum n_rcv,3], level [R] are
     tensors (they may require grad).  The forward pass is the exact FP32 simulation; the backward
     pass is the hand-written adjoint.  method ("auto", "taps" or "basis") selects the
-    accumulation of both passes.  Returns (flat output, per-room views)."""
+    accumulation of both passes.  The batch is planned once per description (a DiffPlan kept for
+    the next calls), so repeated calls with new parameters neither copy them to the host nor
+    re-plan.  Returns (flat output, per-room views); every call's output is its own tensor."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    R = len(rooms)
     desc = []
     for r in rooms:
         ns = int(r["n_src"]) if "n_src" in r else int(np.asarray(r["pos_src"]).reshape(-1, 3).shape[0])
         nr = int(r["n_rcv"]) if "n_rcv" in r else int(np.asarray(r["pos_rcv"]).reshape(-1, 3).shape[0])
-        d = dict(r)
-        d["beta"] = np.zeros(6, np.float32)
-        d["pos_src"] = np.zeros((ns, 3), np.float32)
-        d["pos_rcv"] = np.zeros((nr, 3), np.float32)
-        desc.append(d)
-    B = _BatchArgs(desc, fs, dev)
-    assert tuple(beta.shape) == (R, 6) and tuple(level.shape) == (R,)
-    assert tuple(pos_src.shape) == (int(B.n_src.sum()), 3) and tuple(pos_rcv.shape) == (int(B.n_rcv.sum()), 3)
-    cfg = {"dev": dev, "B": B, "spkr": _pat(spkr_pattern), "mic": _pat(mic_pattern), "fs": float(fs),
-           "c": float(c), "opt": options("fp32", noise, seed, call_offset, room_index, chunk_images=chunk_images,
-                                         tile_samples=tile_samples, method=method)}
-    out = _BatchFn.apply(beta, pos_src, pos_rcv, level, cfg)
-    return out, B.split(out)
+        ov = [None if r.get(k) is None else np.asarray(r[k], np.float32).tobytes() for k in ("orV_src", "orV_rcv")]
+        desc.append((tuple(np.asarray(r["room_sz"], np.float32).tolist()), tuple(int(x) for x in r["nb_img"]),
+                     float(r["Tmax"]), None if r.get("Tdiff") is None else float(r["Tdiff"]), ns, nr, *ov))
+    key = (tuple(desc), float(fs), _pat(spkr_pattern), _pat(mic_pattern), float(c), noise, int(seed),
+           int(call_offset), int(room_index), int(chunk_images), int(tile_samples), str(dev), method)
+    dp = _DIFF_PLANS.pop(key, None)
+    if dp is None:
+        dp = DiffPlan(rooms, fs, spkr_pattern, mic_pattern, c, noise, seed, call_offset, room_index, chunk_images,
+                      tile_samples, dev, method)
+        while len(_DIFF_PLANS) >= _DIFF_PLANS_MAX:
+            _DIFF_PLANS.pop(next(iter(_DIFF_PLANS)))
+    _DIFF_PLANS[key] = dp
+    out = dp(beta, pos_src, pos_rcv, level, keep=True)
+    return out, dp.split(out)
 
 
 def simulate_rir_diff(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, level=None,

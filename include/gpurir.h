@@ -145,6 +145,15 @@ int gpurir_batch_plan_run(gpurir_plan_t plan, void* stream);
 long long gpurir_batch_plan_out_elems(gpurir_plan_t plan);
 long long gpurir_batch_plan_n_rirs(gpurir_plan_t plan);
 int gpurir_batch_plan_destroy(gpurir_plan_t plan);
+/* Reflection coefficients and diffuse levels read from device memory at every run (no
+ * reference counterpart: the reference takes beta per cuda_simulateRIR call,
+ * src/gpuRIR_cuda.cu:766, and has no level).  d_beta [n_rooms][6] and d_level [n_rooms]
+ * float32 (either may be NULL: the values given at creation) must stay allocated while
+ * the plan runs; their contents may change between runs without re-planning, so an
+ * optimiser can update betas, levels and (in the position arrays given at creation)
+ * sources and receivers in place and replay the same graph.  Applies to the plan's runs,
+ * autotuning, profiling and gpurir_batch_plan_adjoint.  Invalidates the captured graph. */
+int gpurir_batch_plan_bind_params(gpurir_plan_t plan, const float* d_beta, const float* d_level);
 /* Autotuner (no reference counterpart): times the plan's launch eagerly on the stream
  * for candidate work-item shapes -- ISM samples per LDS tile, images per work item --
  * and for the persistent launch form, keeps the fastest and re-plans with it (the next

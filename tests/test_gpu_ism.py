@@ -218,8 +218,43 @@ def test_runs_agree():
     b = T.simulate_rir(cfg["room_sz"], cfg["beta"], cfg["pos_src"], cfg["pos_rcv"], cfg["nb_img"], cfg["Tmax"],
                        cfg["fs"], Tdiff=cfg["Tdiff"], spkr_pattern=cfg["spkr_pattern"], mic_pattern=cfg["mic_pattern"],
                        orV_src=cfg["orV_src"], orV_rcv=cfg["orV_rcv"]).cpu().numpy()
-    # LDS accumulation order may differ between runs: agreement to FP32 rounding
-    assert rel_l2(a, b) < 1e-6
+    # taps are summed as integers (order-independent) and partial rows in a fixed order: same bits
+    assert np.array_equal(a, b.astype(np.float32))
+    c = T.simulate_rir(cfg["room_sz"], cfg["beta"], cfg["pos_src"], cfg["pos_rcv"], cfg["nb_img"], cfg["Tmax"],
+                       cfg["fs"], Tdiff=cfg["Tdiff"], spkr_pattern=cfg["spkr_pattern"], mic_pattern=cfg["mic_pattern"],
+                       orV_src=cfg["orV_src"], orV_rcv=cfg["orV_rcv"], method="basis").cpu().numpy()
+    d = T.simulate_rir(cfg["room_sz"], cfg["beta"], cfg["pos_src"], cfg["pos_rcv"], cfg["nb_img"], cfg["Tmax"],
+                       cfg["fs"], Tdiff=cfg["Tdiff"], spkr_pattern=cfg["spkr_pattern"], mic_pattern=cfg["mic_pattern"],
+                       orV_src=cfg["orV_src"], orV_rcv=cfg["orV_rcv"], method="basis").cpu().numpy()
+    assert np.array_equal(c, d)
+
+
+@pytest.mark.parametrize("lut", [False, True])
+def test_drop_in_and_plan_runs_are_bit_identical(lut):
+    """The numpy drop-in with the same noise stream position, and repeated replays of a plan,
+    give the same bits."""
+    cfg = W.plain_room()
+    kw = dict(room_sz=cfg["room_sz"], beta=cfg["beta"], pos_src=cfg["pos_src"], pos_rcv=cfg["pos_rcv"],
+              nb_img=cfg["nb_img"], Tmax=cfg["Tmax"], fs=cfg["fs"], Tdiff=cfg["Tdiff"])
+    gpuRIR.activateLUT(lut)
+    try:
+        gpuRIR.set_seed(1234, 5)
+        a = gpuRIR.simulateRIR(**kw)
+        gpuRIR.set_seed(1234, 5)
+        b = gpuRIR.simulateRIR(**kw)
+    finally:
+        gpuRIR.activateLUT(True)
+    assert np.array_equal(a, b)
+    rooms = W.uneven_rooms(n_rooms=16, seed=3)
+    plan = T.BatchPlan(rooms, W.FS, seed=7, precision="lut" if lut else "fp32")
+    plan.run()
+    torch.cuda.synchronize()
+    first = plan.out.clone()
+    for _ in range(2):
+        plan.out.zero_()
+        plan.run()
+        torch.cuda.synchronize()
+        assert torch.equal(plan.out, first)
 
 
 def test_batch_equals_single_rooms_and_graph_replay():
@@ -438,3 +473,38 @@ def test_basis_tails_independent_of_item_split():
             assert np.allclose(np.nan_to_num(gb), np.nan_to_num(ob), atol=1e-6)
         else:
             check_parity(gb, ob, orc.n_samples(r["Tdiff"], W.FS))
+
+
+@pytest.mark.parametrize("precision,method,tol", [("fp32", "taps", 1e-5), ("fp32", "basis", 1e-5),
+                                                   ("fp16", "taps", 1e-3), ("fp16", "basis", 1e-3)])
+def test_source_outside_the_room(precision, method, tol):
+    """A source 5 cm beyond a wall and a receiver 2 cm beyond another: images nearer than the
+    direct path.  The fixed-point scale is bounded by the nearest image, so no tap is clipped
+    (FP32, 64-bit sums) or wraps (mixed precision, 32-bit sums)."""
+    L = [4.0, 3.5, 2.7]
+    ps = np.array([[4.05, 1.2, 1.1]], np.float32)
+    pr = np.array([[2.5, 3.52, 1.3], [1.0, 1.0, 1.0]], np.float32)
+    beta = np.full(6, 0.9, np.float32)
+    nb = gpuRIR.t2n(0.1, L)
+    kw = dict(room_sz=L, beta=beta, pos_src=ps, pos_rcv=pr, nb_img=nb, Tmax=0.14, fs=16000.0, Tdiff=0.1)
+    g = T.simulate_rir(**kw, precision=precision, method=method).cpu().numpy()
+    o = orc.simulate_rir(**kw)
+    check_parity(g, o, orc.n_samples(0.1, 16000.0), tol=tol)
+
+
+@pytest.mark.parametrize("method", ["taps", "basis"])
+def test_delay_fraction_rounding_up_to_one(method):
+    """Source (1.5949384, 2.3308206, 1) and receiver (1, 1, 1): the direct path's delay is
+    67.99999997 samples, whose fraction (0.99999997) rounds to 1.0 in float32.  The tap at
+    j = 1 is then sin(pi (1 - f)) / (pi (1 - f)) with both factors 0 unless the fraction is
+    kept below 1; it must come out as the oracle's (sinc ~ 1), and so must the tail, whose
+    envelope is fitted on the early part."""
+    L = [4.0, 3.5, 2.7]
+    ps = np.array([[1.5949384, 2.3308206, 1.0]], np.float32)
+    pr = np.array([[1.0, 1.0, 1.0]], np.float32)
+    n0, fr, _, _ = orc.images(L, np.full(6, 0.8), ps, pr, [3, 3, 3], 16000.0)
+    assert n0[0, 13] == 67 and np.float32(fr[0, 13]) == np.float32(1.0)  # image 13 is the direct path
+    kw = dict(room_sz=L, beta=np.full(6, 0.8, np.float32), pos_src=ps, pos_rcv=pr, nb_img=gpuRIR.t2n(0.05, L),
+              Tmax=0.08, fs=16000.0, Tdiff=0.05)
+    g = T.simulate_rir(**kw, method=method).cpu().numpy()
+    check_parity(g, orc.simulate_rir(**kw), orc.n_samples(0.05, 16000.0))

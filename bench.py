@@ -20,6 +20,7 @@ sys.path.insert(0, ROOT)
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (packed); non-packed issue caps at half of it
 FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix
 FLOP_PER_TAP = 12               # sub, x2pi, /Tw, cos, +1, x0.5, xpi, sin, /, xamp, xsinc, +acc
+FLOP_PER_TAP_BWD = 24           # value and derivative of the windowed sinc, both accumulated
 BASIS_TILE = 512                # samples per tile of the basis kernel
 BASIS_MFMA_PER_TILE = 576       # v_mfma_f32_16x16x4_f32 per tile at Tw = 128: 8 rows x 36 K steps x 2 blocks
 MFMA_16X16X4_FLOP = 2048
@@ -47,6 +48,19 @@ def read_traffic(name):
     return None
 
 
+def read_counters(name):
+    """Executed-work utilisation lines (key: value) of a committed rocprofv3 counter summary."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    res = {}
+    for line in open(p):
+        for key in ("valu_issue_frac", "mfma_busy_frac", "lds_bank_conflict_frac"):
+            if line.startswith(key + ":"):
+                res[key] = float(line.split(":", 1)[1])
+    return res or None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,7 +72,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rooms", type=int, default=4096)
-    ap.add_argument("--cpu-sample-rooms", type=int, default=400)
+    ap.add_argument("--cpu-sample-rooms", type=int, default=128)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-batch", action="store_true", help="time the batched workload only (for profiler runs)")
     ap.add_argument("--only-gradients", action="store_true",
@@ -92,7 +106,7 @@ def main():
         log(f"[bench] reference copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
         del src, dst
 
-    def measure_trajectory():
+    def measure_trajectory(check=False):
         # ---------------- trajectory: 128 points x 8 mics, full ISM 0.5 s, 10 s of noise
         c5 = W.trajectory()
         rirs5 = T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs)
@@ -130,51 +144,138 @@ def main():
         ms_traj_rirs = e0.elapsed_time(e1) / 10
         log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms; convolution+OLA {ms_traj:.3f} ms on the device "
             f"({traj_gbs:.0f} GB/s of {traj_bytes / 1e6:.1f} MB algorithmic), {ms_traj_host:.3f} ms per call on the host")
+        res = {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj, "conv_ola_host_ms": ms_traj_host,
+               "conv_ola_gbs": traj_gbs, "conv_ola_bytes": traj_bytes, "conv_ola_frac_hbm": traj_gbs / 8000.0}
+        if check:
+            # accuracy against the FP64 oracle: the RIRs of two trajectory points, and the whole
+            # convolution + float64 overlap-add applied to the GPU's own RIRs; the oracle's overlap-add
+            # (numpy FFT in FP64, one thread) is also the CPU baseline of stage (d)
+            from oracle import oracle as orc
+            g_rirs = rirs5.cpu().numpy()
+            pts = [0, P5 - 1]
+            o = orc.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"][pts], c5["pos_rcv"], c5["nb_img"],
+                                 c5["Tmax"], fs)
+            res["rirs_rel_l2"] = rel_l2(g_rirs[pts], o)
+            t0 = time.perf_counter()
+            want = orc.simulate_trajectory(c5["signal"], g_rirs)
+            res["cpu_conv_ola_ms"] = (time.perf_counter() - t0) * 1e3
+            res["cpu_conv_ola"] = "FP64 numpy FFT overlap-add (the reference's host loop), 1 thread"
+            res["conv_ola_rel_l2"] = rel_l2(y5.cpu().numpy(), want)
+            log(f"[bench] trajectory accuracy: RIRs {res['rirs_rel_l2']:.2e}, convolution+OLA "
+                f"{res['conv_ola_rel_l2']:.2e}; CPU FP64 overlap-add {res['cpu_conv_ola_ms']:.0f} ms")
         del y5
-        return ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes
+        return res
 
-    def measure_gradients():
+    def measure_gradients(fd_check=True):
         # ---------------- gradients: 256 rooms, MSE against targets simulated from perturbed
-        # betas and source positions; one step = forward (graph) + dL/dout + adjoint kernels
+        # betas and source positions.  A step is what an optimiser runs: the parameters are
+        # copied into the plan's device buffers, the captured forward graph replays, the MSE and
+        # dL/dout come from torch autograd, the adjoint kernels run and SGD updates the betas,
+        # the sources and the levels on the device.  Nothing goes to the host.
         c4 = W.gradient_rooms(256, seed=0)
         target = T.BatchPlan(W.gradient_targets(c4, seed=1), fs, seed=1234).run().clone()
-        gplan = T.BatchPlan(c4, fs, seed=1234)
-        gtune = gplan.autotune() if not args.no_autotune else None
+        dp = T.DiffPlan(c4, fs, seed=1234, autotune=not args.no_autotune)
+        beta0 = torch.tensor(np.stack([r["beta"] for r in c4]), device=dev)
+        src0 = torch.tensor(np.concatenate([r["pos_src"] for r in c4]), device=dev)
+        rcv0 = torch.tensor(np.concatenate([r["pos_rcv"] for r in c4]), device=dev)
+        beta = beta0.clone().requires_grad_(True)
+        src = src0.clone().requires_grad_(True)
+        rcv = rcv0.clone().requires_grad_(True)
+        level = torch.ones(len(c4), device=dev, requires_grad=True)
+        opt = torch.optim.SGD([beta, src, level], lr=2e4)
+        lo = torch.tensor([0.1, 0.1, 0.1], device=dev)
+        hi = torch.as_tensor(np.stack([np.asarray(r["room_sz"], np.float32) for r in c4]), device=dev) - 0.1
 
-        def grad_step():
-            y = gplan.run()
-            return gplan.adjoint((2.0 / y.numel()) * (y - target))
+        def sgd_step():
+            opt.zero_grad(set_to_none=True)
+            rcv.grad = None
+            y = dp(beta, src, rcv, level)
+            loss = ((y - target) ** 2).mean()
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                beta.clamp_(0.0, 1.0)
+                src.copy_(torch.minimum(torch.maximum(src, lo), hi))
+            return loss
 
-        for _ in range(max(args.warmup, 1)):
-            grads = grad_step()
+        # FD agreement at the initial parameters, through the same DiffPlan
+        y = dp(beta, src, rcv, level)
+        ((y - target) ** 2).mean().backward()
         torch.cuda.synchronize()
-        n_grad = max(args.steps // 2, 3)
+        fd = None
+        if fd_check:
+            from oracle import grad_check as gc
+            t_host = target.cpu().numpy()
+            n_img = [int(np.prod(r["nb_img"])) for r in c4]
+            n_ism = [T.n_samples(r["Tdiff"], fs) for r in c4]
+            ks = list(dict.fromkeys([0, int(np.argmax(n_img)), int(np.argmax(n_ism)), int(np.argmin(n_img)),
+                                     len(c4) - 1]))
+            per = T.n_samples(0.5, fs) * 4
+            worst = {}
+            for k in ks:
+                r = c4[k]
+                want = gc.room_mse_grads(r, t_host[k * per:(k + 1) * per].reshape(1, 4, -1), 1.0 / y.numel(), fs,
+                                         seed=1234, room_index=k)
+                got = {"beta": beta.grad[k].double().cpu().numpy(), "src": src.grad[k:k + 1].double().cpu().numpy(),
+                       "rcv": rcv.grad[4 * k:4 * k + 4].double().cpu().numpy(), "level": float(level.grad[k])}
+                for g, e in gc.group_errors(got, want).items():
+                    worst[g] = max(worst.get(g, 0.0), e)
+            fd = {"rooms": ks, "rel_l2_max_per_group": worst, "fd_rel_l2_max": max(worst.values()),
+                  "h": 1e-6, "reference": "FP64 central differences of the C oracle"}
+            log(f"[bench] gradients vs FP64 finite differences on rooms {ks}: max rel-L2 {fd['fd_rel_l2_max']:.2e}")
+        for _ in range(max(args.warmup, 2)):
+            sgd_step()
+        torch.cuda.synchronize()
+        n_grad = max(args.steps, 5)
+        loss0 = float(((dp(beta.detach(), src.detach(), rcv.detach(), level.detach()) - target) ** 2).mean())
         t0 = time.perf_counter()
         for _ in range(n_grad):
-            grads = grad_step()
+            loss = sgd_step()
         torch.cuda.synchronize()
         ms_grad = (time.perf_counter() - t0) * 1e3 / n_grad
+        loss1 = float(loss.detach())
+        grads_per_s = len(c4) / (ms_grad / 1e3)
+        # replay of the forward graph alone, and device time of the adjoint kernels (events on
+        # the current stream, which the adjoint runs on)
+        gplan = dp.plan
         t0 = time.perf_counter()
         for _ in range(n_grad):
             gplan.run()
         torch.cuda.synchronize()
-        ms_fwd4 = (time.perf_counter() - t0) * 1e3 / n_grad
-        grads_per_s = len(c4) / (ms_grad / 1e3)
-        gfinite = all(bool(torch.isfinite(gk).all()) for gk in grads)
-        log(f"[bench] gradients: {ms_grad:.3f} ms per fwd+bwd step of {len(c4)} rooms ({grads_per_s:.4e} rooms/s), "
-            f"forward alone {ms_fwd4:.3f} ms, finite={gfinite}")
-        return ms_grad, ms_fwd4, grads_per_s, gtune
+        ms_fwd = (time.perf_counter() - t0) * 1e3 / n_grad
+        gy = (2.0 / gplan.out.numel()) * (gplan.out - target)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        gplan.adjoint(gy)
+        e0.record()
+        for _ in range(n_grad):
+            gplan.adjoint(gy)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_adj = e0.elapsed_time(e1) / n_grad
+        ms_acc_fwd = float(np.mean([gplan.profile()[0] for _ in range(3)]))
+        taps4 = gplan.count_taps()
+        bwd_tflops = FLOP_PER_TAP_BWD * taps4 / (ms_adj / 1e3) / 1e12
+        log(f"[bench] gradients: {ms_grad:.3f} ms per SGD step of {len(c4)} rooms ({grads_per_s:.4e} rooms/s; "
+            f"loss {loss0:.3e} -> {loss1:.3e} over {n_grad} steps), forward replay {ms_fwd:.3f} ms, adjoint kernels "
+            f"{ms_adj:.3f} ms ({bwd_tflops:.1f} TFLOP/s of 24 flop x {taps4:.3e} taps)")
+        return {"grads_per_s": grads_per_s, "ms_per_step": ms_grad, "ms_fwd": ms_fwd, "ms_adjoint": ms_adj,
+                "ms_forward_accumulate": ms_acc_fwd, "taps": taps4, "loss_first": loss0, "loss_last": loss1,
+                "fd": fd, "forward_autotune": dp.tune,
+                "roofline": {"bound": "valu", "achieved": bwd_tflops, "peak": FP32_VALU_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": bwd_tflops / FP32_VALU_PEAK_TFLOPS,
+                             "kernel": "adjoint kernels (tail_adjoint, ism_adjoint_basis, grad_rir, grad_param)",
+                             "work": f"{taps4} in-window taps x {FLOP_PER_TAP_BWD} flop (value and derivative)"}}
 
     if args.only_gradients:
-        ms_grad, ms_fwd4, grads_per_s, gtune = measure_gradients()
-        print(json.dumps({"metric": "fwd+bwd grads/s (profiler run)", "value": grads_per_s, "unit": "rooms/s",
-                          "ms_per_fwd_bwd": ms_grad, "ms_fwd": ms_fwd4}), flush=True)
+        G = measure_gradients(fd_check=False)
+        print(json.dumps({"metric": "fwd+bwd grads/s (profiler run)", "value": G["grads_per_s"], "unit": "rooms/s",
+                          "ms_per_step": G["ms_per_step"], "ms_fwd": G["ms_fwd"], "ms_adjoint": G["ms_adjoint"]}),
+              flush=True)
         return
 
     if args.only_trajectory:
-        ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes = measure_trajectory()
-        print(json.dumps({"metric": "trajectory (profiler run)", "rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj,
-                          "conv_ola_host_ms": ms_traj_host, "conv_ola_gbs": traj_gbs}), flush=True)
+        tr = measure_trajectory()
+        print(json.dumps(dict({"metric": "trajectory (profiler run)"}, **tr)), flush=True)
         return
 
     # ---------------- batched: 4096 uneven rooms, one graph replay per step
@@ -239,7 +340,7 @@ def main():
                           "ms_per_step": ms_step, "kernel_ms": ms_acc, "taps": taps}), flush=True)
         return
 
-    ms_grad, ms_fwd4, grads_per_s, gtune = measure_gradients()
+    G = measure_gradients()
 
     # ---------------- plain drop-in: one room, numpy in / numpy out, FP32 exact sinc
     c1 = W.plain_room()
@@ -281,25 +382,64 @@ def main():
     err_mid = rel_l2(out2.cpu().numpy(), orc.simulate_rir(**kw2))
     log(f"[bench] mid room: {ms_mid:.3f} ms/call ({128 / ms_mid * 1e3:.4e} RIRs/s), rel-L2 {err_mid:.3e}")
 
-    ms_traj_rirs, ms_traj, ms_traj_host, traj_gbs, traj_bytes = measure_trajectory()
+    traj = measure_trajectory(check=True)
+
+    # ---------------- the reference's own float32 arithmetic against FP64 (gather order of
+    # generateRIR_kernel), early part: the plain room, and 2 x 4 RIRs of the mid room
+    fp32_ref = {}
+    for name, kwr, sel in (("plain_room", kw, None), ("mid_room_2x4", kw2, (slice(0, 2), slice(0, 4)))):
+        k2 = dict(kwr)
+        if sel is not None:
+            k2["pos_src"], k2["pos_rcv"] = k2["pos_src"][sel[0]], k2["pos_rcv"][sel[1]]
+            k2["orV_src"], k2["orV_rcv"] = k2["orV_src"][sel[0]], k2["orV_rcv"][sel[1]]
+        n_ism = T.n_samples(k2["Tdiff"], fs)
+        ref32 = orc.simulate_rir(**k2, mode=orc.FP32_REF)
+        ref64 = orc.simulate_rir(**k2)
+        fp32_ref[name] = rel_l2(ref32[..., :n_ism], ref64[..., :n_ism])
+    log(f"[bench] the reference's float32 arithmetic against FP64 (early part): {fp32_ref}")
 
     # ---------------- CPU baseline: the FP64 oracle on a bounded sample of the same batch
     cpu = None
     if not args.skip_cpu:
         nthreads = orc.max_threads()
+        model = "unknown CPU"
+        try:
+            for ln in open("/proc/cpuinfo"):
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+
+        def cpu_run(sample):
+            t0 = time.perf_counter()
+            for k, r in enumerate(sample):
+                orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
+                                 Tdiff=r["Tdiff"], room_index=k)
+            return time.perf_counter() - t0
+
         sample = rooms[:args.cpu_sample_rooms]
         n_s = sum(len(r["pos_rcv"]) for r in sample)
-        t0 = time.perf_counter()
-        for k, r in enumerate(sample):
-            orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
-                             Tdiff=r["Tdiff"], room_index=k)
-        dt = time.perf_counter() - t0
+        cpu_run(sample[:8])  # warm-up
+        runs = sorted(cpu_run(sample) for _ in range(3))
+        dt = runs[1]
+        one = rooms[:max(args.cpu_sample_rooms // 8, 1)]
+        n_one = sum(len(r["pos_rcv"]) for r in one)
+        orc.set_threads(1)
+        dt1 = cpu_run(one)
+        orc.set_threads(nthreads)
         cpu = {"value": n_s / dt, "unit": "RIRs/s", "cores": nthreads, "kind": "port",
-               "sample": f"first {len(sample)} rooms of the 4096-room batch ({n_s} RIRs), FP64 oracle, "
-                         f"windowed scatter, {nthreads} OpenMP threads, {dt:.1f} s"}
-        log(f"[bench] CPU oracle: {cpu['value']:.3e} RIRs/s on {nthreads} threads")
+               "sample": f"first {len(sample)} rooms of the 4096-room batch ({n_s} RIRs), FP64 oracle, windowed "
+                         f"scatter, {nthreads} OpenMP threads on {model}: median of 3 runs after a warm-up "
+                         f"({', '.join(f'{x:.2f}' for x in runs)} s)",
+               "one_thread": {"value": n_one / dt1, "unit": "RIRs/s", "rooms": len(one), "rirs": n_one,
+                              "seconds": dt1},
+               "cpu_model": model}
+        log(f"[bench] CPU oracle: {cpu['value']:.3e} RIRs/s on {nthreads} threads ({model}); "
+            f"1 thread {n_one / dt1:.3e} RIRs/s")
 
     traffic = read_traffic("traffic_ism_basis_uneven_rooms.txt")
+    counters = read_counters("counters_ism_basis_uneven_rooms.txt")
     # work the basis kernel executes: its matrix-core filters per 512-sample tile
     tiles = sum(len(r["pos_src"]) * len(r["pos_rcv"]) * -(-T.n_samples(r["Tdiff"], fs) // BASIS_TILE)
                 for r in rooms)
@@ -327,22 +467,25 @@ def main():
                      "work": f"{taps} in-window taps x {FLOP_PER_TAP} flop (the tap formula's work; the basis "
                              "method executes 8 fixed-point sums per image and matrix-core filters instead)",
                      "executed": {"tiles": tiles, "mfma_tflops": mfma_tflops,
-                                  "mfma_frac": mfma_tflops / FP32_MFMA_PEAK_TFLOPS}},
+                                  "mfma_frac": mfma_tflops / FP32_MFMA_PEAK_TFLOPS,
+                                  "counters": counters}},
         "cpu_baseline": cpu,
-        "accuracy": {"batch_rel_l2_max_3_rooms": max(errs), "plain_rel_l2": err_plain, "mid_room_rel_l2": err_mid},
+        "accuracy": {"batch_rel_l2_max_3_rooms": max(errs), "plain_rel_l2": err_plain, "mid_room_rel_l2": err_mid,
+                     "trajectory_rirs_rel_l2": traj.get("rirs_rel_l2"),
+                     "trajectory_conv_ola_rel_l2": traj.get("conv_ola_rel_l2"),
+                     "gradients_fd_rel_l2_max": (G["fd"] or {}).get("fd_rel_l2_max"),
+                     "reference_fp32_arithmetic_rel_l2": fp32_ref},
         "plain_rirs_per_s": plain_rirs_s,
         "batched_over_plain": rirs_per_s / plain_rirs_s,
         "mid_room_ms": ms_mid,
         "finalize_ms": ms_fin,
-        "trajectory": {"rirs_ms": ms_traj_rirs, "conv_ola_ms": ms_traj, "conv_ola_host_ms": ms_traj_host,
-                       "conv_ola_gbs": traj_gbs, "conv_ola_bytes": traj_bytes,
-                       "conv_ola_frac_hbm": traj_gbs / 8000.0},
+        "trajectory": traj,
         "batched_modes": batch,
-        "gradients": {"workload": "256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 src x 4 rcv, Tmax 0.5 s, "
-                                  "Tdiff at 20 dB), MSE to targets from perturbed betas and sources; "
-                                  "d/d(6 betas, source xyz, receiver xyz, level)",
-                      "grads_per_s": grads_per_s, "ms_per_fwd_bwd": ms_grad, "ms_fwd": ms_fwd4,
-                      "forward_autotune": gtune},
+        "gradients": dict({"workload": "256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 src x 4 rcv, Tmax "
+                                       "0.5 s, Tdiff at 20 dB), MSE to targets from perturbed betas and sources; "
+                                       "d/d(6 betas, source xyz, receiver xyz, level); a step is one SGD update of "
+                                       "the betas, sources and levels through DiffPlan (device-resident "
+                                       "parameters, captured forward graph, hand-written adjoint)"}, **G),
     }
     print(json.dumps(line), flush=True)
 

@@ -45,6 +45,12 @@ _SIG = {
                                                   ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_float,
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(Options), ctypes.c_void_p, ctypes.c_void_p]),
+    "gpurir_simulate_rir_device_noalloc": (ctypes.c_int, [c_float_p, c_float_p, ctypes.c_void_p, ctypes.c_int,
+                                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_int_p,
+                                                          ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                                          ctypes.c_float, ctypes.POINTER(Options), ctypes.c_void_p,
+                                                          ctypes.c_void_p]),
     "gpurir_image_list_device": (ctypes.c_int, [c_float_p, c_float_p, ctypes.c_void_p, ctypes.c_int,
                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_float, ctypes.c_float,

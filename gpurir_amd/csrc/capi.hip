@@ -401,6 +401,22 @@ int gpurir_simulate_rir_device(const float room_sz[3], const float beta[6], cons
                              static_cast<hipStream_t>(stream));
 }
 
+int gpurir_simulate_rir_device_noalloc(const float room_sz[3], const float beta[6], const float* d_pos_src,
+                                       int M_src, const float* d_pos_rcv, int M_rcv, const float* d_orV_src,
+                                       const float* d_orV_rcv, int spkr_pattern, int mic_pattern, const int nb_img[3],
+                                       float Tdiff, float Tmax, float fs, float c, const gpurir_options_t* opt,
+                                       float* d_out, void* stream) {
+    if (int rc = check_common(M_src, M_rcv, spkr_pattern, mic_pattern, nb_img)) return rc;
+    GPURIR_CHECK(room_sz && beta && d_pos_src && d_pos_rcv && d_out, "null pointer argument");
+    GPURIR_CHECK(spkr_pattern == 0 || d_orV_src, "the sources are not omni but their orientation is undefined");
+    GPURIR_CHECK(mic_pattern == 0 || d_orV_rcv, "the mics are not omni but their orientation is undefined");
+    std::vector<RoomHost> rooms(1, room_from(room_sz, beta, nb_img, M_src, M_rcv, Tdiff, Tmax,
+                                             opt ? opt->room_index : 0u, opt ? opt->level : 1.0f));
+    SimOptions so = sim_options(opt, spkr_pattern, mic_pattern, fs, c);
+    return launch_simulation_noalloc(rooms, d_pos_src, d_pos_rcv, d_orV_src, d_orV_rcv, so, d_out,
+                                     static_cast<hipStream_t>(stream));
+}
+
 int gpurir_simulate_rir(const float room_sz[3], const float beta[6], const float* pos_src, int M_src,
                         const float* pos_rcv, int M_rcv, const float* orV_src, const float* orV_rcv,
                         int spkr_pattern, int mic_pattern, const int nb_img[3], float Tdiff, float Tmax,

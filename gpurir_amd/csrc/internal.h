@@ -155,6 +155,12 @@ int launch_image_list(const RoomHost& room, const float* d_pos_src, const float*
 int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
                       const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, float* d_out,
                       hipStream_t stream);
+// The same without allocating device or pinned memory: plan, partial rows and work counters
+// in a static per-GPU scratch array of NOALLOC_SCRATCH_BYTES (calls on one GPU share it).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+int launch_simulation_noalloc(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
+                              const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, float* d_out,
+                              hipStream_t stream);
 // The same with the source/receiver positions and orientations in host memory ([n][3] each;
 // orientations may be null): they travel in the plan's upload, one host-to-device copy.
 int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_src, int n_src, const float* pos_rcv,

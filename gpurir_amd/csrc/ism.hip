@@ -2949,6 +2949,75 @@ int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_
                        reinterpret_cast<const float*>(d + o_os), reinterpret_cast<const float*>(d + o_or), d_out, stream);
 }
 
+// ---- the no-allocation launch (gpurir_simulate_rir_device_noalloc)
+// Its plan, partial rows and work counters live in `scratch`, one array per GPU, held for
+// the life of the process.  On the host the array's name is not a device address: each
+// device's address comes from hipGetSymbolAddress (noalloc_scratch) and only that address
+// is handed to copies and kernels.  Calls on one GPU share the array; calls that overlap
+// on one GPU race.
+namespace {
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEV] = {};
+
+// The current device's copy of `scratch`, resolved once per device with that device current.
+int noalloc_scratch(void** ptr) {
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    GPURIR_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEV, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            set_error(std::string("hipGetSymbolAddress of the no-allocation scratch failed: ") +
+                      hipGetErrorString(e));
+            return -2;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Fills n words with zero (a kernel rather than a memset node, so that captured graphs
+// replay it correctly).
+__global__ void zero_words_kernel(unsigned* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+}  // namespace
+
+int launch_simulation_noalloc(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
+                              const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, float* d_out,
+                              hipStream_t stream) {
+    SimLaunch S;
+    if (int rc = sim_prepare(rooms, opt, S)) return rc;
+    // scratch: counters [0, 256) | plan | partial rows, 256-byte aligned sections
+    const size_t o_plan = 256, o_part = align_up(o_plan + S.plan_bytes, 256);
+    const size_t need = o_part + align_up(sim_partial_bytes(S), 256);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error("this launch needs " + std::to_string(need) + " bytes of device scratch (plan " +
+                  std::to_string(S.plan_bytes) + ", partial rows " + std::to_string(sim_partial_bytes(S)) +
+                  "); the no-allocation entry point has " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                  " (use gpurir_simulate_rir_device, or fewer receivers per call)");
+        return -1;
+    }
+    void* base = nullptr;
+    if (int rc = noalloc_scratch(&base)) return rc;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    unsigned* counters = reinterpret_cast<unsigned*>(b);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counters, 2);
+    GPURIR_HIP(hipGetLastError());
+    // the plan goes from pageable host memory: the runtime stages it, no pinned buffer is allocated
+    GPURIR_HIP(hipMemcpyAsync(b + o_plan, S.blob.data(), S.plan_bytes, hipMemcpyHostToDevice, stream));
+    float* dpart = S.P.part_elems > 0 ? reinterpret_cast<float*>(b + o_part) : nullptr;
+    sim_set_counters(S, S.basis ? nullptr : counters);
+    return sim_enqueue(S, b + o_plan, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
+}
+
 int launch_image_list(const RoomHost& room, const float* d_pos_src, const float* d_pos_rcv, const float* d_orv_src,
                       const float* d_orv_rcv, const SimOptions& opt, int* d_n0, float* d_frac, float* d_amp,
                       hipStream_t stream) {

@@ -108,6 +108,24 @@ int gpurir_simulate_rir_device(const float room_sz[3], const float beta[6], cons
                                int spkr_pattern, int mic_pattern, const int nb_img[3], float Tdiff, float Tmax,
                                float fs, float c, const gpurir_options_t* opt, float* d_out, void* stream);
 
+/* gpurir_simulate_rir_device without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results; it calls no hipMalloc,
+ * hipFree or pinned-memory allocation: the launch's plan, partial rows and work counters live
+ * in a static scratch array of GPURIR_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+ * ends), and the plan is copied from pageable host memory.  A launch that needs more scratch
+ * is rejected with status -1 and a message.  Partial rows, the largest part, take 4 bytes per
+ * early sample and image chunk of every RIR whose images the tap method splits over several
+ * work items (about 2048 items per launch by default): early parts of up to 8000 samples fit,
+ * the 1024 RIRs of 8000 samples and 246,906 images of a full trajectory do not.  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (different streams, or threads) race and must be ordered by the caller. */
+#define GPURIR_NOALLOC_SCRATCH_BYTES (64 << 20)
+int gpurir_simulate_rir_device_noalloc(const float room_sz[3], const float beta[6], const float* d_pos_src,
+                                       int M_src, const float* d_pos_rcv, int M_rcv, const float* d_orV_src,
+                                       const float* d_orV_rcv, int spkr_pattern, int mic_pattern,
+                                       const int nb_img[3], float Tdiff, float Tmax, float fs, float c,
+                                       const gpurir_options_t* opt, float* d_out, void* stream);
+
 /* Stage (a) listing: per (RIR, image) the integer sample n0 = floor(tau), the fraction
  * tau - n0 and the signed amplitude; RIR m = src*M_rcv + rcv, image i = (nx*Ny + ny)*Nz + nz.
  * Replaces calcAmpTau_kernel's amp/tau outputs (src/gpuRIR_cuda.cu:245-375). Device arrays. */

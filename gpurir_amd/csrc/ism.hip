@@ -722,6 +722,16 @@ static_assert(NWB == 4 * NOT, "four basis quarters per output block");
 constexpr int BASIS_NB = 8; // Chebyshev terms (FP32: within 5e-7 of each tap)
 constexpr int BASIS_NB_FP16 = 6;  // mixed precision: 6 terms, within 6e-5 (1e-4 relative L2)
 constexpr int BCS = 2;      // columns per thread of the in-place conversion: DL <= BCS * WGB
+// Lookup-table mode on the same machinery: the table's taps are linear in the weight of
+// the 1/16-sample grid, so an image at 16 tau = 16 n0 + p + a (phase p < 16, 0 <= a < 1)
+// adds amp (1 - a) to phase row p and amp a to row p + 1 (row 0 of column n0 + 1 for
+// p = 15) at column n0, and the tile is 16 Toeplitz filters with the table's entries
+// T[16 j - p + center] (src/gpuRIR_cuda.cu:236-239, 578-616).  The rows hold 32-bit
+// fixed-point sums (top bit 2^24 of the largest image amplitude: 128 images of it per
+// word before the sums could wrap), so 16 rows take the LDS of 8 rows of 64-bit sums.
+constexpr int LUT_NB = 16;
+constexpr int QB_LUT = 1536;  // queue entries per round in the table mode (its rows and
+                              // coefficients take more LDS)
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
@@ -801,9 +811,10 @@ __device__ inline void axis_range(const AxisRuns& A, int q, double u, double v, 
 // eight atomics are then immediates), or 0 for the launch's a.bdl.  H16 (mixed precision):
 // the filters take FP16 inputs (D scaled per tile by a power of two, the coefficients
 // rounded) with FP32 accumulation, v_mfma_f32_16x16x16_f16.
-template <int NB, int DLC, bool H16>
+template <int NB, int DLC, bool H16, bool LUT>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
-    static_assert(NB >= 4 && NB <= 8, "basis rows mq, mq + 4 of wave quarter mq");
+    static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
+    constexpr int QBK = LUT ? QB_LUT : QB;
     const Item it = a.items[item];
     const RoomK R = load_room(a.rooms, it.room, a.prm);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -816,7 +827,11 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     int* xrun = reinterpret_cast<int*>(att + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
     int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4] rows before x entry n
     long long* D = reinterpret_cast<long long*>(rowStart + ntab + 4);  // [NB][DL] fixed point
-    float* ctab = reinterpret_cast<float*>(D + NB * DL);              // [NB][CL] (H16: see chs)
+    int* D32 = reinterpret_cast<int*>(D);                              // LUT: [NB][DL] 32-bit sums
+    // the rows as FP32 [NB][DLP] after the conversion fit the 64-bit rows; the LUT's 32-bit
+    // rows are given the larger of the two
+    const int dreg = LUT ? max(NB * DL, NB * DLP) : 2 * NB * DL;     // floats of the row region
+    float* ctab = reinterpret_cast<float*>(D) + dreg;                 // [NB][CL] (H16: see chs)
     // H16: four copies of the coefficients in FP16, copy rho shifted by rho entries, so that a
     // lane's four A operands start at a multiple of 4 halves: [4][NB][CL + 4]
     _Float16* chs = reinterpret_cast<_Float16*>(ctab);
@@ -824,8 +839,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
-    int* queue = rxy + WGB;                                           // [QB] slot << 12 | nz
-    int* misc = queue + QB;                                           // [16] scan scratch, att maxima
+    int* queue = rxy + WGB;                                           // [QBK] slot << 12 | nz
+    int* misc = queue + QBK;                                          // [16] scan scratch, att maxima
     float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
     float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
     _Float16* Dh = reinterpret_cast<_Float16*>(D);                    // H16: [NB][DLP] (aliases D)
@@ -836,7 +851,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     if (tid < 4) misc[8 + tid] = 0;
     if (tid >= 4 && tid < 7) misc[9 + tid] = 0x7f800000;  // misc[13..15]: per-axis min |offset| bits
     build_axis_tables(R, ps, pr, offs, att, tid, WGB);
-    for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
+    for (int e = tid; e < NB * DL; e += WGB) {
+        if (LUT) D32[e] = 0;
+        else D[e] = 0;
+    }
     if (H16) {
         for (int e = tid; e < 4 * NB * (CL + 4); e += WGB) {
             const int rho = e / (NB * (CL + 4)), rm = e - rho * (NB * (CL + 4));
@@ -869,7 +887,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         const float dx = __int_as_float(misc[13]), dy = __int_as_float(misc[14]), dz = __int_as_float(misc[15]);
         const float bound = __int_as_float(misc[9]) * __int_as_float(misc[10]) * __int_as_float(misc[11]) *
                             __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
-        int E = (bound > 0.0f && bound < 3.0e38f) ? 29 - ilogbf(bound) : 0;
+        const int top = LUT ? 24 : 29;  // LUT: 32-bit sums with room for 128 images of the bound
+        int E = (bound > 0.0f && bound < 3.0e38f) ? top - ilogbf(bound) : 0;
         E = min(max(E, -120), 120);
         scale = ldexpf(1.0f, E);
         inv_scale = ldexpf(1.0f, -E);
@@ -955,16 +974,16 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             }
             int tot;
             const int ex = block_scan_excl(cnt, misc, &tot);
-            for (int q0 = 0; q0 < tot; q0 += QB) {
+            for (int q0 = 0; q0 < tot; q0 += QBK) {
                 int e = ex;
 #pragma unroll
                 for (int g = 0; g < 4; g++) {
-                    const int e0 = max(e, q0), e1 = min(e + zc[g], q0 + QB);
+                    const int e0 = max(e, q0), e1 = min(e + zc[g], q0 + QBK);
                     for (int p = e0; p < e1; p++) queue[p - q0] = (tid << 12) | (zf[g] + 2 * (p - e));
                     e += zc[g];
                 }
                 __syncthreads();
-                const int nq = min(QB, tot - q0);
+                const int nq = min(QBK, tot - q0);
                 for (int k = tid; k < nq; k += WGB) {
                     const int qe = queue[k];
                     const int slot = qe >> 12, nz = qe & 4095;
@@ -988,6 +1007,17 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     }
                     if (amp == 0.0f) continue;
                     const float v = amp * scale;
+                    if (LUT) {
+                        // 16 (tau - n0) = p + w on the table's grid (exact in FP64)
+                        const double g16 = (tau - fl) * 16.0;
+                        const double gp = floor(g16);
+                        const int ph = (int)gp;
+                        const float w = (float)(g16 - gp);
+                        int* dp = D32 + ph * DL + (n0 - lo_n0);
+                        lds_add32(dp, fmaf(-v, w, v));
+                        lds_add32(ph < LUT_NB - 1 ? dp + DL : dp - (LUT_NB - 1) * DL + 1, v * w);
+                        continue;
+                    }
                     const float u = fmaf(2.0f, f, -1.0f);
                     const float u2 = u + u;
                     long long* dp = D + (n0 - lo_n0);
@@ -1019,7 +1049,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
 #pragma unroll
                 for (int sl = 0; sl < BCS; sl++) {
                     const int d = tid + sl * WGB;
-                    fv[m][sl] = d < DL ? (float)((double)D[m * DL + d] * (double)inv_scale) : 0.0f;
+                    if (LUT) fv[m][sl] = d < DL ? (float)D32[m * DL + d] * inv_scale : 0.0f;
+                    else fv[m][sl] = d < DL ? (float)((double)D[m * DL + d] * (double)inv_scale) : 0.0f;
                     mx = fmaxf(mx, fabsf(fv[m][sl]));
                 }
             if (H16) atomicMax(misc + 12, __float_as_int(mx));  // non-negative floats order as ints
@@ -1108,20 +1139,23 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         for (int t = tid; t < th - tl; t += WGB)
             orow[tl + t] = ((ybuf[t] + ybuf[BT + t]) + ybuf[2 * BT + t]) + ybuf[3 * BT + t];
         __syncthreads();
-        for (int e = tid; e < NB * DL; e += WGB) D[e] = 0;
+        for (int e = tid; e < NB * DL; e += WGB) {
+            if (LUT) D32[e] = 0;
+            else D[e] = 0;
+        }
         if (H16 && tid == 0) misc[12] = 0;  // read by every thread before the barrier above
         // the row scan of the next tile has barriers before its first atomic
     }
 }
 
-template <int NB, int DLC, bool H16>
+template <int NB, int DLC, bool H16, bool LUT>
 __global__ __launch_bounds__(WGB) void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    basis_item<NB, DLC, H16>(a, blockIdx.x, smem_raw);
+    basis_item<NB, DLC, H16, LUT>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
-template <int NB, int DLC, bool H16>
+template <int NB, int DLC, bool H16, bool LUT>
 __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -1131,7 +1165,7 @@ __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, in
         const int item = next;
         __syncthreads();
         if (item >= n_items) break;
-        basis_item<NB, DLC, H16>(a, item, smem_raw);
+        basis_item<NB, DLC, H16, LUT>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -2206,6 +2240,7 @@ struct TapTables {
     std::vector<float4> jtab;  // [n_pass * 64]: (sigma j, cos, sin, sigma)
     int lut_center = 0, lut_row = 0, lut_k_lo = 0;
     std::vector<float2> lut;   // polyphase [16][lut_row]
+    std::vector<double> lut_tab;  // the table itself (FP64), lut_len entries
 };
 
 // Tap offsets j = t - floor(tau) that can lie inside the window, and their
@@ -2270,6 +2305,7 @@ TapTables make_tap_tables(int precision, int Tw) {
         T.lut_k_lo = fdiv(i_min, 16);
         T.lut_row = fdiv(i_max, 16) - T.lut_k_lo + 2;
         T.lut.assign((size_t)16 * T.lut_row, make_float2(0.0f, 0.0f));
+        T.lut_tab.assign(tab.begin(), tab.begin() + len);
         auto val = [&](int i) { return (i >= 0 && i < len) ? tab[i] : 0.0; };  // zero border
         for (int ph = 0; ph < 16; ph++)
             for (int k = 0; k < T.lut_row; k++) {
@@ -2441,6 +2477,27 @@ BasisTables make_basis_tables(const TapTables& T, int Tw, int NB) {
     return B;
 }
 
+// Lookup-table mode on the basis machinery: phase row p holds the table's entries at the
+// taps of an image whose 1/16-sample delay grid point has phase p, reversed in j with 16
+// zeros in front like the Chebyshev rows: ctab[p][16 + x] = T[16 (J1 - x) - p + center]
+// (zero outside the table), for the offsets j = J1 - x of the table mode's window.  The
+// rows are one column longer than the tile needs: the weight of phase p + 1 of an image
+// at phase 15 goes to phase 0 of the next column.
+BasisTables make_lut_basis_tables(const TapTables& T) {
+    BasisTables B;
+    B.ks = 4 * ((T.nJ + 15 + 15) / 16);
+    B.cl = 4 * B.ks + 16;
+    B.dl = (int)align_up((size_t)std::max(BT + T.nJ, BT - 16 + 4 * B.ks), 16);
+    B.ctab.assign((size_t)LUT_NB * B.cl, 0.0f);
+    const int J1 = T.J0 + T.nJ - 1, len = (int)T.lut_tab.size();
+    for (int p = 0; p < LUT_NB; p++)
+        for (int x = 0; x < T.nJ; x++) {
+            const int i = 16 * (J1 - x) - p + T.lut_center;
+            B.ctab[(size_t)p * B.cl + 16 + x] = (i >= 0 && i < len) ? (float)T.lut_tab[i] : 0.0f;
+        }
+    return B;
+}
+
 // Chebyshev interpolation coefficients of every tap offset for the basis adjoint: row m
 // holds 16 zeros, then C[m][k] for k = j - J0 at 16 + k, then zeros up to cl (FP64 on the
 // host), the Toeplitz operand of its correlation products.
@@ -2460,10 +2517,11 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB,
     return c;
 }
 
-size_t basis_lds_bytes(int ntab, const BasisTables& B, bool h16, int nb) {
+size_t basis_lds_bytes(int ntab, const BasisTables& B, bool h16, int nb, bool lut) {
     const size_t coef = h16 ? 8 * (size_t)nb * (B.cl + 4) : 4 * (size_t)nb * B.cl;
-    return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + 8 * (size_t)nb * B.dl + coef +
-           (8 + 4 + 4) * (size_t)WGB + 4 * (size_t)QB + 4 * 16;
+    const size_t rows = lut ? 4 * (size_t)nb * (B.dl + B.dl / 4) : 8 * (size_t)nb * B.dl;
+    return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + rows + coef + (8 + 4 + 4) * (size_t)WGB +
+           4 * (size_t)(lut ? QB_LUT : QB) + 4 * 16;
 }
 
 // Basis plan: one item per RIR and range of BT-sample tiles (tile_samples rounded up
@@ -2553,6 +2611,7 @@ struct SimLaunch {
     BasisTables Bt;
     bool basis = false;  // accumulation by the basis method (FP32, or FP16 filter inputs)
     bool h16 = false;    // mixed precision: FP16 inputs of the basis filters
+    bool lut = false;    // table mode on the basis machinery (16 phase rows)
     int nb = 8;          // Chebyshev terms of the basis method
     int prec = PREC_FP32;
     int Tw = 0;
@@ -2593,9 +2652,10 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         S.T = f->second;
     }
     GPURIR_CHECK(opt.method >= METHOD_AUTO && opt.method <= METHOD_BASIS, "unknown accumulation method");
-    S.basis = (S.prec == PREC_FP32 || S.prec == PREC_FP16) && opt.method != METHOD_TAPS;
+    S.basis = opt.method != METHOD_TAPS;
     S.h16 = S.prec == PREC_FP16;
-    S.nb = S.h16 ? BASIS_NB_FP16 : BASIS_NB;
+    S.lut = S.prec == PREC_LUT;
+    S.nb = S.lut ? LUT_NB : (S.h16 ? BASIS_NB_FP16 : BASIS_NB);
     if (S.basis && opt.method == METHOD_AUTO) {
         // a launch of fewer tiles than the device has CUs (e.g. one room with two
         // receivers) is faster as a tap scatter, whose image chunks fill the device
@@ -2612,12 +2672,13 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
             std::lock_guard<std::mutex> lk(mu);
             const auto key = std::make_pair(S.Tw, S.nb);
             auto f = cache.find(key);
-            if (f == cache.end()) f = cache.emplace(key, make_basis_tables(S.T, S.Tw, S.nb)).first;
+            if (f == cache.end())
+                f = cache.emplace(key, S.lut ? make_lut_basis_tables(S.T) : make_basis_tables(S.T, S.Tw, S.nb)).first;
             S.Bt = f->second;
         }
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
-        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
             // scatter takes the launch
@@ -2656,7 +2717,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
             (size_t)(QCAP + QPAD) * 5 * 4 + 16 + sizeof(float2) * T.lut.size();
-    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb);
+    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut);
     GPURIR_CHECK(T.n_pass <= 12, "sampling frequency too high: at most 768 window taps are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -2757,12 +2818,14 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
         // the 16 kHz window (128 taps) has its row length compiled in
         const bool dl640 = S.Bt.dl == 640;
-#define GPURIR_BASIS_FN(K, N, H) \
-    (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H>) : reinterpret_cast<const void*>(&K<N, 0, H>))
-        const void* fn_once = S.h16 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, true)
-                                    : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, false);
-        const void* fn_pers = S.h16 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, true)
-                                    : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, false);
+#define GPURIR_BASIS_FN(K, N, H, U) \
+    (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H, U>) : reinterpret_cast<const void*>(&K<N, 0, H, U>))
+        const void* fn_once = S.lut ? GPURIR_BASIS_FN(ism_basis_kernel, LUT_NB, false, true)
+                            : S.h16 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, true, false)
+                                    : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, false, false);
+        const void* fn_pers = S.lut ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, LUT_NB, false, true)
+                            : S.h16 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, true, false)
+                                    : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, false, false);
 #undef GPURIR_BASIS_FN
         int n_items = (int)S.P.items.size();
         int resident = 0;

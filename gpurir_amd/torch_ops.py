@@ -69,11 +69,12 @@ def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, 
     """simulateRIR on device tensors; returns float32 [n_src, n_rcv, nSamples] on the GPU.
 
     crossfade: samples of linear cross-fade from the early part into the diffuse model before
-    Tdiff (0, the default, is the reference's hard switch).  method (precision "fp32" and "fp16";
-    "lut" always scatters taps): "auto" (the default: "basis" when the launch has at least 256
-    tiles of 512 early samples and fs <= 64 kHz, else "taps"), "basis" (Chebyshev expansion of the
-    taps in the fractional delay, matrix-core filters; 8 terms in FP32, 6 terms and FP16 filter
-    inputs in "fp16") or "taps" (every image scatters its windowed-sinc taps)."""
+    Tdiff (0, the default, is the reference's hard switch).  method: "auto" (the default: "basis"
+    when the launch has at least 256 tiles of 512 early samples and fs <= 64 kHz, else "taps"),
+    "basis" (per-image sums on a few rows and matrix-core filters: a Chebyshev expansion of the
+    taps in the fractional delay, 8 terms in "fp32", 6 terms and FP16 filter inputs in "fp16";
+    16 phase rows of the table's 1/16-sample grid in "lut") or "taps" (every image scatters its
+    windowed-sinc taps)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     ps = _dev_f32(pos_src, dev).reshape(-1, 3)
     pr = _dev_f32(pos_rcv, dev).reshape(-1, 3)

@@ -39,15 +39,18 @@ extern "C" {
                            /* filter inputs on the matrix cores (basis) or packed FP16  */
                            /* windowed-sinc taps (tap scatter)                          */
 
-/* Accumulation of the exact sinc, FP32 or mixed precision (the lookup table always scatters taps) */
+/* Accumulation of the windowed sinc, in every precision mode */
 #define GPURIR_METHOD_AUTO 0  /* library default: BASIS, or TAPS for launches of fewer    */
                               /* than 256 tiles of 512 early samples (too few to fill    */
                               /* the device) and for windows of more than 513 taps     */
                               /* (fs above 64 kHz)                                       */
 #define GPURIR_METHOD_TAPS 1  /* every image scatters its Tw windowed-sinc taps           */
-#define GPURIR_METHOD_BASIS 2 /* 8-term Chebyshev expansion of the taps in the fractional */
-                              /* delay; matrix-core FIR (<= 5e-7 per tap of 1); 6 terms  */
-                              /* and FP16 filter inputs in mixed precision (<= 6e-5)      */
+#define GPURIR_METHOD_BASIS 2 /* per-image sums on a few rows, then matrix-core FIRs:     */
+                              /* FP32, 8-term Chebyshev expansion of the taps in the     */
+                              /* fractional delay (<= 5e-7 per tap of 1); mixed precision */
+                              /* 6 terms and FP16 filter inputs (<= 6e-5); table mode,    */
+                              /* 16 phase rows of the 1/16-sample grid (the table's own   */
+                              /* linear interpolation, exactly)                           */
 
 /* Diffuse-tail noise */
 #define GPURIR_NOISE_LOGISTIC 0 /* unit-variance logistic, as src/gpuRIR_cuda.cu:454-455 */
@@ -65,7 +68,7 @@ typedef struct gpurir_options {
     int persistent;             /* batch plans: 1 = persistent accumulation launch       */
     int crossfade;              /* samples of linear cross-fade into the diffuse model   */
                                 /* before Tdiff (0 = the reference's hard switch)        */
-    int method;                 /* GPURIR_METHOD_* (FP32 precision only)                 */
+    int method;                 /* GPURIR_METHOD_*                                       */
 } gpurir_options_t;
 
 /* Library version (major*10000 + minor*100 + patch). */

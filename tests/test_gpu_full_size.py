@@ -2,7 +2,8 @@
 
   * 4096 uneven rooms (40,709 RIRs, up to 240,768 images and 16 receivers per room) in one
     autotuned, graph-replayed plan: six rooms checked at 1e-5 relative L2 (early part and
-    tail separately, same Philox stream), mixed precision at 1e-3; replays bit-identical.
+    tail separately, same Philox stream), mixed precision at 1e-3, the table mode against the
+    oracle's table mode at 1e-5; replays bit-identical.
   * the moving-source trajectory (128 points x 8 mics, full ISM of 0.5 s, 246,906 images per
     RIR, 10 s of noise): the RIRs of 8 points against the oracle at 1e-5, and the whole
     convolution + float64 overlap-add against the oracle's FP64 overlap-add of the same RIRs
@@ -38,7 +39,7 @@ def uneven_batch():
     return W.uneven_rooms(n_rooms=4096, seed=0)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp16", 1e-3)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp16", 1e-3), ("lut", 1e-5)])
 def test_batch_4096_rooms_full_size(uneven_batch, precision, tol):
     rooms, fs = uneven_batch, W.FS
     plan = T.BatchPlan(rooms, fs, precision=precision, seed=1234, call_offset=0)
@@ -55,7 +56,8 @@ def test_batch_4096_rooms_full_size(uneven_batch, precision, tol):
     for k in batch_rooms_to_check(rooms, fs):
         r = rooms[k]
         o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
-                             Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
+                             Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k,
+                             mode=orc.LUT if precision == "lut" else orc.EXACT_SCATTER)
         g = plan.rirs[k].cpu().numpy()
         n_ism = orc.n_samples(r["Tdiff"], fs)
         e_ism = rel_l2(g[..., :n_ism], o[..., :n_ism])

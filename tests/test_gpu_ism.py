@@ -67,6 +67,18 @@ def test_mid_room_lut_against_oracle_lut():
     assert rel_l2(g[..., :3200], ex[..., :3200]) < 5e-3
 
 
+def test_mid_room_lut_full_size():
+    """The reference's default mode on the whole mid room (4 x 32 directional RIRs, 896 tiles:
+    the automatic choice takes the 16 phase rows on the matrix cores) against the oracle's
+    table mode, and the tap scatter of the same table."""
+    cfg = W.mid_room()
+    g, o = run_both(cfg, precision="lut")
+    g_t, _ = run_both(cfg, precision="lut", method="taps")
+    e = check_parity(g, o, 3200)
+    e_t = check_parity(g_t, o, 3200)
+    print(f"mid room LUT: phase rows {e:.2e}, tap scatter {e_t:.2e}")
+
+
 @pytest.mark.parametrize("cfg_name", ["plain", "mid"])
 def test_image_offsets_identical(cfg_name):
     cfg = W.plain_room() if cfg_name == "plain" else W.mid_room(n_src=2, n_rcv=4)
@@ -101,6 +113,9 @@ def test_sampling_rates_odd_and_even_windows(fs):
     check_parity(g1, o, orc.n_samples(0.06, fs))
     g2, o2 = run_both(cfg, precision="lut")
     check_parity(g2, o2, orc.n_samples(0.06, fs))
+    if fs <= 64000.0:  # the table mode on the matrix-core filters (16 phase rows)
+        g3, _ = run_both(cfg, precision="lut", method="basis")
+        check_parity(g3, o2, orc.n_samples(0.06, fs))
 
 
 def test_basis_and_taps_methods():

@@ -421,7 +421,10 @@ __device__ __forceinline__ void taps_image(const AccArgs& a, long long* acc, con
 }
 
 // One work item: stages (a) and (b) for an image chunk of one RIR over one time tile.
-template <int MODE, int PMAX, bool MASK>
+// GRP: windows of more than 12 passes (fs above ~96 kHz) in groups of PMAX = 12 passes: the
+// per-lane constants of a group are loaded before the queue is walked for that group, and the
+// table of the LUT mode (too large for LDS there) is read from global memory.
+template <int MODE, int PMAX, bool MASK, bool GRP>
 __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     const Item it = a.items[item];
     const RoomK R = load_room(a.rooms, it.room, a.prm);
@@ -445,30 +448,35 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
     __syncthreads();
     build_axis_tables(R, ps, pr, offs, att, tid, WG);
     for (int e = tid; e < a.acc_len; e += WG) acc[e] = 0;
-    if (MODE == PREC_LUT)
+    if (MODE == PREC_LUT && !GRP)
         for (int e = tid; e < 16 * a.lut_row; e += WG) lut[e] = a.lut[e];
-    // per-lane constants of every pass
+    const float2* tab = GRP ? a.lut : lut;
+    const int n_groups = GRP ? a.n_pass / PMAX : 1;
+    // per-lane constants of every pass (of group g)
     float jf[PMAX], cj[PMAX], sj[PMAX], hj[PMAX];
-#pragma unroll
-    for (int p = 0; p < PMAX; p++) {
-        const float4 c = a.jtab[64 * p + lane];  // the table holds exactly PMAX passes
-        jf[p] = c.x;
-        cj[p] = c.y;
-        sj[p] = c.z;
-        hj[p] = c.w;
-    }
     // FP16 mode: the same constants, paired by pass (a lone last pass is paired with itself)
     half2_t J2[(PMAX + 1) / 2], H2[(PMAX + 1) / 2], C2[(PMAX + 1) / 2], S2[(PMAX + 1) / 2];
-    if (MODE == PREC_FP16) {
+    auto load_group = [&](int g) {
 #pragma unroll
-        for (int pp = 0; pp < (PMAX + 1) / 2; pp++) {
-            const int lo = 2 * pp, hi = 2 * pp + 1 < PMAX ? 2 * pp + 1 : 2 * pp;
-            J2[pp] = h2(jf[lo], jf[hi]);
-            H2[pp] = h2(hj[lo], hj[hi]);
-            C2[pp] = h2(cj[lo], cj[hi]);
-            S2[pp] = h2(sj[lo], sj[hi]);
+        for (int p = 0; p < PMAX; p++) {
+            const float4 c = a.jtab[64 * (PMAX * g + p) + lane];  // the table holds n_groups PMAX passes
+            jf[p] = c.x;
+            cj[p] = c.y;
+            sj[p] = c.z;
+            hj[p] = c.w;
         }
-    }
+        if (MODE == PREC_FP16) {
+#pragma unroll
+            for (int pp = 0; pp < (PMAX + 1) / 2; pp++) {
+                const int lo = 2 * pp, hi = 2 * pp + 1 < PMAX ? 2 * pp + 1 : 2 * pp;
+                J2[pp] = h2(jf[lo], jf[hi]);
+                H2[pp] = h2(hj[lo], hj[hi]);
+                C2[pp] = h2(cj[lo], cj[hi]);
+                S2[pp] = h2(sj[lo], sj[hi]);
+            }
+        }
+    };
+    load_group(0);
     float os[4] = {0, 0, 0, 0}, orr[4] = {0, 0, 0, 0};
     if (a.spkr != 0) load_orient(a.orv_src + 3 * (R.src_off + s), os);
     if (a.mic != 0) load_orient(a.orv_rcv + 3 * (R.rcv_off + r), orr);
@@ -601,29 +609,33 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
             }
             since += nq;
         }
-        if (wave < nq) {
-            const float4* pq = q4 + (tid >> 6);
-            // this wave's S row: group g holds its images 4g .. 4g + 3 (xA, xB, yA, yB)
-            const float4* pS4 = reinterpret_cast<const float4*>(qS + wave * QSW);
-            float4 xA = pq[0], xB = pq[NWAVE];
-            float4 s4 = pS4[0];
-            for (int e = wave;;) {
-                const float4 yA = pq[2 * NWAVE], yB = pq[3 * NWAVE];
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xA, s4.x, jf, cj, sj, hj, J2, H2, C2, S2);
-                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, xB, s4.y, jf, cj, sj, hj, J2, H2, C2, S2);
-                e += 2 * NWAVE;
-                if (e >= nq) break;
-                pq += 2 * NWAVE;
-                xA = pq[2 * NWAVE];
-                xB = pq[3 * NWAVE];
-                const float4 s4n = pS4[1];  // group <= 32: inside the row (QSW = 132 floats)
-                taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yA, s4.z, jf, cj, sj, hj, J2, H2, C2, S2);
-                if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, lut, lane, yB, s4.w, jf, cj, sj, hj, J2, H2, C2, S2);
-                e += 2 * NWAVE;
-                if (e >= nq) break;
-                pq += 2 * NWAVE;
-                pS4 += 1;
-                s4 = s4n;
+        for (int gi = 0; gi < n_groups; gi++) {
+            if (GRP && gi > 0) load_group(gi);
+            const int lg = lane + 64 * PMAX * gi;  // the lane's tap offset in pass 0 of group gi
+            if (wave < nq) {
+                const float4* pq = q4 + (tid >> 6);
+                // this wave's S row: group g holds its images 4g .. 4g + 3 (xA, xB, yA, yB)
+                const float4* pS4 = reinterpret_cast<const float4*>(qS + wave * QSW);
+                float4 xA = pq[0], xB = pq[NWAVE];
+                float4 s4 = pS4[0];
+                for (int e = wave;;) {
+                    const float4 yA = pq[2 * NWAVE], yB = pq[3 * NWAVE];
+                    taps_image<MODE, PMAX, MASK>(a, acc, tab, lg, xA, s4.x, jf, cj, sj, hj, J2, H2, C2, S2);
+                    if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, tab, lg, xB, s4.y, jf, cj, sj, hj, J2, H2, C2, S2);
+                    e += 2 * NWAVE;
+                    if (e >= nq) break;
+                    pq += 2 * NWAVE;
+                    xA = pq[2 * NWAVE];
+                    xB = pq[3 * NWAVE];
+                    const float4 s4n = pS4[1];  // group <= 32: inside the row (QSW = 132 floats)
+                    taps_image<MODE, PMAX, MASK>(a, acc, tab, lg, yA, s4.z, jf, cj, sj, hj, J2, H2, C2, S2);
+                    if (e + NWAVE < nq) taps_image<MODE, PMAX, MASK>(a, acc, tab, lg, yB, s4.w, jf, cj, sj, hj, J2, H2, C2, S2);
+                    e += 2 * NWAVE;
+                    if (e >= nq) break;
+                    pq += 2 * NWAVE;
+                    pS4 += 1;
+                    s4 = s4n;
+                }
             }
         }
         __syncthreads();
@@ -648,10 +660,10 @@ __device__ __forceinline__ void accumulate_item(const AccArgs& a, int item, unsi
 }
 
 // One workgroup per work item.
-template <int MODE, int PMAX, bool MASK>
+template <int MODE, int PMAX, bool MASK, bool GRP>
 __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    accumulate_item<MODE, PMAX, MASK>(a, blockIdx.x, smem_raw);
+    accumulate_item<MODE, PMAX, MASK, GRP>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form: a grid sized to the resident workgroups takes items from a
@@ -659,7 +671,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_kernel(AccArgs a) {
 // number of workgroups done; the last one resets both for the next launch (every
 // other workgroup has made its final fetch by then), so a captured graph can be
 // replayed without any memset node.
-template <int MODE, int PMAX, bool MASK>
+template <int MODE, int PMAX, bool MASK, bool GRP>
 __global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a, int n_items,
                                                                        unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -670,7 +682,7 @@ __global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a
         const int item = next;
         __syncthreads();  // every thread has read next; the item may overwrite all of LDS
         if (item >= n_items) break;
-        accumulate_item<MODE, PMAX, MASK>(a, item, smem_raw);
+        accumulate_item<MODE, PMAX, MASK, GRP>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -2263,10 +2275,10 @@ TapTables make_tap_tables(int precision, int Tw) {
         T.nJ = J1 - T.J0 + 1;
     }
     const int J1 = T.J0 + T.nJ - 1;
-    // passes are instantiated for 1, 2, 3, 4, 6 and 12; other counts are padded with
-    // masked lanes up to the next one
+    // passes are instantiated for 1, 2, 3, 4, 6 and 12, and beyond 12 in groups of 12; other
+    // counts are padded with masked lanes up to the next one
     const int need = (T.nJ + 63) / 64;
-    T.n_pass = need <= 4 ? need : (need <= 6 ? 6 : 12);
+    T.n_pass = need <= 4 ? need : (need <= 6 ? 6 : 12 * ((need + 11) / 12));
     // a pass needs the window test if it holds padding lanes (j > J1) or, for an odd
     // window (half-integer Tw/2, or integer Tw/2 in LUT mode with odd Tw), the end
     // offsets J0, J1 whose membership depends on f.  For an even window the end
@@ -2337,7 +2349,8 @@ constexpr int MIN_TILE = 512;
 size_t acc_lds_bytes(int tt, int ntab, const TapTables& T) {
     const size_t nt = (size_t)(ntab + 3) / 4 * 4;
     const size_t acc = ((size_t)tt + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64 + 3) / 4 * 4;
-    return 12 * nt + 8 * acc + (size_t)(QCAP + QPAD) * 20 + 32 + sizeof(float2) * T.lut.size();
+    const size_t lut = T.n_pass <= 12 ? sizeof(float2) * T.lut.size() : 0;  // grouped: read from global memory
+    return 12 * nt + 8 * acc + (size_t)(QCAP + QPAD) * 20 + 32 + lut;
 }
 
 Plan make_plan(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt, bool sort_items) {
@@ -2716,9 +2729,9 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     // acc index = n0 - t_lo + J1 + (j - J0), n0 <= t_hi - 1 - J0, j - J0 < 64 n_pass
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
-            (size_t)(QCAP + QPAD) * 5 * 4 + 16 + sizeof(float2) * T.lut.size();
+            (size_t)(QCAP + QPAD) * 5 * 4 + 32 + (T.n_pass <= 12 ? sizeof(float2) * T.lut.size() : 0);
     if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut);
-    GPURIR_CHECK(T.n_pass <= 12, "sampling frequency too high: at most 768 window taps are supported");
+    GPURIR_CHECK(T.n_pass <= 48, "sampling frequency too high: at most 3072 window taps (fs 384 kHz) are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
 }
@@ -2848,8 +2861,9 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         const void* fn = nullptr;
         const bool mk = S.T.mask_bits != 0;
 #define GPURIR_ACC_FN(M, P) \
-    (mk ? reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, true>) \
-        : reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, false>))
+    (mk ? reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, true, false>) \
+        : reinterpret_cast<const void*>(&ism_accumulate_kernel<M, P, false, false>))
+#define GPURIR_ACC_GRP(M) reinterpret_cast<const void*>(&ism_accumulate_kernel<M, 12, true, true>)
 #define GPURIR_ACC_PICK(M)                                         \
     switch (S.T.n_pass) {                                          \
         case 1: fn = GPURIR_ACC_FN(M, 1); break;                   \
@@ -2857,7 +2871,8 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         case 3: fn = GPURIR_ACC_FN(M, 3); break;                   \
         case 4: fn = GPURIR_ACC_FN(M, 4); break;                   \
         case 6: fn = GPURIR_ACC_FN(M, 6); break;                   \
-        default: fn = GPURIR_ACC_FN(M, 12); break;                 \
+        case 12: fn = GPURIR_ACC_FN(M, 12); break;                 \
+        default: fn = GPURIR_ACC_GRP(M); break;                    \
     }
         if (S.prec == PREC_LUT) {
             GPURIR_ACC_PICK(PREC_LUT)
@@ -2868,12 +2883,14 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         }
 #undef GPURIR_ACC_PICK
 #undef GPURIR_ACC_FN
+#undef GPURIR_ACC_GRP
         const void* fn_once = fn;  // one workgroup per item
         if (S.counters) {
             // the same instantiation in its persistent form
 #define GPURIR_ACC_FN(M, P) \
-    (mk ? reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, P, true>) \
-        : reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, P, false>))
+    (mk ? reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, P, true, false>) \
+        : reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, P, false, false>))
+#define GPURIR_ACC_GRP(M) reinterpret_cast<const void*>(&ism_accumulate_persistent_kernel<M, 12, true, true>)
 #define GPURIR_ACC_PICK(M)                                         \
     switch (S.T.n_pass) {                                          \
         case 1: fn = GPURIR_ACC_FN(M, 1); break;                   \
@@ -2881,7 +2898,8 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         case 3: fn = GPURIR_ACC_FN(M, 3); break;                   \
         case 4: fn = GPURIR_ACC_FN(M, 4); break;                   \
         case 6: fn = GPURIR_ACC_FN(M, 6); break;                   \
-        default: fn = GPURIR_ACC_FN(M, 12); break;                 \
+        case 12: fn = GPURIR_ACC_FN(M, 12); break;                 \
+        default: fn = GPURIR_ACC_GRP(M); break;                    \
     }
             if (S.prec == PREC_LUT) {
                 GPURIR_ACC_PICK(PREC_LUT)
@@ -2892,6 +2910,7 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
             }
 #undef GPURIR_ACC_PICK
 #undef GPURIR_ACC_FN
+#undef GPURIR_ACC_GRP
         }
         if (int rc = allow_lds(fn, S.lds)) return rc;
         int resident = 0;

@@ -44,7 +44,8 @@ extern "C" {
                               /* than 256 tiles of 512 early samples (too few to fill    */
                               /* the device) and for windows of more than 513 taps     */
                               /* (fs above 64 kHz)                                       */
-#define GPURIR_METHOD_TAPS 1  /* every image scatters its Tw windowed-sinc taps           */
+#define GPURIR_METHOD_TAPS 1  /* every image scatters its Tw windowed-sinc taps (windows  */
+                              /* of up to 3072 taps, fs up to 384 kHz)                    */
 #define GPURIR_METHOD_BASIS 2 /* per-image sums on a few rows, then matrix-core FIRs:     */
                               /* FP32, 8-term Chebyshev expansion of the taps in the     */
                               /* fractional delay (<= 5e-7 per tap of 1); mixed precision */

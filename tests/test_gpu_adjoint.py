@@ -94,6 +94,24 @@ def test_adjoint_without_tail():
         assert rel(got[k], want[k]) <= TOL
 
 
+def test_adjoint_192k():
+    """fs = 192 kHz: a window of 1536 taps (the tap adjoint; the automatic choice)."""
+    L, beta, src, rcv, _ = rounded_case(7)
+    fs = 192000.0
+    nS = orc.n_samples(0.012, fs)
+    g = np.random.default_rng(8).standard_normal((1, 2, nS))
+    kw = dict(nb=[3, 3, 3], Tmax=0.012, Tdiff=0.008)
+    got = gpu_grads(L, beta, src, rcv, g, fs=fs, **kw)
+    b = torch.tensor(beta, requires_grad=True)
+    s = torch.tensor(src, requires_grad=True)
+    r = torch.tensor(rcv, requires_grad=True)
+    lv = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    y = tr.simulate(L, b, s, r, kw["nb"], kw["Tmax"], fs, Tdiff=kw["Tdiff"], level=lv)
+    (y * torch.as_tensor(g)).sum().backward()
+    want = {"beta": b.grad.numpy(), "src": s.grad.numpy(), "rcv": r.grad.numpy(), "level": float(lv.grad)}
+    assert_close(got, want, what="192 kHz vs autograd")
+
+
 @pytest.mark.parametrize("method", ["taps", "basis"])
 def test_adjoint_odd_window_44k1(method):
     """fs = 44.1 kHz: Tw = 353 (odd): the masked tap adjoint, and the basis adjoint."""

@@ -96,8 +96,9 @@ def test_image_offsets_identical(cfg_name):
 
 
 # 8 kHz: Tw = 64 (1 pass of 64 lanes), 16 kHz: 128 (2), 22.05 kHz: 176 (3), 32 kHz: 256 (4),
-# 44.1 kHz: 353 (odd, 6 with masked lanes), 48 kHz: 384 (6), 96 kHz: 768 (12)
-ALL_FS = [8000.0, 16000.0, 22050.0, 32000.0, 44100.0, 48000.0, 96000.0]
+# 44.1 kHz: 353 (odd, 6 with masked lanes), 48 kHz: 384 (6), 96 kHz: 768 (12), 176.4 kHz: 1411
+# (odd, two groups of 12 passes), 192 kHz: 1536 (two groups)
+ALL_FS = [8000.0, 16000.0, 22050.0, 32000.0, 44100.0, 48000.0, 96000.0, 176400.0, 192000.0]
 
 
 @pytest.mark.parametrize("fs", ALL_FS)

@@ -65,7 +65,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's default work-item shape")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "lut"],
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "lut"],
                     help="sinc evaluation of the batch measured by --only-batch")
     ap.add_argument("--method", default="auto", choices=["auto", "taps", "basis"],
                     help="accumulation of the batch measured by --only-batch (tap scatter or basis expansion)")
@@ -312,7 +312,7 @@ def main():
             o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
                                  Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
             errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
-        res = {"precision": precision, "method": method if precision in ("fp32", "fp16") else "taps", "autotune": tune,
+        res = {"precision": precision, "method": method, "autotune": tune,
                "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
                "rirs_per_s": n_rirs / (ms_step / 1e3), "accumulate_ms": ms_acc, "finalize_ms": ms_fin,
                "tflops_algorithmic": FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12, "rel_l2_vs_fp64": max(errs),
@@ -324,7 +324,7 @@ def main():
         torch.cuda.empty_cache()
         return res
 
-    modes = [args.precision] if args.only_batch else ["fp32", "fp32_taps", "fp16", "lut"]
+    modes = [args.precision] if args.only_batch else ["fp32", "fp32_taps", "fp16", "bf16", "lut"]
     if args.only_batch:
         batch = {args.precision: measure_batch(args.precision, args.method)}
     else:

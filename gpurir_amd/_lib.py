@@ -23,7 +23,7 @@ class Options(ctypes.Structure):
                 ("crossfade", ctypes.c_int), ("method", ctypes.c_int)]
 
 
-PREC_FP32, PREC_LUT, PREC_FP16 = 0, 1, 2
+PREC_FP32, PREC_LUT, PREC_FP16, PREC_BF16 = 0, 1, 2, 3
 NOISE_LOGISTIC, NOISE_GAUSSIAN = 0, 1
 METHOD_AUTO, METHOD_TAPS, METHOD_BASIS = 0, 1, 2
 

@@ -89,7 +89,7 @@ struct RoomHost {
 };
 
 // precision modes (keep in sync with include/gpurir.h)
-enum { PREC_FP32 = 0, PREC_LUT = 1, PREC_FP16 = 2 };
+enum { PREC_FP32 = 0, PREC_LUT = 1, PREC_FP16 = 2, PREC_BF16 = 3 };
 enum { METHOD_AUTO = 0, METHOD_TAPS = 1, METHOD_BASIS = 2 };
 
 struct SimOptions {

@@ -747,6 +747,7 @@ constexpr int QB_LUT = 1536;  // queue entries per round in the table mode (its 
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short short4_t __attribute__((ext_vector_type(4)));
 
 // Inclusive prefix sum over the 64 lanes of a wave with DPP moves (VALU only):
 // row_shr 1, 2, 4, 8 within each row of 16 lanes (lanes shifted in from outside the
@@ -819,11 +820,26 @@ __device__ inline void axis_range(const AxisRuns& A, int q, double u, double v, 
     n_first = 2 * m0 - q + A.half;
 }
 
+// FP32 -> BF16, rounded to nearest even (finite inputs)
+__device__ inline unsigned short f32_to_bf16(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// A 16-bit filter input: FP16 (H16 = 1) or BF16 (H16 = 2), as its bits.
+template <int H16>
+__device__ inline unsigned short half_bits(float x) {
+    if (H16 == 2) return f32_to_bf16(x);
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
 // DLC: the basis row length as a compile-time constant (the LDS offsets of an image's
 // eight atomics are then immediates), or 0 for the launch's a.bdl.  H16 (mixed precision):
-// the filters take FP16 inputs (D scaled per tile by a power of two, the coefficients
-// rounded) with FP32 accumulation, v_mfma_f32_16x16x16_f16.
-template <int NB, int DLC, bool H16, bool LUT>
+// the filters take 16-bit inputs (D scaled per tile by a power of two, the coefficients
+// rounded) with FP32 accumulation: FP16 (1, v_mfma_f32_16x16x16_f16) or BF16 (2,
+// v_mfma_f32_16x16x16_bf16).
+template <int NB, int DLC, int H16, bool LUT>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
     constexpr int QBK = LUT ? QB_LUT : QB;
@@ -846,7 +862,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     float* ctab = reinterpret_cast<float*>(D) + dreg;                 // [NB][CL] (H16: see chs)
     // H16: four copies of the coefficients in FP16, copy rho shifted by rho entries, so that a
     // lane's four A operands start at a multiple of 4 halves: [4][NB][CL + 4]
-    _Float16* chs = reinterpret_cast<_Float16*>(ctab);
+    unsigned short* chs = reinterpret_cast<unsigned short*>(ctab);
     const int crw = H16 ? 2 * NB * (CL + 4) : NB * CL;                 // floats of the region
     double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
@@ -855,7 +871,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     int* misc = queue + QBK;                                          // [16] scan scratch, att maxima
     float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
     float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
-    _Float16* Dh = reinterpret_cast<_Float16*>(D);                    // H16: [NB][DLP] (aliases D)
+    unsigned short* Dh = reinterpret_cast<unsigned short*>(D);        // H16: [NB][DLP] (aliases D)
 
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
@@ -871,7 +887,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         for (int e = tid; e < 4 * NB * (CL + 4); e += WGB) {
             const int rho = e / (NB * (CL + 4)), rm = e - rho * (NB * (CL + 4));
             const int m = rm / (CL + 4), z = rm - m * (CL + 4), src = z - rho;
-            chs[e] = (src >= 0 && src < CL) ? (_Float16)a.ctab[m * CL + src] : (_Float16)0.0f;
+            chs[e] = half_bits<H16>((src >= 0 && src < CL) ? a.ctab[m * CL + src] : 0.0f);
         }
         if (tid == 0) misc[12] = 0;
     } else {
@@ -1081,7 +1097,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 for (int sl = 0; sl < BCS; sl++) {
                     const int d = tid + sl * WGB;
                     if (d < DL) {
-                        if (H16) Dh[m * DLP + d + 4 * (d >> 4)] = (_Float16)(fv[m][sl] * sc);
+                        if (H16) Dh[m * DLP + d + 4 * (d >> 4)] = half_bits<H16>(fv[m][sl] * sc);
                         else Df[m * DLP + d + 4 * (d >> 4)] = fv[m][sl];
                     }
                 }
@@ -1103,14 +1119,20 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     // rows 2 mq, 2 mq + 1 (NB = 8) or mq, mq + 4 (NB = 6: waves 2 and 3 take one)
                     const int m = NB % 4 == 0 ? mq * (NB / 4) + mm : mq + 4 * mm;
                     if (NB % 4 != 0 && m >= NB) break;
-                    const _Float16* Ap = chs + (rho * NB + m) * (CL + 4) + g * KS + 16 - c + rho;
-                    const _Float16* Bp = Dh + m * DLP;
+                    const unsigned short* Ap = chs + (rho * NB + m) * (CL + 4) + g * KS + 16 - c + rho;
+                    const unsigned short* Bp = Dh + m * DLP;
                     const int kb = 256 * ot + 16 * c + g * KS;
                     for (int t4 = 0; t4 < KS / 4; t4++) {
                         const int dl = kb + 4 * t4;
-                        const half4_t xv = *reinterpret_cast<const half4_t*>(Bp + dl + 4 * (dl >> 4));
-                        const half4_t av = *reinterpret_cast<const half4_t*>(Ap + 4 * t4);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, xv, acc, 0, 0, 0);
+                        if (H16 == 2) {
+                            const short4_t xv = *reinterpret_cast<const short4_t*>(Bp + dl + 4 * (dl >> 4));
+                            const short4_t av = *reinterpret_cast<const short4_t*>(Ap + 4 * t4);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, xv, acc, 0, 0, 0);
+                        } else {
+                            const half4_t xv = *reinterpret_cast<const half4_t*>(Bp + dl + 4 * (dl >> 4));
+                            const half4_t av = *reinterpret_cast<const half4_t*>(Ap + 4 * t4);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, xv, acc, 0, 0, 0);
+                        }
                     }
                 }
                 acc = acc * out_scale;
@@ -1160,14 +1182,14 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     }
 }
 
-template <int NB, int DLC, bool H16, bool LUT>
+template <int NB, int DLC, int H16, bool LUT>
 __global__ __launch_bounds__(WGB) void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     basis_item<NB, DLC, H16, LUT>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
-template <int NB, int DLC, bool H16, bool LUT>
+template <int NB, int DLC, int H16, bool LUT>
 __global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -2623,7 +2645,7 @@ struct SimLaunch {
     TapTables T;
     BasisTables Bt;
     bool basis = false;  // accumulation by the basis method (FP32, or FP16 filter inputs)
-    bool h16 = false;    // mixed precision: FP16 inputs of the basis filters
+    int h16 = 0;         // mixed precision: 16-bit inputs of the basis filters (1 FP16, 2 BF16)
     bool lut = false;    // table mode on the basis machinery (16 phase rows)
     int nb = 8;          // Chebyshev terms of the basis method
     int prec = PREC_FP32;
@@ -2638,7 +2660,8 @@ struct SimLaunch {
 };
 
 int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLaunch& S) {
-    GPURIR_CHECK(opt.precision == PREC_FP32 || opt.precision == PREC_LUT || opt.precision == PREC_FP16,
+    GPURIR_CHECK(opt.precision == PREC_FP32 || opt.precision == PREC_LUT || opt.precision == PREC_FP16 ||
+                     opt.precision == PREC_BF16,
                  "unknown precision mode");
     GPURIR_CHECK(opt.fs > 0.0f && opt.c > 0.0f, "fs and c must be positive");
     GPURIR_CHECK(!rooms.empty(), "no rooms");
@@ -2665,11 +2688,13 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         S.T = f->second;
     }
     GPURIR_CHECK(opt.method >= METHOD_AUTO && opt.method <= METHOD_BASIS, "unknown accumulation method");
+    GPURIR_CHECK(S.prec != PREC_BF16 || opt.method != METHOD_TAPS,
+                 "the BF16 mode runs on the matrix cores only (method basis or auto)");
     S.basis = opt.method != METHOD_TAPS;
-    S.h16 = S.prec == PREC_FP16;
+    S.h16 = S.prec == PREC_FP16 ? 1 : (S.prec == PREC_BF16 ? 2 : 0);
     S.lut = S.prec == PREC_LUT;
     S.nb = S.lut ? LUT_NB : (S.h16 ? BASIS_NB_FP16 : BASIS_NB);
-    if (S.basis && opt.method == METHOD_AUTO) {
+    if (S.basis && opt.method == METHOD_AUTO && S.prec != PREC_BF16) {
         // a launch of fewer tiles than the device has CUs (e.g. one room with two
         // receivers) is faster as a tap scatter, whose image chunks fill the device
         long long tiles = 0;
@@ -2695,7 +2720,8 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
             // scatter takes the launch
-            GPURIR_CHECK(opt.method == METHOD_AUTO, "basis method: LDS budget exceeded (window too long)");
+            GPURIR_CHECK(opt.method == METHOD_AUTO && S.prec != PREC_BF16,
+                         "basis method: LDS budget exceeded (window longer than 513 taps, fs above 64 kHz)");
             S.basis = false;
         }
     }
@@ -2833,12 +2859,14 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         const bool dl640 = S.Bt.dl == 640;
 #define GPURIR_BASIS_FN(K, N, H, U) \
     (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H, U>) : reinterpret_cast<const void*>(&K<N, 0, H, U>))
-        const void* fn_once = S.lut ? GPURIR_BASIS_FN(ism_basis_kernel, LUT_NB, false, true)
-                            : S.h16 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, true, false)
-                                    : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, false, false);
-        const void* fn_pers = S.lut ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, LUT_NB, false, true)
-                            : S.h16 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, true, false)
-                                    : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, false, false);
+        const void* fn_once = S.lut        ? GPURIR_BASIS_FN(ism_basis_kernel, LUT_NB, 0, true)
+                              : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2, false)
+                              : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1, false)
+                                           : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0, false);
+        const void* fn_pers = S.lut        ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, LUT_NB, 0, true)
+                              : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2, false)
+                              : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1, false)
+                                           : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 0, false);
 #undef GPURIR_BASIS_FN
         int n_items = (int)S.P.items.size();
         int resident = 0;

@@ -20,7 +20,7 @@ import torch
 from . import _lib
 
 PATTERNS = {"omni": 0, "homni": 1, "card": 2, "hypcard": 3, "subcard": 4, "bidir": 5}
-PRECISIONS = {"fp32": _lib.PREC_FP32, "lut": _lib.PREC_LUT, "fp16": _lib.PREC_FP16}
+PRECISIONS = {"fp32": _lib.PREC_FP32, "lut": _lib.PREC_LUT, "fp16": _lib.PREC_FP16, "bf16": _lib.PREC_BF16}
 NOISES = {"logistic": _lib.NOISE_LOGISTIC, "gaussian": _lib.NOISE_GAUSSIAN}
 
 

@@ -38,6 +38,9 @@ extern "C" {
 #define GPURIR_PREC_FP16 2 /* FP16 arithmetic, FP32 accumulation (mixed_precision): FP16 */
                            /* filter inputs on the matrix cores (basis) or packed FP16  */
                            /* windowed-sinc taps (tap scatter)                          */
+#define GPURIR_PREC_BF16 3 /* BF16 filter inputs on the matrix cores, FP32 accumulation */
+                           /* (basis method only; no reference counterpart: the error   */
+                           /* table's BF16 entry)                                       */
 
 /* Accumulation of the windowed sinc, in every precision mode */
 #define GPURIR_METHOD_AUTO 0  /* library default: BASIS, or TAPS for launches of fewer    */

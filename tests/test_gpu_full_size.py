@@ -39,7 +39,7 @@ def uneven_batch():
     return W.uneven_rooms(n_rooms=4096, seed=0)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp16", 1e-3), ("lut", 1e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp16", 1e-3), ("lut", 1e-5), ("bf16", 1e-2)])
 def test_batch_4096_rooms_full_size(uneven_batch, precision, tol):
     rooms, fs = uneven_batch, W.FS
     plan = T.BatchPlan(rooms, fs, precision=precision, seed=1234, call_offset=0)

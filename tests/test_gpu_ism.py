@@ -524,3 +524,25 @@ def test_delay_fraction_rounding_up_to_one(method):
               Tmax=0.08, fs=16000.0, Tdiff=0.05)
     g = T.simulate_rir(**kw, method=method).cpu().numpy()
     check_parity(g, orc.simulate_rir(**kw), orc.n_samples(0.05, 16000.0))
+
+
+# ---- BF16 filter inputs (the error table's BF16 entry): 8-bit mantissas, so its error is
+# stated rather than held to the FP16 mode's 1e-3; bar 1e-2 here
+BF16_TOL = 1e-2
+
+
+@pytest.mark.parametrize("fs", [8000.0, 16000.0, 44100.0, 48000.0])
+def test_bf16_basis(fs):
+    cfg = W.mid_room(n_src=2, n_rcv=8)
+    cfg["fs"] = fs
+    cfg["Tmax"] = 0.12
+    cfg["Tdiff"] = 0.06
+    cfg["nb_img"] = gpuRIR.t2n(0.06, cfg["room_sz"])
+    g, o = run_both(cfg, precision="bf16")
+    e = check_parity(g, o, orc.n_samples(0.06, fs), tol=BF16_TOL)
+    g16, _ = run_both(cfg, precision="fp16", method="basis")
+    e16 = check_parity(g16, o, orc.n_samples(0.06, fs), tol=FP16_TOL)
+    print(f"fs {fs:.0f}: BF16 rel-L2 {e:.3e}, FP16 {e16:.3e}")
+    assert e > e16  # fewer mantissa bits, larger error
+    with pytest.raises(ValueError, match="BF16"):
+        run_both(cfg, precision="bf16", method="taps")

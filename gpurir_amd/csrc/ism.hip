@@ -38,6 +38,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 
 #include "internal.h"
@@ -3003,14 +3004,115 @@ void sim_set_stream_opts(SimLaunch& S, unsigned long long seed, unsigned call_of
     S.opt.call_offset = call_offset;
 }
 
+// ---- plans of recent eager launches
+// An eager launch plans its rooms on the host and uploads the plan.  Repeated calls for the
+// same rooms and options (positions and orientations are read from device memory at run
+// time, and the noise stream's seed and call counter are launch arguments) reuse the last
+// plans: per device the PLAN_CACHE most recent, each with its own device copy.  A plan's
+// device copy is only read after its upload, so launches on any stream may share it; the
+// host entry point's positions section is rewritten by each of its (synchronous) calls.
+namespace {
+struct CachedPlan {
+    std::string key;
+    SimLaunch S;
+    void* d = nullptr;
+    size_t bytes = 0;  // plan, then the host entry point's positions section
+    hipEvent_t uploaded = nullptr;  // after the plan's upload; launches on any stream wait for it
+};
+constexpr int PLAN_CACHE = 8;
+constexpr int PLAN_CACHE_DEV = 64;
+std::mutex g_plan_mu;
+std::vector<CachedPlan*> g_plans[PLAN_CACHE_DEV];  // most recently used last
+
+std::string plan_key(const std::vector<RoomHost>& rooms, const SimOptions& o, int n_pos_src, int n_pos_rcv) {
+    std::string k;
+    auto put = [&k](const void* p, size_t n) { k.append(static_cast<const char*>(p), n); };
+    for (const RoomHost& h : rooms) {
+        put(h.L, sizeof(h.L));
+        put(h.beta, sizeof(h.beta));
+        put(h.nb, sizeof(h.nb));
+        const int ints[4] = {h.n_src, h.n_rcv, h.src_off, h.rcv_off};
+        put(ints, sizeof(ints));
+        const float fl[3] = {h.Tdiff, h.Tmax, h.level};
+        put(fl, sizeof(fl));
+        put(&h.out_off, sizeof(h.out_off));
+        put(&h.room_id, sizeof(h.room_id));
+    }
+    const int oi[10] = {o.precision, o.noise, o.spkr, o.mic, o.chunk_images, o.tile_samples, o.crossfade, o.method,
+                        n_pos_src, n_pos_rcv};
+    put(oi, sizeof(oi));
+    const float of[2] = {o.fs, o.c};
+    put(of, sizeof(of));
+    return k;
+}
+
+// The cached plan of these rooms and options (planned and uploaded on a miss, with
+// extra_bytes of device memory after the plan).  The caller holds g_plan_mu while it uses
+// the entry.
+int cached_plan(const std::vector<RoomHost>& rooms, const SimOptions& opt, int n_pos_src, int n_pos_rcv,
+                size_t extra_bytes, hipStream_t stream, CachedPlan** out) {
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    GPURIR_CHECK(dev >= 0 && dev < PLAN_CACHE_DEV, "device index out of range");
+    const std::string key = plan_key(rooms, opt, n_pos_src, n_pos_rcv);
+    std::vector<CachedPlan*>& L = g_plans[dev];
+    for (size_t i = 0; i < L.size(); i++) {
+        if (L[i]->key == key) {
+            CachedPlan* c = L[i];
+            L.erase(L.begin() + i);
+            L.push_back(c);
+            GPURIR_HIP(hipStreamWaitEvent(stream, c->uploaded, 0));
+            *out = c;
+            return 0;
+        }
+    }
+    CachedPlan* c = new CachedPlan();
+    c->key = key;
+    if (int rc = sim_prepare(rooms, opt, c->S)) {
+        delete c;
+        return rc;
+    }
+    c->bytes = align_up(c->S.plan_bytes, 256) + extra_bytes;
+    if (hipMalloc(&c->d, c->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        set_error("out of device memory for an eager plan");
+        return -3;
+    }
+    int rc = upload_async(c->d, c->S.blob.data(), c->S.plan_bytes, stream);
+    if (!rc && (hipEventCreateWithFlags(&c->uploaded, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(c->uploaded, stream) != hipSuccess)) {
+        (void)hipGetLastError();
+        set_error("HIP error while recording a plan upload");
+        rc = -2;
+    }
+    if (rc) {
+        if (c->uploaded) (void)hipEventDestroy(c->uploaded);
+        (void)hipFree(c->d);
+        delete c;
+        return rc;
+    }
+    if (L.size() >= (size_t)PLAN_CACHE) {
+        CachedPlan* old = L.front();
+        L.erase(L.begin());
+        (void)hipFree(old->d);  // waits for the launches that still read it
+        (void)hipEventDestroy(old->uploaded);
+        delete old;
+    }
+    L.push_back(c);
+    *out = c;
+    return 0;
+}
+}  // namespace
+
 int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src, const float* d_pos_rcv,
                       const float* d_orv_src, const float* d_orv_rcv, const SimOptions& opt, float* d_out,
                       hipStream_t stream) {
-    SimLaunch S;
-    if (int rc = sim_prepare(rooms, opt, S)) return rc;
-    void* dplan = nullptr;
-    if (int rc = workspace(WS_PLAN, S.plan_bytes, &dplan)) return rc;
-    if (int rc = upload_async(dplan, S.blob.data(), S.plan_bytes, stream)) return rc;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    CachedPlan* c = nullptr;
+    if (int rc = cached_plan(rooms, opt, 0, 0, 0, stream, &c)) return rc;
+    SimLaunch& S = c->S;
+    sim_set_stream_opts(S, opt.seed, opt.call_offset);
     float* dpart = nullptr;
     if (S.P.part_elems > 0) {
         void* pv = nullptr;
@@ -3022,27 +3124,27 @@ int launch_simulation(const std::vector<RoomHost>& rooms, const float* d_pos_src
     // basis launches take one workgroup per item (their persistent form holds more
     // registers and fits fewer workgroups per CU; plans choose it only when timed faster)
     sim_set_counters(S, S.basis ? nullptr : counters);
-    return sim_enqueue(S, dplan, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
+    return sim_enqueue(S, c->d, dpart, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out, stream);
 }
 
 int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_src, int n_src, const float* pos_rcv,
                            int n_rcv, const float* orv_src, const float* orv_rcv, const SimOptions& opt, float* d_out,
                            hipStream_t stream) {
-    SimLaunch S;
-    if (int rc = sim_prepare(rooms, opt, S)) return rc;
-    // plan | pos_src | pos_rcv | orv_src | orv_rcv, 256-byte aligned sections
+    // after the plan: pos_src | pos_rcv | orv_src | orv_rcv, 256-byte aligned sections
     const size_t bs = align_up(sizeof(float) * 3 * (size_t)n_src, 256), br = align_up(sizeof(float) * 3 * (size_t)n_rcv, 256);
-    const size_t o_ps = align_up(S.plan_bytes, 256), o_pr = o_ps + bs, o_os = o_pr + br, o_or = o_os + bs;
-    const size_t total = o_or + br;
-    S.blob.resize(total, 0);
-    unsigned char* h = S.blob.data();
-    std::memcpy(h + o_ps, pos_src, sizeof(float) * 3 * n_src);
-    std::memcpy(h + o_pr, pos_rcv, sizeof(float) * 3 * n_rcv);
-    if (orv_src) std::memcpy(h + o_os, orv_src, sizeof(float) * 3 * n_src);
-    if (orv_rcv) std::memcpy(h + o_or, orv_rcv, sizeof(float) * 3 * n_rcv);
-    void* dplan = nullptr;
-    if (int rc = workspace(WS_PLAN, total, &dplan)) return rc;
-    if (int rc = upload_async(dplan, h, total, stream)) return rc;
+    const size_t o_pr = bs, o_os = o_pr + br, o_or = o_os + bs, total = o_or + br;
+    std::vector<unsigned char> h(total, 0);
+    std::memcpy(h.data(), pos_src, sizeof(float) * 3 * n_src);
+    std::memcpy(h.data() + o_pr, pos_rcv, sizeof(float) * 3 * n_rcv);
+    if (orv_src) std::memcpy(h.data() + o_os, orv_src, sizeof(float) * 3 * n_src);
+    if (orv_rcv) std::memcpy(h.data() + o_or, orv_rcv, sizeof(float) * 3 * n_rcv);
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    CachedPlan* c = nullptr;
+    if (int rc = cached_plan(rooms, opt, n_src, n_rcv, total, stream, &c)) return rc;
+    SimLaunch& S = c->S;
+    sim_set_stream_opts(S, opt.seed, opt.call_offset);
+    unsigned char* d = static_cast<unsigned char*>(c->d) + align_up(S.plan_bytes, 256);
+    if (int rc = upload_async(d, h.data(), total, stream)) return rc;
     float* dpart = nullptr;
     if (S.P.part_elems > 0) {
         void* pv = nullptr;
@@ -3051,11 +3153,8 @@ int launch_simulation_host(const std::vector<RoomHost>& rooms, const float* pos_
     }
     unsigned* counters = nullptr;
     if (int rc = device_counters(&counters)) return rc;
-    // basis launches take one workgroup per item (their persistent form holds more
-    // registers and fits fewer workgroups per CU; plans choose it only when timed faster)
     sim_set_counters(S, S.basis ? nullptr : counters);
-    const unsigned char* d = static_cast<const unsigned char*>(dplan);
-    return sim_enqueue(S, dplan, dpart, reinterpret_cast<const float*>(d + o_ps), reinterpret_cast<const float*>(d + o_pr),
+    return sim_enqueue(S, c->d, dpart, reinterpret_cast<const float*>(d), reinterpret_cast<const float*>(d + o_pr),
                        reinterpret_cast<const float*>(d + o_os), reinterpret_cast<const float*>(d + o_or), d_out, stream);
 }
 

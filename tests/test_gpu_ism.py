@@ -546,3 +546,30 @@ def test_bf16_basis(fs):
     assert e > e16  # fewer mantissa bits, larger error
     with pytest.raises(ValueError, match="BF16"):
         run_both(cfg, precision="bf16", method="taps")
+
+
+def test_eager_plan_cache_reuse_and_invalidation():
+    """Eager calls reuse the plan of their rooms and options: new positions each call (read from
+    device memory), a changed beta (a new plan), and a call on another stream right after the
+    call that planned and uploaded it, all against the oracle."""
+    base = W.mid_room(n_src=2, n_rcv=8)
+    kw = dict(room_sz=base["room_sz"], nb_img=base["nb_img"], Tmax=0.15, fs=base["fs"], Tdiff=0.1,
+              spkr_pattern="hypcard", mic_pattern="card")
+    for seed in range(3):
+        c = W.mid_room(seed=seed, n_src=2, n_rcv=8)
+        for beta in (base["beta"], base["beta"] * 0.9):
+            args = dict(kw, beta=beta, pos_src=c["pos_src"], pos_rcv=c["pos_rcv"], orV_src=c["orV_src"],
+                        orV_rcv=c["orV_rcv"])
+            g = T.simulate_rir(**args).cpu().numpy()
+            check_parity(g, orc.simulate_rir(**args), orc.n_samples(0.1, kw["fs"]))
+    c = W.mid_room(seed=7, n_src=2, n_rcv=8)
+    args = dict(kw, beta=base["beta"] * 0.8, pos_src=c["pos_src"], pos_rcv=c["pos_rcv"], orV_src=c["orV_src"],
+                orV_rcv=c["orV_rcv"])
+    first = T.simulate_rir(**args)  # plans and uploads on the current stream
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        second = T.simulate_rir(**args)  # reuses the plan on another stream
+    torch.cuda.synchronize()
+    want = orc.simulate_rir(**args)
+    check_parity(first.cpu().numpy(), want, orc.n_samples(0.1, kw["fs"]))
+    assert torch.equal(first, second)

@@ -306,16 +306,22 @@ def main():
         # accuracy of the batch against the FP64 oracle (exact sinc, same Philox stream)
         plan.run()
         torch.cuda.synchronize()
-        errs = []
+        errs, errs_mode = [], []
         for k in range(3):
             r = rooms[k]
             o = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"], fs,
                                  Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k)
-            errs.append(rel_l2(plan.rirs[k].cpu().numpy(), o))
+            g = plan.rirs[k].cpu().numpy()
+            errs.append(rel_l2(g, o))
+            if precision == "lut":  # and against the oracle's own table mode
+                ol = orc.simulate_rir(r["room_sz"], r["beta"], r["pos_src"], r["pos_rcv"], r["nb_img"], r["Tmax"],
+                                      fs, Tdiff=r["Tdiff"], seed=1234, call_offset=0, room_index=k, mode=orc.LUT)
+                errs_mode.append(rel_l2(g, ol))
         res = {"precision": precision, "method": method, "autotune": tune,
                "rirs": n_rirs, "taps": taps, "ms_per_step": ms_step,
                "rirs_per_s": n_rirs / (ms_step / 1e3), "accumulate_ms": ms_acc, "finalize_ms": ms_fin,
                "tflops_algorithmic": FLOP_PER_TAP * taps / (ms_acc / 1e3) / 1e12, "rel_l2_vs_fp64": max(errs),
+               "rel_l2_vs_oracle_table_mode": max(errs_mode) if errs_mode else None,
                "out_gb": plan.out.numel() * 4 / 1e9}
         log(f"[bench] batched {precision}: {n_rirs} RIRs, W = {taps:.4e} taps, {ms_step:.3f} ms/step, "
             f"{res['rirs_per_s']:.4e} RIRs/s; accumulate {ms_acc:.3f} ms ({res['tflops_algorithmic']:.2f} TFLOP/s "

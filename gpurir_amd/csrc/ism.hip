@@ -764,6 +764,18 @@ __device__ inline int wave_scan_incl(int x) {
     return x;
 }
 
+// Largest of non-negative integers (e.g. the bits of non-negative floats) over a wave, in lane
+// 63: the prefix maximum by the same DPP moves as wave_scan_incl (lanes shifted in give 0).
+__device__ inline int wave_max_nonneg(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+
 // Exclusive prefix sum of one int per thread over the WGB threads; *total gets the sum.
 // Every thread must call it (two barriers); tmp holds NWB ints.
 __device__ inline int block_scan_excl(int v, int* tmp, int* total) {
@@ -1082,7 +1094,13 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     else fv[m][sl] = d < DL ? (float)((double)D[m * DL + d] * (double)inv_scale) : 0.0f;
                     mx = fmaxf(mx, fabsf(fv[m][sl]));
                 }
-            if (H16) atomicMax(misc + 12, __float_as_int(mx));  // non-negative floats order as ints
+            if (H16) {
+                // non-negative floats order as ints; one atomic per wave (atomics of the lanes
+                // of a wave on one LDS word run one lane at a time: 512 per tile had cost ~1 ms
+                // of the 4096-room batch)
+                const int wm = wave_max_nonneg(__float_as_int(mx));
+                if (lane == 63) atomicMax(misc + 12, wm);
+            }
             __syncthreads();
             float sc = 1.0f;
             if (H16) {

@@ -402,6 +402,339 @@ __global__ __launch_bounds__(256) void ola_kernel(const float* __restrict__ conv
     }
 }
 
+// ---- Segment-aligned partitioned convolution (the trajectory's usual case) ----------------
+//
+// When the segments have S or S + 1 samples (simulateTrajectory's own segmentation) the
+// output is cut at the segment starts o_m = w_ini[m] (continued past the signal in steps of
+// S_max) and each RIR of point p into blocks at the same boundaries: block j of RIR p covers
+// RIR samples [o_{p+j} - o_p, o_{p+j+1} - o_p).  Every product x_p * block_j(h_p) then starts
+// at the same window start o_{p+j}, so window m's result is
+//   y_m = IFFT( sum_p X_p . FFT(block_{m-p}(h_p)) )
+// with one inverse transform per window, and it is shorter than o_{m+2} - o_m (blocks and
+// segments differ by at most one sample).  Output block [o_b, o_{b+1}) is
+// y_b[0 : S_b] + y_{b-1}[S_{b-1} :], a two-term FP64 sum.  Windows are taken in pairs: the
+// blocks of windows 2J and 2J + 1 go into the real and imaginary parts of one complex
+// transform of length N = 128 V >= 2 S_max; the products with the (Hermitian) segment
+// spectra keep the two apart, so the inverse gives y_2J + i y_2J+1.
+//
+// A transform unit is 128 threads holding V values each: element tau + 128 i in register i.
+// Forward: DFT-V over i in registers, twiddles W_N^(tau k2), then a DFT-128 over tau as
+// 16 x 8 (DFT-8 over tau >> 4 after an LDS exchange, twiddles W_128^(a c), DFT-16 over
+// tau & 15 after a second exchange, the DFT-16 split over two lanes as two DFT-8s and one
+// radix-2 step through a DPP swap).  The spectrum stays in that unit's own (permuted)
+// register layout: segment spectra are stored in it and the inverse is the exact adjoint of
+// the forward steps in reverse order, so no reordering pass is needed.  The XOR-swizzled
+// LDS positions keep every 32-lane group of each exchange on distinct 8-byte bank pairs.
+constexpr int UT = 128;
+
+struct WinArgs {
+    const float* signal;
+    const int* w_ini;      // [P + 1]
+    const float* rirs;     // [P][R][L]
+    const float2* tw;      // [N] W_N^m
+    float2* spec;          // [P][V][UT] segment spectra / N, unit register layout
+    const int* p_lo;       // [n_jobs] first point of window pair J
+    const int* ola_blk0;   // [n_ola_wg] block of each OLA chunk's first sample
+    float* ybuf;           // [2 n_jobs][R][ystride] window results
+    double* out;           // [n_out][R]
+    int P, R, L, n_sig, s_max, n_jobs, per_xcd, ystride;
+    long long n_out;
+};
+
+__device__ __forceinline__ long long win_start(const WinArgs& a, int m) {
+    return m <= a.P ? (long long)a.w_ini[m] : (long long)a.n_sig + (long long)(m - a.P) * a.s_max;
+}
+
+__device__ __forceinline__ int e1_pos(int k2, int n1) { return k2 * UT + (n1 ^ ((k2 & 1) << 4)); }
+__device__ __forceinline__ int e2_pos(int k2, int c, int a) {
+    return k2 * UT + ((c * 16 + a) ^ (((k2 & 1) << 4) ^ ((c >> 1) << 1) ^ ((c & 1) << 3)));
+}
+
+__device__ __forceinline__ float swap_xor1(float v) {  // value of lane ^ 1 (quad_perm 1,0,3,2)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float2 swap_ri(float2 v) { return make_float2(v.y, v.x); }
+
+// Raw buffer view of n floats: loads past the end return 0 without a branch (the range
+// check of the buffer resource), which is how zero-padded blocks are read.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t float_range(const float* p, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, n * (int)sizeof(float), 0x00020000);
+}
+__device__ __forceinline__ float load_or_zero(__amdgpu_buffer_rsrc_t r, int lane_i, int const_i) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane_i * (int)sizeof(float),
+                                                                          const_i * (int)sizeof(float), 0));
+}
+
+// v, returned through an empty asm that the compiler cannot see into: addresses derived from
+// the result are recomputed where it is used, and cannot be hoisted out of a loop or above
+// the point where dep is ready.  Keeps the thread's loop-invariant LDS and twiddle addresses
+// and its spectrum loads inside the loop, where their registers are briefly live; hoisted
+// they hold ~100 VGPRs across the whole loop and cut the occupancy to one or two waves per SIMD.
+__device__ __forceinline__ int pinned_v(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ int pinned_after(int v, float dep) {
+    asm volatile("" : "+v"(v) : "v"(dep));
+    return v;
+}
+
+// Good-Thomas DFT-24 = 3 x 8 (coprime, no twiddles): input x[(8 n1 + 3 n2) mod 24],
+// output X[(16 k1 + 9 k2) mod 24].
+__device__ __forceinline__ void dft24(float2* v) {
+    float2 b[3][8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; n2++) {
+        float2 t[3] = {v[(3 * n2) % 24], v[(8 + 3 * n2) % 24], v[(16 + 3 * n2) % 24]};
+        dft3(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; k1++) b[k1][n2] = t[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 3; k1++) dft_reg<8>(b[k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) v[(16 * k1 + 9 * k2) % 24] = b[k1][k2];
+}
+
+template <int V>
+__device__ __forceinline__ void dft_v(float2* v) {
+    if constexpr (V == 24)
+        dft24(v);
+    else
+        dft_reg<V>(v);
+}
+
+// Unnormalized inverse (conjugate) DFT: swap(DFT(swap(v))), swap(x + iy) = y + ix.
+template <int R, typename F>
+__device__ __forceinline__ void conj_dft(float2* v, F dft) {
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = swap_ri(v[k]);
+    dft(v);
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = swap_ri(v[k]);
+}
+
+// W16^e for the radix-2 step of the lane-pair DFT-16 (odd lanes), 1 on even lanes.
+__device__ __forceinline__ float2 lane_w16(int e, bool odd) {
+    return odd ? make_float2(W16C[e], W16S[e]) : make_float2(1.0f, 0.0f);
+}
+
+// Forward transform of one unit, in place: z[i] = element tau + 128 i on entry, the spectrum
+// in the unit's register layout on exit.  buf: V * 128 float2 of LDS owned by the unit.  All
+// threads of the workgroup call it together (it synchronizes the workgroup).
+template <int V>
+__device__ __forceinline__ void unit_forward(float2 (&z)[V], float2* buf, const float2* __restrict__ tw, int tau) {
+    tau = pinned_v(tau);
+    dft_v<V>(z);
+    const int tb = pinned_after(tau, z[V - 1].x);  // twiddle loads after the DFT
+#pragma unroll
+    for (int k2 = 1; k2 < V; k2++) z[k2] = cmul(z[k2], tw[tb * k2]);
+    __syncthreads();  // the previous transform's last reads of buf are done
+#pragma unroll
+    for (int k2 = 0; k2 < V; k2++) buf[e1_pos(k2, tau)] = z[k2];
+    __syncthreads();
+    const int a = tau & 15, g = tau >> 4;
+    float2 w[8];
+#pragma unroll
+    for (int c = 1; c < 8; c++) w[c] = tw[V * a * c];
+#pragma unroll
+    for (int t = 0; t < V / 8; t++) {
+        float2 u[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) u[b] = buf[e1_pos(g + 8 * t, a + 16 * b)];
+        dft_reg<8>(u);
+        z[8 * t] = u[0];
+#pragma unroll
+        for (int c = 1; c < 8; c++) z[8 * t + c] = cmul(u[c], w[c]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < V / 8; t++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) buf[e2_pos(g + 8 * t, c, a)] = z[8 * t + c];
+    __syncthreads();
+    const bool odd = tau & 1;
+    const float sgn = odd ? -1.0f : 1.0f;
+    const int cc = (tau >> 1) & 7, gg = tau >> 4;
+#pragma unroll
+    for (int t = 0; t < V / 8; t++) {
+        float2 x[8];
+#pragma unroll
+        for (int ap = 0; ap < 8; ap++) x[ap] = buf[e2_pos(gg + 8 * t, cc, 2 * ap + (odd ? 1 : 0))];
+        dft_reg<8>(x);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float2 m = e == 0 ? x[e] : cmul(x[e], lane_w16(e, odd));
+            const float2 r = make_float2(swap_xor1(m.x), swap_xor1(m.y));
+            z[8 * t + e] = make_float2(fmaf(sgn, m.x, r.x), fmaf(sgn, m.y, r.y));  // even: P0 + T, odd: P0 - T
+        }
+    }
+}
+
+// Adjoint of unit_forward (N times the inverse DFT), in place: the spectrum in the unit's
+// register layout on entry, element tau + 128 i in z[i] on exit.
+template <int V>
+__device__ __forceinline__ void unit_inverse(float2 (&z)[V], float2* buf, const float2* __restrict__ tw, int tau) {
+    tau = pinned_v(tau);
+    const bool odd = tau & 1;
+    const float sgn = odd ? -1.0f : 1.0f;
+    const int cc = (tau >> 1) & 7, gg = tau >> 4;
+    __syncthreads();  // the previous transform's last reads of buf are done
+#pragma unroll
+    for (int t = 0; t < V / 8; t++) {
+        float2 x[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float2 m = z[8 * t + e];
+            const float2 r = make_float2(swap_xor1(m.x), swap_xor1(m.y));
+            const float2 q = make_float2(fmaf(sgn, m.x, r.x), fmaf(sgn, m.y, r.y));  // even: X0 + X1, odd: X0 - X1
+            x[e] = e == 0 ? q : cmulc(q, lane_w16(e, odd));
+        }
+        conj_dft<8>(x, [](float2* v) { dft_reg<8>(v); });
+#pragma unroll
+        for (int ap = 0; ap < 8; ap++) buf[e2_pos(gg + 8 * t, cc, 2 * ap + (odd ? 1 : 0))] = x[ap];
+    }
+    __syncthreads();
+    const int a = tau & 15, g = tau >> 4;
+    float2 w[8];
+#pragma unroll
+    for (int c = 1; c < 8; c++) w[c] = tw[V * a * c];
+#pragma unroll
+    for (int t = 0; t < V / 8; t++) {
+        float2 u[8];
+        u[0] = buf[e2_pos(g + 8 * t, 0, a)];
+#pragma unroll
+        for (int c = 1; c < 8; c++) u[c] = cmulc(buf[e2_pos(g + 8 * t, c, a)], w[c]);
+        conj_dft<8>(u, [](float2* v) { dft_reg<8>(v); });
+#pragma unroll
+        for (int b = 0; b < 8; b++) z[8 * t + b] = u[b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < V / 8; t++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) buf[e1_pos(g + 8 * t, a + 16 * b)] = z[8 * t + b];
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < V; k2++) z[k2] = buf[e1_pos(k2, tau)];
+#pragma unroll
+    for (int k2 = 1; k2 < V; k2++) z[k2] = cmulc(z[k2], tw[tau * k2]);
+    conj_dft<V>(z, [](float2* v) { dft_v<V>(v); });
+}
+
+// One 128-thread workgroup per segment p: spec[p] = FFT_N(x_p zero-padded) / N.
+template <int V>
+__global__ __launch_bounds__(UT) void window_spectrum_kernel(WinArgs a) {
+    __shared__ float2 buf[V * UT];
+    const int p = blockIdx.x, tau = threadIdx.x;
+    const int w0 = a.w_ini[p], len = a.w_ini[p + 1] - w0;
+    const __amdgpu_buffer_rsrc_t x = float_range(a.signal + w0, len);
+    float2 z[V];
+#pragma unroll
+    for (int i = 0; i < V; i++) z[i] = make_float2(load_or_zero(x, tau, UT * i), 0.0f);
+    unit_forward<V>(z, buf, a.tw, tau);
+    const float sc = 1.0f / (float)(V * UT);
+    float2* S = a.spec + (size_t)p * V * UT;
+#pragma unroll
+    for (int i = 0; i < V; i++) S[i * UT + tau] = make_float2(z[i].x * sc, z[i].y * sc);
+}
+
+// One 256-thread workgroup per (window pair J, receiver r): two units take alternate points
+// of the pair, each accumulates sum_p X_p . FFT(block pair) and transforms its sum back; the
+// two halves are added through LDS (unit 0 writes window 2J, unit 1 window 2J + 1).
+// Workgroup b runs on XCD b mod 8, which takes a contiguous range of jobs, so the segment
+// spectra a range needs stay in that XCD's L2.
+template <int V>
+__global__ __launch_bounds__(2 * UT) void window_conv_kernel(WinArgs a) {
+    __shared__ float2 bufs[2][V * UT];
+    const int q = (int)(blockIdx.x & 7) * a.per_xcd + (int)(blockIdx.x >> 3);
+    if (q >= a.n_jobs * a.R) return;  // whole workgroup
+    const int J = q / a.R, r = q - J * a.R;
+    const int u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);  // unit: the same for a whole wave
+    const int tau = threadIdx.x & (UT - 1);
+    float2* buf = bufs[u];
+    const int m0 = 2 * J;
+    const int p_lo = a.p_lo[J], p_hi = min(a.P - 1, m0 + 1);
+    const int n_it = p_hi >= p_lo ? (p_hi - p_lo + 2) / 2 : 0;
+    const long long o0 = win_start(a, m0), o1 = win_start(a, m0 + 1), o2 = win_start(a, m0 + 2);
+    float2 acc[V];
+#pragma unroll
+    for (int i = 0; i < V; i++) acc[i] = make_float2(0.0f, 0.0f);
+    for (int it = 0; it < n_it; it++) {
+        const int p = p_lo + 2 * it + u;
+        const bool act = p <= p_hi;  // the same for the whole unit
+        float2 z[V];
+        if (act) {
+            const float* h = a.rirs + ((size_t)p * a.R + r) * a.L;
+            const long long op = a.w_ini[p];
+            const long long s0 = o0 - op, s1 = o1 - op;
+            const int len0 = (p <= m0 && s0 < a.L) ? (int)(min(o1 - op, (long long)a.L) - s0) : 0;
+            const int len1 = s1 < a.L ? (int)(min(o2 - op, (long long)a.L) - s1) : 0;
+            const __amdgpu_buffer_rsrc_t h0 = float_range(h + (len0 > 0 ? s0 : 0), len0);
+            const __amdgpu_buffer_rsrc_t h1 = float_range(h + (len1 > 0 ? s1 : 0), len1);
+#pragma unroll
+            for (int i = 0; i < V; i++) z[i] = make_float2(load_or_zero(h0, tau, UT * i), load_or_zero(h1, tau, UT * i));
+        } else {
+#pragma unroll
+            for (int i = 0; i < V; i++) z[i] = make_float2(0.0f, 0.0f);
+        }
+        unit_forward<V>(z, buf, a.tw, tau);
+        if (act) {
+            const float2* X = a.spec + (size_t)p * V * UT + pinned_after(tau, z[V - 1].x);
+#pragma unroll
+            for (int i = 0; i < V; i++) {
+                const float2 s = X[i * UT];
+                acc[i].x = fmaf(z[i].x, s.x, fmaf(-z[i].y, s.y, acc[i].x));
+                acc[i].y = fmaf(z[i].x, s.y, fmaf(z[i].y, s.x, acc[i].y));
+            }
+        }
+    }
+    unit_inverse<V>(acc, buf, a.tw, tau);
+    __syncthreads();  // both units' last reads of their buffers are done
+    float* mine = reinterpret_cast<float*>(bufs[u]);
+#pragma unroll
+    for (int i = 0; i < V; i++) mine[i * UT + tau] = u == 0 ? acc[i].y : acc[i].x;
+    __syncthreads();
+    const float* other = reinterpret_cast<const float*>(bufs[u ^ 1]);
+    const int m = m0 + u;
+    const int ylen = (int)(win_start(a, m + 2) - win_start(a, m));
+    float* y = a.ybuf + ((size_t)m * a.R + r) * a.ystride;
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+        const int n = tau + UT * i;
+        if (n < ylen) y[n] = (u == 0 ? acc[i].x : acc[i].y) + other[i * UT + tau];
+    }
+}
+
+// out[t][r] = y_b[t - o_b] + y_{b-1}[t - o_{b-1}] in FP64 for the block b holding t; each
+// workgroup owns 1024 consecutive (t, r) elements, starting its block search at ola_blk0.
+constexpr int WOLA_E = 1024;
+
+__global__ __launch_bounds__(256) void window_ola_kernel(WinArgs a) {
+    const long long total = a.n_out * a.R;
+    const int n_win = 2 * a.n_jobs;
+    int b = a.ola_blk0[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < WOLA_E / 256; k++) {
+        const long long e = (long long)blockIdx.x * WOLA_E + k * 256 + threadIdx.x;
+        if (e >= total) break;
+        const long long t = e / a.R;
+        const int r = (int)(e - t * a.R);
+        while (win_start(a, b + 1) <= t) b++;
+        const long long ob = win_start(a, b);
+        double v = 0.0;
+        if (b < n_win) v = (double)a.ybuf[((size_t)b * a.R + r) * a.ystride + (t - ob)];
+        if (b >= 1 && b - 1 < n_win) {
+            const long long op = win_start(a, b - 1);
+            v += (double)a.ybuf[((size_t)(b - 1) * a.R + r) * a.ystride + (t - op)];
+        }
+        a.out[e] = v;
+    }
+}
+
 struct FftTables {
     std::vector<float2> tw, twN;
 };
@@ -561,6 +894,96 @@ int run_conv(const float* d_src, const std::vector<int>& seg_start, const std::v
     return 0;
 }
 
+// The segment-aligned partitioned convolution applies when the segments start at 0 and have
+// S or S + 1 samples with 256 <= S_max <= 1536 (transforms of 1024, 2048 or 3072 points).
+// Shorter segments would pad each block to at least 1024 points, longer ones need larger
+// transforms: those go through run_conv and the gather overlap-add.
+bool window_path_applies(const int* w_ini, int P) {
+    if (w_ini[0] != 0) return false;
+    int s_min = w_ini[1] - w_ini[0], s_max = s_min;
+    for (int p = 1; p < P; p++) {
+        const int s = w_ini[p + 1] - w_ini[p];
+        s_min = std::min(s_min, s);
+        s_max = std::max(s_max, s);
+    }
+    return s_min >= 1 && s_max - s_min <= 1 && s_max >= 256 && 2 * s_max <= 24 * UT;
+}
+
+template <int V>
+int launch_window(WinArgs& a, int n_ola_wg, hipStream_t st) {
+    hipLaunchKernelGGL(window_spectrum_kernel<V>, dim3((unsigned)a.P), dim3(UT), 0, st, a);
+    GPURIR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(window_conv_kernel<V>, dim3((unsigned)(8 * a.per_xcd)), dim3(2 * UT), 0, st, a);
+    GPURIR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(window_ola_kernel, dim3((unsigned)n_ola_wg), dim3(256), 0, st, a);
+    GPURIR_HIP(hipGetLastError());
+    return 0;
+}
+
+int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, int P, int R, int L,
+                      const int* w_ini, double* d_out, hipStream_t st) {
+    int s_max = 0;
+    for (int p = 0; p < P; p++) s_max = std::max(s_max, w_ini[p + 1] - w_ini[p]);
+    const int V = 2 * s_max <= 8 * UT ? 8 : 2 * s_max <= 16 * UT ? 16 : 24;
+    const float2 *d_tw = nullptr, *d_twN = nullptr;
+    if (int rc = fft_tables_device(V * UT, &d_tw, &d_twN)) return rc;
+    auto o = [&](long long m) -> long long {
+        return m <= P ? (long long)w_ini[m] : (long long)n_signal + (m - P) * (long long)s_max;
+    };
+    const long long n_out = (long long)n_signal + L - 1;
+    long long M = P;  // windows 0 .. M-1 receive products
+    while (o(M) - o(P - 1) < L) M++;
+    const long long n_jobs = (M + 1) / 2;
+    GPURIR_CHECK(n_jobs * R < (1LL << 30) && n_out * R / WOLA_E < (1LL << 30), "trajectory too long");
+    std::vector<int> p_lo((size_t)n_jobs);
+    for (long long J = 0, p = 0; J < n_jobs; J++) {
+        while (p < P && o(2 * J) - o(p) >= L) p++;
+        p_lo[(size_t)J] = (int)p;
+    }
+    const long long total = n_out * R;
+    const int n_ola_wg = (int)((total + WOLA_E - 1) / WOLA_E);
+    std::vector<int> blk0((size_t)n_ola_wg);
+    for (long long w = 0, b = 0; w < n_ola_wg; w++) {
+        const long long t0 = w * WOLA_E / R;
+        while (o(b + 1) <= t0) b++;
+        blk0[(size_t)w] = (int)b;
+    }
+    const int ystride = (2 * s_max + 63) / 64 * 64;
+    const size_t b_w = al(sizeof(int) * (P + 1)), b_j = al(sizeof(int) * n_jobs), b_o = al(sizeof(int) * n_ola_wg);
+    const size_t b_spec = al(sizeof(float2) * (size_t)P * V * UT);
+    const size_t b_y = al(sizeof(float) * (size_t)(2 * n_jobs) * R * ystride);
+    void* ws = nullptr;
+    if (int rc = workspace(WS_CONV, b_w + b_j + b_o + b_spec + b_y, &ws)) return rc;
+    unsigned char* w = static_cast<unsigned char*>(ws);
+    std::vector<unsigned char> host(b_w + b_j + b_o, 0);
+    std::memcpy(host.data(), w_ini, sizeof(int) * (P + 1));
+    std::memcpy(host.data() + b_w, p_lo.data(), sizeof(int) * n_jobs);
+    std::memcpy(host.data() + b_w + b_j, blk0.data(), sizeof(int) * n_ola_wg);
+    if (int rc = upload_async(w, host.data(), host.size(), st)) return rc;
+    WinArgs a;
+    a.signal = d_signal;
+    a.w_ini = reinterpret_cast<const int*>(w);
+    a.rirs = d_rirs;
+    a.tw = d_tw;
+    a.p_lo = reinterpret_cast<const int*>(w + b_w);
+    a.ola_blk0 = reinterpret_cast<const int*>(w + b_w + b_j);
+    a.spec = reinterpret_cast<float2*>(w + b_w + b_j + b_o);
+    a.ybuf = reinterpret_cast<float*>(w + b_w + b_j + b_o + b_spec);
+    a.out = d_out;
+    a.P = P;
+    a.R = R;
+    a.L = L;
+    a.n_sig = n_signal;
+    a.s_max = s_max;
+    a.n_jobs = (int)n_jobs;
+    a.per_xcd = (int)((n_jobs * R + 7) / 8);
+    a.ystride = ystride;
+    a.n_out = n_out;
+    if (V == 8) return launch_window<8>(a, n_ola_wg, st);
+    if (V == 16) return launch_window<16>(a, n_ola_wg, st);
+    return launch_window<24>(a, n_ola_wg, st);
+}
+
 }  // namespace
 }  // namespace gpurir
 
@@ -603,6 +1026,8 @@ int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d
         sl[p] = w_ini[p + 1] - w_ini[p];
     }
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (window_path_applies(w_ini, n_pts))
+        return window_trajectory(d_signal, n_signal, d_rirs, n_pts, n_rcv, rir_len, w_ini, d_out, st);
     float* conv = nullptr;
     int stride = 0;
     void* extra = nullptr;

@@ -68,6 +68,28 @@ def test_trajectory_default_timestamps():
     assert rel_l2(got, want) < 1e-5
 
 
+# Segment-aligned partitioned convolution (segments of S or S + 1 samples, 256 <= S <= 1536:
+# transforms of 1024, 2048 and 3072 points): S at the limits of each transform, RIRs shorter
+# than a segment and many segments long, one point, one-tap RIRs; S = 255 and 1537 take the
+# general path next to it.
+@pytest.mark.parametrize("n,P,R,L", [(16000, 16, 3, 2000), (10007, 8, 2, 3001), (4096, 16, 2, 5000),
+                                     (8192, 16, 1, 700), (8208, 16, 2, 900), (16384, 16, 2, 333),
+                                     (24576, 16, 2, 4100), (1300, 1, 4, 2500), (3000, 2, 2, 1),
+                                     (4080, 16, 2, 600), (24592, 16, 2, 600)])
+def test_trajectory_segment_aligned(n, P, R, L):
+    rng = np.random.default_rng(n + P + L)
+    sig = rng.standard_normal(n).astype(np.float32)
+    rirs = (rng.standard_normal((P, R, L)) * np.exp(-np.arange(L) / (L / 3.0 + 1.0))).astype(np.float32)
+    d_rirs = torch.as_tensor(rirs, device="cuda")
+    got = T.trajectory(sig, d_rirs)
+    again = T.trajectory(sig, d_rirs)
+    torch.cuda.synchronize()
+    assert torch.equal(got, again), "two calls differ"
+    want = orc.simulate_trajectory(sig, rirs)
+    assert tuple(got.shape) == want.shape
+    assert rel_l2(got.cpu().numpy(), want) < 1e-5
+
+
 def test_trajectory_uneven_custom_timestamps():
     rng = np.random.default_rng(5)
     fs = 16000.0

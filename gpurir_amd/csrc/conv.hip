@@ -33,6 +33,10 @@
 namespace gpurir {
 namespace {
 
+// The transforms in this file may contract a * b + c into FMAs (the library is built with
+// -ffp-contract=off for the image delays, which must round like the CPU oracle's).
+#pragma clang fp contract(fast)
+
 constexpr int CWG = 512;
 
 struct ConvArgs {
@@ -434,10 +438,11 @@ struct WinArgs {
     const float2* tw;      // [N] W_N^m
     float2* spec;          // [P][V][UT] segment spectra / N, unit register layout
     const int* p_lo;       // [n_jobs] first point of window pair J
+    const int* job_order;  // [n_conv_wg] job J * R + r of each workgroup, -1: none
     const int* ola_blk0;   // [n_ola_wg] block of each OLA chunk's first sample
     float* ybuf;           // [2 n_jobs][R][ystride] window results
     double* out;           // [n_out][R]
-    int P, R, L, n_sig, s_max, n_jobs, per_xcd, ystride;
+    int P, R, L, n_sig, s_max, n_jobs, n_conv_wg, ystride;
     long long n_out;
 };
 
@@ -451,7 +456,7 @@ __device__ __forceinline__ int e2_pos(int k2, int c, int a) {
 }
 
 __device__ __forceinline__ float swap_xor1(float v) {  // value of lane ^ 1 (quad_perm 1,0,3,2)
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float2 swap_ri(float2 v) { return make_float2(v.y, v.x); }
@@ -643,15 +648,14 @@ __global__ __launch_bounds__(UT) void window_spectrum_kernel(WinArgs a) {
 }
 
 // One 256-thread workgroup per (window pair J, receiver r): two units take alternate points
-// of the pair, each accumulates sum_p X_p . FFT(block pair) and transforms its sum back; the
-// two halves are added through LDS (unit 0 writes window 2J, unit 1 window 2J + 1).
-// Workgroup b runs on XCD b mod 8, which takes a contiguous range of jobs, so the segment
-// spectra a range needs stay in that XCD's L2.
+// of the pair and each accumulates sum_p X_p . FFT(block pair); unit 1 then hands its sum to
+// unit 0 through LDS and ends, and unit 0 transforms the total back (s_barrier waits only for
+// the waves that have not ended): window 2J is the real part, window 2J + 1 the imaginary.
 template <int V>
 __global__ __launch_bounds__(2 * UT) void window_conv_kernel(WinArgs a) {
     __shared__ float2 bufs[2][V * UT];
-    const int q = (int)(blockIdx.x & 7) * a.per_xcd + (int)(blockIdx.x >> 3);
-    if (q >= a.n_jobs * a.R) return;  // whole workgroup
+    const int q = a.job_order[blockIdx.x];
+    if (q < 0) return;  // whole workgroup
     const int J = q / a.R, r = q - J * a.R;
     const int u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);  // unit: the same for a whole wave
     const int tau = threadIdx.x & (UT - 1);
@@ -692,46 +696,67 @@ __global__ __launch_bounds__(2 * UT) void window_conv_kernel(WinArgs a) {
             }
         }
     }
-    unit_inverse<V>(acc, buf, a.tw, tau);
-    __syncthreads();  // both units' last reads of their buffers are done
-    float* mine = reinterpret_cast<float*>(bufs[u]);
+    __syncthreads();  // the last transform's reads of both buffers are done
+    if (u == 1) {
 #pragma unroll
-    for (int i = 0; i < V; i++) mine[i * UT + tau] = u == 0 ? acc[i].y : acc[i].x;
+        for (int i = 0; i < V; i++) bufs[1][i * UT + tau] = acc[i];
+    }
     __syncthreads();
-    const float* other = reinterpret_cast<const float*>(bufs[u ^ 1]);
-    const int m = m0 + u;
-    const int ylen = (int)(win_start(a, m + 2) - win_start(a, m));
-    float* y = a.ybuf + ((size_t)m * a.R + r) * a.ystride;
+    if (u == 1) return;
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+        const float2 o = bufs[1][i * UT + tau];
+        acc[i] = make_float2(acc[i].x + o.x, acc[i].y + o.y);
+    }
+    unit_inverse<V>(acc, bufs[0], a.tw, tau);
+    const int ylen0 = (int)(o2 - o0), ylen1 = (int)(win_start(a, m0 + 3) - o1);
+    float* y0 = a.ybuf + ((size_t)m0 * a.R + r) * a.ystride;
+    float* y1 = y0 + (size_t)a.R * a.ystride;
 #pragma unroll
     for (int i = 0; i < V; i++) {
         const int n = tau + UT * i;
-        if (n < ylen) y[n] = (u == 0 ? acc[i].x : acc[i].y) + other[i * UT + tau];
+        if (n < ylen0) y0[n] = acc[i].x;
+        if (n < ylen1) y1[n] = acc[i].y;
     }
 }
 
-// out[t][r] = y_b[t - o_b] + y_{b-1}[t - o_{b-1}] in FP64 for the block b holding t; each
-// workgroup owns 1024 consecutive (t, r) elements, starting its block search at ola_blk0.
-constexpr int WOLA_E = 1024;
+// out[t][r] = y_b[t - o_b] + y_{b-1}[t - o_{b-1}] in FP64 for the block b holding t.  A
+// workgroup owns WOLA_T consecutive samples (fewer than a segment, so at most two blocks:
+// ola_blk0[wg] and the next) and all receivers, eight at a time: the reads run along the
+// window rows, the sums go through LDS so that the [t][r] stores are contiguous.
+constexpr int WOLA_T = 128;
 
 __global__ __launch_bounds__(256) void window_ola_kernel(WinArgs a) {
-    const long long total = a.n_out * a.R;
+    __shared__ double tile[WOLA_T * 9];
+    const long long t0 = (long long)blockIdx.x * WOLA_T;
+    const int nt = (int)min((long long)WOLA_T, a.n_out - t0);
     const int n_win = 2 * a.n_jobs;
-    int b = a.ola_blk0[blockIdx.x];
+    const int b0 = a.ola_blk0[blockIdx.x];
+    const long long om = b0 >= 1 ? win_start(a, b0 - 1) : 0, o0 = win_start(a, b0), o1 = win_start(a, b0 + 1);
+    const int i = threadIdx.x & (WOLA_T - 1), j0 = threadIdx.x >> 7;
+    const long long t = t0 + i;
+    const int b = t >= o1 ? b0 + 1 : b0;
+    const long long ob = b == b0 ? o0 : o1, op = b == b0 ? om : o0;  // o_b, o_{b-1}
+    const bool head = b < n_win && i < nt, tail = b >= 1 && b - 1 < n_win && i < nt;
+    for (int r0 = 0; r0 < a.R; r0 += 8) {
+        const int rc = min(8, a.R - r0);
 #pragma unroll
-    for (int k = 0; k < WOLA_E / 256; k++) {
-        const long long e = (long long)blockIdx.x * WOLA_E + k * 256 + threadIdx.x;
-        if (e >= total) break;
-        const long long t = e / a.R;
-        const int r = (int)(e - t * a.R);
-        while (win_start(a, b + 1) <= t) b++;
-        const long long ob = win_start(a, b);
-        double v = 0.0;
-        if (b < n_win) v = (double)a.ybuf[((size_t)b * a.R + r) * a.ystride + (t - ob)];
-        if (b >= 1 && b - 1 < n_win) {
-            const long long op = win_start(a, b - 1);
-            v += (double)a.ybuf[((size_t)(b - 1) * a.R + r) * a.ystride + (t - op)];
+        for (int k = 0; k < 4; k++) {
+            const int j = j0 + 2 * k;
+            if (j < rc) {
+                const int r = r0 + j;
+                double v = 0.0;
+                if (head) v = (double)a.ybuf[((size_t)b * a.R + r) * a.ystride + (t - ob)];
+                if (tail) v += (double)a.ybuf[((size_t)(b - 1) * a.R + r) * a.ystride + (t - op)];
+                tile[i * 9 + j] = v;
+            }
         }
-        a.out[e] = v;
+        __syncthreads();
+        for (int e = threadIdx.x; e < nt * rc; e += 256) {
+            const int ii = e / rc, jj = e - ii * rc;
+            a.out[(t0 + ii) * a.R + r0 + jj] = tile[ii * 9 + jj];
+        }
+        __syncthreads();
     }
 }
 
@@ -913,7 +938,7 @@ template <int V>
 int launch_window(WinArgs& a, int n_ola_wg, hipStream_t st) {
     hipLaunchKernelGGL(window_spectrum_kernel<V>, dim3((unsigned)a.P), dim3(UT), 0, st, a);
     GPURIR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(window_conv_kernel<V>, dim3((unsigned)(8 * a.per_xcd)), dim3(2 * UT), 0, st, a);
+    hipLaunchKernelGGL(window_conv_kernel<V>, dim3((unsigned)a.n_conv_wg), dim3(2 * UT), 0, st, a);
     GPURIR_HIP(hipGetLastError());
     hipLaunchKernelGGL(window_ola_kernel, dim3((unsigned)n_ola_wg), dim3(256), 0, st, a);
     GPURIR_HIP(hipGetLastError());
@@ -934,31 +959,62 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     long long M = P;  // windows 0 .. M-1 receive products
     while (o(M) - o(P - 1) < L) M++;
     const long long n_jobs = (M + 1) / 2;
-    GPURIR_CHECK(n_jobs * R < (1LL << 30) && n_out * R / WOLA_E < (1LL << 30), "trajectory too long");
+    GPURIR_CHECK(n_jobs * R < (1LL << 30) && n_out / WOLA_T < (1LL << 30), "trajectory too long");
     std::vector<int> p_lo((size_t)n_jobs);
     for (long long J = 0, p = 0; J < n_jobs; J++) {
         while (p < P && o(2 * J) - o(p) >= L) p++;
         p_lo[(size_t)J] = (int)p;
     }
-    const long long total = n_out * R;
-    const int n_ola_wg = (int)((total + WOLA_E - 1) / WOLA_E);
+    // Jobs in dispatch order.  Workgroup b runs on XCD b mod 8; each XCD takes a contiguous
+    // range of the full-cost window pairs (the segment spectra of a range stay in its L2),
+    // then its share of the cheaper pairs at the ends of the signal, cheapest last: when the
+    // jobs do not divide evenly over the CUs, the workgroups dispatched last are the ones that
+    // share a CU with two others, and they are then the shortest.
+    std::vector<int> n_it((size_t)n_jobs);
+    int it_max = 0;
+    for (long long J = 0; J < n_jobs; J++) {
+        const int hi = (int)std::min<long long>(P - 1, 2 * J + 1);
+        n_it[(size_t)J] = hi >= p_lo[(size_t)J] ? (hi - p_lo[(size_t)J] + 2) / 2 : 0;
+        it_max = std::max(it_max, n_it[(size_t)J]);
+    }
+    std::vector<int> full, edge;
+    for (long long J = 0; J < n_jobs; J++) (n_it[(size_t)J] == it_max ? full : edge).push_back((int)J);
+    std::stable_sort(edge.begin(), edge.end(), [&](int x, int y) { return n_it[(size_t)x] > n_it[(size_t)y]; });
+    std::vector<std::vector<int>> per_xcd(8);
+    const size_t chunk = (full.size() * R + 7) / 8;
+    for (size_t k = 0; k < full.size() * R; k++) per_xcd[k / std::max<size_t>(chunk, 1)].push_back(full[k / R] * R + (int)(k % R));
+    for (size_t k = 0; k < edge.size() * R; k++) {
+        size_t x = 0;  // the XCD with the fewest jobs so far
+        for (size_t y = 1; y < 8; y++)
+            if (per_xcd[y].size() < per_xcd[x].size()) x = y;
+        per_xcd[x].push_back(edge[k / R] * R + (int)(k % R));
+    }
+    size_t slots = 0;
+    for (auto& v : per_xcd) slots = std::max(slots, v.size());
+    std::vector<int> order(8 * slots, -1);
+    for (size_t x = 0; x < 8; x++)
+        for (size_t k = 0; k < per_xcd[x].size(); k++) order[8 * k + x] = per_xcd[x][k];
+    const int n_conv_wg = (int)order.size();
+    const int n_ola_wg = (int)((n_out + WOLA_T - 1) / WOLA_T);
     std::vector<int> blk0((size_t)n_ola_wg);
     for (long long w = 0, b = 0; w < n_ola_wg; w++) {
-        const long long t0 = w * WOLA_E / R;
+        const long long t0 = w * WOLA_T;
         while (o(b + 1) <= t0) b++;
         blk0[(size_t)w] = (int)b;
     }
     const int ystride = (2 * s_max + 63) / 64 * 64;
     const size_t b_w = al(sizeof(int) * (P + 1)), b_j = al(sizeof(int) * n_jobs), b_o = al(sizeof(int) * n_ola_wg);
+    const size_t b_q = al(sizeof(int) * n_conv_wg);
     const size_t b_spec = al(sizeof(float2) * (size_t)P * V * UT);
     const size_t b_y = al(sizeof(float) * (size_t)(2 * n_jobs) * R * ystride);
     void* ws = nullptr;
-    if (int rc = workspace(WS_CONV, b_w + b_j + b_o + b_spec + b_y, &ws)) return rc;
+    if (int rc = workspace(WS_CONV, b_w + b_j + b_o + b_q + b_spec + b_y, &ws)) return rc;
     unsigned char* w = static_cast<unsigned char*>(ws);
-    std::vector<unsigned char> host(b_w + b_j + b_o, 0);
+    std::vector<unsigned char> host(b_w + b_j + b_o + b_q, 0);
     std::memcpy(host.data(), w_ini, sizeof(int) * (P + 1));
     std::memcpy(host.data() + b_w, p_lo.data(), sizeof(int) * n_jobs);
     std::memcpy(host.data() + b_w + b_j, blk0.data(), sizeof(int) * n_ola_wg);
+    std::memcpy(host.data() + b_w + b_j + b_o, order.data(), sizeof(int) * n_conv_wg);
     if (int rc = upload_async(w, host.data(), host.size(), st)) return rc;
     WinArgs a;
     a.signal = d_signal;
@@ -967,8 +1023,9 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     a.tw = d_tw;
     a.p_lo = reinterpret_cast<const int*>(w + b_w);
     a.ola_blk0 = reinterpret_cast<const int*>(w + b_w + b_j);
-    a.spec = reinterpret_cast<float2*>(w + b_w + b_j + b_o);
-    a.ybuf = reinterpret_cast<float*>(w + b_w + b_j + b_o + b_spec);
+    a.job_order = reinterpret_cast<const int*>(w + b_w + b_j + b_o);
+    a.spec = reinterpret_cast<float2*>(w + b_w + b_j + b_o + b_q);
+    a.ybuf = reinterpret_cast<float*>(w + b_w + b_j + b_o + b_q + b_spec);
     a.out = d_out;
     a.P = P;
     a.R = R;
@@ -976,7 +1033,7 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     a.n_sig = n_signal;
     a.s_max = s_max;
     a.n_jobs = (int)n_jobs;
-    a.per_xcd = (int)((n_jobs * R + 7) / 8);
+    a.n_conv_wg = n_conv_wg;
     a.ystride = ystride;
     a.n_out = n_out;
     if (V == 8) return launch_window<8>(a, n_ola_wg, st);

@@ -435,7 +435,7 @@ struct WinArgs {
     const float* signal;
     const int* w_ini;      // [P + 1]
     const float* rirs;     // [P][R][L]
-    const float2* tw;      // [N] W_N^m
+    const float2* tw;      // [V + 8][UT] unit twiddles: W_N^(tau k), then W_128^((tau & 15) c)
     float2* spec;          // [P][V][UT] segment spectra / N, unit register layout
     const int* p_lo;       // [n_jobs] first point of window pair J
     const int* job_order;  // [n_conv_wg] job J * R + r of each workgroup, -1: none
@@ -480,6 +480,10 @@ __device__ __forceinline__ int pinned_v(int v) {
     asm volatile("" : "+v"(v));
     return v;
 }
+__device__ __forceinline__ float2 pinned_f2(float2 v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y));
+    return v;
+}
 __device__ __forceinline__ int pinned_after(int v, float dep) {
     asm volatile("" : "+v"(v) : "v"(dep));
     return v;
@@ -504,6 +508,95 @@ __device__ __forceinline__ void dft24(float2* v) {
         for (int k2 = 0; k2 < 8; k2++) v[(16 * k1 + 9 * k2) % 24] = b[k1][k2];
 }
 
+// The same transforms for input whose upper half is known to be zero (v[R/2 ..] unread): the
+// zero-padded blocks fill at most half of a transform (N >= 2 S_max).
+template <bool Z0, bool Z1, bool Z2>
+__device__ __forceinline__ void dft3z(float2* v) {  // Zj: v[j] is zero
+    constexpr float H3 = 0.86602540378443865f;
+    const float2 zero = make_float2(0.0f, 0.0f);
+    if constexpr (Z1 && Z2) {
+        const float2 a = Z0 ? zero : v[0];
+        v[0] = a;
+        v[1] = a;
+        v[2] = a;
+    } else {
+        float2 t1, t2;
+        if constexpr (Z1) {
+            t1 = v[2];
+            t2 = make_float2(-v[2].x, -v[2].y);
+        } else if constexpr (Z2) {
+            t1 = v[1];
+            t2 = v[1];
+        } else {
+            t1 = cadd(v[1], v[2]);
+            t2 = csub(v[1], v[2]);
+        }
+        float2 m;
+        if constexpr (Z0) {
+            m = make_float2(-0.5f * t1.x, -0.5f * t1.y);
+            v[0] = t1;
+        } else {
+            const float2 a = v[0];
+            m = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+            v[0] = cadd(a, t1);
+        }
+        const float2 d = make_float2(H3 * t2.x, H3 * t2.y);
+        v[1] = make_float2(m.x + d.y, m.y - d.x);
+        v[2] = make_float2(m.x - d.y, m.y + d.x);
+    }
+}
+
+template <int N2 = 0>
+__device__ __forceinline__ void pfa24_low_cols(const float2* v, float2 (&b)[3][8]) {
+    if constexpr (N2 < 8) {
+        constexpr int i0 = (3 * N2) % 24, i1 = (8 + 3 * N2) % 24, i2 = (16 + 3 * N2) % 24;
+        float2 t[3] = {i0 < 12 ? v[i0] : make_float2(0.0f, 0.0f), i1 < 12 ? v[i1] : make_float2(0.0f, 0.0f),
+                       i2 < 12 ? v[i2] : make_float2(0.0f, 0.0f)};
+        dft3z<(i0 >= 12), (i1 >= 12), (i2 >= 12)>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; k1++) b[k1][N2] = t[k1];
+        pfa24_low_cols<N2 + 1>(v, b);
+    }
+}
+
+__device__ __forceinline__ void dft24_low(float2* v) {
+    float2 b[3][8];
+    pfa24_low_cols(v, b);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; k1++) dft_reg<8>(b[k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) v[(16 * k1 + 9 * k2) % 24] = b[k1][k2];
+}
+
+template <int R, int I = 0>
+__device__ __forceinline__ void dif_first_low(float2* v) {  // first DIF stage, v[R/2 ..] zero
+    if constexpr (I < R / 2) {
+        v[I + R / 2] = w16mul<I * (16 / R)>(v[I]);
+        dif_first_low<R, I + 1>(v);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_reg_low(float2* v) {
+    dif_first_low<R>(v);
+    dif_stages<R, R / 4>(v);
+    float2 t[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) t[k] = v[brev_c<R>(k)];
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = t[k];
+}
+
+template <int V>
+__device__ __forceinline__ void dft_v_low(float2* v) {
+    if constexpr (V == 24)
+        dft24_low(v);
+    else
+        dft_reg_low<V>(v);
+}
+
 template <int V>
 __device__ __forceinline__ void dft_v(float2* v) {
     if constexpr (V == 24)
@@ -522,34 +615,87 @@ __device__ __forceinline__ void conj_dft(float2* v, F dft) {
     for (int k = 0; k < R; k++) v[k] = swap_ri(v[k]);
 }
 
+// The thread's twiddles, made in registers from V/8 + 1 values loaded once per kernel instead
+// of 30 table loads per transform (whose latency the transform cannot hide): W_N^(tau k) for
+// k < V applied as W_N^(tau j) then W_N^(8 tau h), k = 8 h + j, and W_128^(a c),
+// a = tau mod 16, c < 8, with the powers of W_N^tau and W_128^a built by squaring (at most
+// four roundings more than a table entry).
+template <int V>
+struct UnitSeeds {
+    float2 w1, wa, wh[V / 8];  // W_N^tau, W_128^a, wh[h] = W_N^(8 h tau) for h >= 1
+};
+
+template <int V>
+__device__ __forceinline__ UnitSeeds<V> load_seeds(const float2* __restrict__ tw, int tau) {
+    UnitSeeds<V> sd;
+    const float2* ut = tw + tau;
+    sd.w1 = ut[UT];
+    sd.wa = ut[(V + 1) * UT];
+#pragma unroll
+    for (int h = 1; h < V / 8; h++) sd.wh[h] = ut[8 * h * UT];
+    return sd;
+}
+
+// p[j] = w^j for j = 1..7 (p[0] unused), made where it is used (not hoisted out of a loop,
+// where the powers would hold 14 VGPRs throughout)
+__device__ __forceinline__ void powers7(float2 w, float2 (&p)[8]) {
+    w = pinned_f2(w);
+    p[1] = w;
+    p[2] = cmul(w, w);
+    p[3] = cmul(p[2], w);
+    p[4] = cmul(p[2], p[2]);
+    p[5] = cmul(p[4], w);
+    p[6] = cmul(p[4], p[2]);
+    p[7] = cmul(p[4], p[3]);
+}
+
+// z[k] *= W_N^(tau k) (CONJ: by its conjugate), k = 8 h + j, as z[k] * W_N^(tau j) * W_N^(8 tau h)
+template <int V, bool CONJ>
+__device__ __forceinline__ void tau_twiddles(float2 (&z)[V], const UnitSeeds<V>& sd) {
+    float2 lo[8];
+    powers7(sd.w1, lo);
+#pragma unroll
+    for (int k = 1; k < V; k++) {
+        const int j = k & 7, h = k >> 3;
+        if (j != 0) z[k] = CONJ ? cmulc(z[k], lo[j]) : cmul(z[k], lo[j]);
+        if (h != 0) z[k] = CONJ ? cmulc(z[k], pinned_f2(sd.wh[h])) : cmul(z[k], pinned_f2(sd.wh[h]));
+    }
+}
+
 // W16^e for the radix-2 step of the lane-pair DFT-16 (odd lanes), 1 on even lanes.
 __device__ __forceinline__ float2 lane_w16(int e, bool odd) {
     return odd ? make_float2(W16C[e], W16S[e]) : make_float2(1.0f, 0.0f);
 }
 
-// Forward transform of one unit, in place: z[i] = element tau + 128 i on entry, the spectrum
-// in the unit's register layout on exit.  buf: V * 128 float2 of LDS owned by the unit.  All
-// threads of the workgroup call it together (it synchronizes the workgroup).
+// Forward transform of one unit, in two parts.  unit_forward_front: z[i] = element tau + 128 i
+// on entry, the elements past N/2 zero (z[V/2 ..] is not read); runs the DFT-V, the twiddles, the DFT-8 over tau >> 4 and both exchanges, and
+// leaves the last stage's input in buf (z is free on return).  unit_forward_back: the lane-pair
+// DFT-16 over tau & 15, one chunk of eight outputs at a time, each handed to sink(t, out) --
+// out[e] is register 8 t + e of the spectrum in the unit's layout.  buf: V * 128 float2 of LDS
+// owned by the unit.  All threads of the workgroup call the front together (it synchronizes
+// the workgroup); the back only reads buf.
 template <int V>
-__device__ __forceinline__ void unit_forward(float2 (&z)[V], float2* buf, const float2* __restrict__ tw, int tau) {
+__device__ __forceinline__ void unit_forward_front(float2 (&z)[V], float2* buf, const UnitSeeds<V>& sd,
+                                                   const float2* __restrict__ tw, int tau) {
     tau = pinned_v(tau);
-    dft_v<V>(z);
-    const int tb = pinned_after(tau, z[V - 1].x);  // twiddle loads after the DFT
+    dft_v_low<V>(z);
+    tau_twiddles<V, false>(z, sd);
+    const int a = tau & 15, g = tau >> 4;
+    float2 w[8];
+    const float2* ut = tw + tau;
 #pragma unroll
-    for (int k2 = 1; k2 < V; k2++) z[k2] = cmul(z[k2], tw[tb * k2]);
+    for (int c = 1; c < 8; c++) w[c] = ut[(V + c) * UT];  // in flight across the barriers
     __syncthreads();  // the previous transform's last reads of buf are done
 #pragma unroll
     for (int k2 = 0; k2 < V; k2++) buf[e1_pos(k2, tau)] = z[k2];
     __syncthreads();
-    const int a = tau & 15, g = tau >> 4;
-    float2 w[8];
-#pragma unroll
-    for (int c = 1; c < 8; c++) w[c] = tw[V * a * c];
 #pragma unroll
     for (int t = 0; t < V / 8; t++) {
+        // chunk t's reads after chunk t - 1's results: one chunk of temporaries at a time
+        const int gt = t == 0 ? g : pinned_after(g, z[8 * t - 1].x);
         float2 u[8];
 #pragma unroll
-        for (int b = 0; b < 8; b++) u[b] = buf[e1_pos(g + 8 * t, a + 16 * b)];
+        for (int b = 0; b < 8; b++) u[b] = buf[e1_pos(gt + 8 * t, a + 16 * b)];
         dft_reg<8>(u);
         z[8 * t] = u[0];
 #pragma unroll
@@ -561,6 +707,10 @@ __device__ __forceinline__ void unit_forward(float2 (&z)[V], float2* buf, const 
 #pragma unroll
         for (int c = 0; c < 8; c++) buf[e2_pos(g + 8 * t, c, a)] = z[8 * t + c];
     __syncthreads();
+}
+
+template <int V, typename F>
+__device__ __forceinline__ void unit_forward_back(const float2* buf, int tau, F&& sink) {
     const bool odd = tau & 1;
     const float sgn = odd ? -1.0f : 1.0f;
     const int cc = (tau >> 1) & 7, gg = tau >> 4;
@@ -570,19 +720,32 @@ __device__ __forceinline__ void unit_forward(float2 (&z)[V], float2* buf, const 
 #pragma unroll
         for (int ap = 0; ap < 8; ap++) x[ap] = buf[e2_pos(gg + 8 * t, cc, 2 * ap + (odd ? 1 : 0))];
         dft_reg<8>(x);
+        float2 out[8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const float2 m = e == 0 ? x[e] : cmul(x[e], lane_w16(e, odd));
             const float2 r = make_float2(swap_xor1(m.x), swap_xor1(m.y));
-            z[8 * t + e] = make_float2(fmaf(sgn, m.x, r.x), fmaf(sgn, m.y, r.y));  // even: P0 + T, odd: P0 - T
+            out[e] = make_float2(fmaf(sgn, m.x, r.x), fmaf(sgn, m.y, r.y));  // even: P0 + T, odd: P0 - T
         }
+        sink(t, out);
     }
+}
+
+// The whole forward transform, in place: the spectrum in the unit's register layout on exit.
+template <int V>
+__device__ __forceinline__ void unit_forward(float2 (&z)[V], float2* buf, const UnitSeeds<V>& sd,
+                                             const float2* __restrict__ tw, int tau) {
+    unit_forward_front<V>(z, buf, sd, tw, tau);
+    unit_forward_back<V>(buf, tau, [&](int t, const float2 (&out)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) z[8 * t + e] = out[e];
+    });
 }
 
 // Adjoint of unit_forward (N times the inverse DFT), in place: the spectrum in the unit's
 // register layout on entry, element tau + 128 i in z[i] on exit.
 template <int V>
-__device__ __forceinline__ void unit_inverse(float2 (&z)[V], float2* buf, const float2* __restrict__ tw, int tau) {
+__device__ __forceinline__ void unit_inverse(float2 (&z)[V], float2* buf, const UnitSeeds<V>& sd, int tau) {
     tau = pinned_v(tau);
     const bool odd = tau & 1;
     const float sgn = odd ? -1.0f : 1.0f;
@@ -605,8 +768,7 @@ __device__ __forceinline__ void unit_inverse(float2 (&z)[V], float2* buf, const 
     __syncthreads();
     const int a = tau & 15, g = tau >> 4;
     float2 w[8];
-#pragma unroll
-    for (int c = 1; c < 8; c++) w[c] = tw[V * a * c];
+    powers7(sd.wa, w);
 #pragma unroll
     for (int t = 0; t < V / 8; t++) {
         float2 u[8];
@@ -625,8 +787,7 @@ __device__ __forceinline__ void unit_inverse(float2 (&z)[V], float2* buf, const 
     __syncthreads();
 #pragma unroll
     for (int k2 = 0; k2 < V; k2++) z[k2] = buf[e1_pos(k2, tau)];
-#pragma unroll
-    for (int k2 = 1; k2 < V; k2++) z[k2] = cmulc(z[k2], tw[tau * k2]);
+    tau_twiddles<V, true>(z, sd);
     conj_dft<V>(z, [](float2* v) { dft_v<V>(v); });
 }
 
@@ -636,11 +797,11 @@ __global__ __launch_bounds__(UT) void window_spectrum_kernel(WinArgs a) {
     __shared__ float2 buf[V * UT];
     const int p = blockIdx.x, tau = threadIdx.x;
     const int w0 = a.w_ini[p], len = a.w_ini[p + 1] - w0;
-    const __amdgpu_buffer_rsrc_t x = float_range(a.signal + w0, len);
+    const __amdgpu_buffer_rsrc_t x = float_range(a.signal + w0, len);  // len <= N / 2
     float2 z[V];
 #pragma unroll
-    for (int i = 0; i < V; i++) z[i] = make_float2(load_or_zero(x, tau, UT * i), 0.0f);
-    unit_forward<V>(z, buf, a.tw, tau);
+    for (int i = 0; i < V / 2; i++) z[i] = make_float2(load_or_zero(x, tau, UT * i), 0.0f);
+    unit_forward<V>(z, buf, load_seeds<V>(a.tw, tau), a.tw, tau);
     const float sc = 1.0f / (float)(V * UT);
     float2* S = a.spec + (size_t)p * V * UT;
 #pragma unroll
@@ -651,8 +812,10 @@ __global__ __launch_bounds__(UT) void window_spectrum_kernel(WinArgs a) {
 // of the pair and each accumulates sum_p X_p . FFT(block pair); unit 1 then hands its sum to
 // unit 0 through LDS and ends, and unit 0 transforms the total back (s_barrier waits only for
 // the waves that have not ended): window 2J is the real part, window 2J + 1 the imaginary.
+// amdgpu_waves_per_eu(3): at most 168 VGPRs, three waves per SIMD (three workgroups per CU,
+// as the LDS allows); unconstrained, the compiler takes ~173 and halves the waves per SIMD.
 template <int V>
-__global__ __launch_bounds__(2 * UT) void window_conv_kernel(WinArgs a) {
+__global__ __launch_bounds__(2 * UT) __attribute__((amdgpu_waves_per_eu(3))) void window_conv_kernel(WinArgs a) {
     __shared__ float2 bufs[2][V * UT];
     const int q = a.job_order[blockIdx.x];
     if (q < 0) return;  // whole workgroup
@@ -664,36 +827,63 @@ __global__ __launch_bounds__(2 * UT) void window_conv_kernel(WinArgs a) {
     const int p_lo = a.p_lo[J], p_hi = min(a.P - 1, m0 + 1);
     const int n_it = p_hi >= p_lo ? (p_hi - p_lo + 2) / 2 : 0;
     const long long o0 = win_start(a, m0), o1 = win_start(a, m0 + 1), o2 = win_start(a, m0 + 2);
+    // RIR blocks of point p for windows m0 (real part) and m0 + 1 (imaginary part), zero
+    // past their ends and for points outside [p_lo, p_hi]; at most S_max <= N / 2 samples
+    auto load_blocks = [&](int p, float2 (&z)[V]) {
+        int len0 = 0, len1 = 0;
+        long long s0 = 0, s1 = 0;
+        if (p <= p_hi) {
+            const long long op = a.w_ini[p];
+            s0 = o0 - op;
+            s1 = o1 - op;
+            len0 = (p <= m0 && s0 < a.L) ? (int)(min(o1 - op, (long long)a.L) - s0) : 0;
+            len1 = s1 < a.L ? (int)(min(o2 - op, (long long)a.L) - s1) : 0;
+        }
+        const float* h = a.rirs + ((size_t)min(p, p_hi) * a.R + r) * a.L;
+        const __amdgpu_buffer_rsrc_t h0 = float_range(h + (len0 > 0 ? s0 : 0), len0);
+        const __amdgpu_buffer_rsrc_t h1 = float_range(h + (len1 > 0 ? s1 : 0), len1);
+#pragma unroll
+        for (int i = 0; i < V / 2; i++) z[i] = make_float2(load_or_zero(h0, tau, UT * i), load_or_zero(h1, tau, UT * i));
+    };
+    const UnitSeeds<V> sd = load_seeds<V>(a.tw, tau);
     float2 acc[V];
 #pragma unroll
     for (int i = 0; i < V; i++) acc[i] = make_float2(0.0f, 0.0f);
+    float2 z[V];
+    if (n_it > 0) load_blocks(p_lo + u, z);
+    // software pipeline: the next point's blocks load while the last stage of this point's
+    // transform runs and its products accumulate
     for (int it = 0; it < n_it; it++) {
         const int p = p_lo + 2 * it + u;
         const bool act = p <= p_hi;  // the same for the whole unit
-        float2 z[V];
+        unit_forward_front<V>(z, buf, sd, a.tw, tau);
+        // the spectrum of point p (two chunks of it), then the next point's blocks: loads return
+        // in issue order, so the first chunks do not wait for the HBM reads of the blocks
+        // the spectrum of point p chunk by chunk: chunk 0 before the next point's blocks (loads
+        // return in issue order, so it does not wait for their HBM reads), chunk t + 1 at the end
+        // of chunk t
+        const float2* X = a.spec + (size_t)p * V * UT;
+        float2 xs[8];
         if (act) {
-            const float* h = a.rirs + ((size_t)p * a.R + r) * a.L;
-            const long long op = a.w_ini[p];
-            const long long s0 = o0 - op, s1 = o1 - op;
-            const int len0 = (p <= m0 && s0 < a.L) ? (int)(min(o1 - op, (long long)a.L) - s0) : 0;
-            const int len1 = s1 < a.L ? (int)(min(o2 - op, (long long)a.L) - s1) : 0;
-            const __amdgpu_buffer_rsrc_t h0 = float_range(h + (len0 > 0 ? s0 : 0), len0);
-            const __amdgpu_buffer_rsrc_t h1 = float_range(h + (len1 > 0 ? s1 : 0), len1);
+            const float2* X0 = X + pinned_v(tau);
 #pragma unroll
-            for (int i = 0; i < V; i++) z[i] = make_float2(load_or_zero(h0, tau, UT * i), load_or_zero(h1, tau, UT * i));
-        } else {
-#pragma unroll
-            for (int i = 0; i < V; i++) z[i] = make_float2(0.0f, 0.0f);
+            for (int e = 0; e < 8; e++) xs[e] = X0[e * UT];
         }
-        unit_forward<V>(z, buf, a.tw, tau);
+        if (it + 1 < n_it) load_blocks(p + 2, z);
         if (act) {
-            const float2* X = a.spec + (size_t)p * V * UT + pinned_after(tau, z[V - 1].x);
+            unit_forward_back<V>(buf, tau, [&](int t, const float2 (&out)[8]) {
 #pragma unroll
-            for (int i = 0; i < V; i++) {
-                const float2 s = X[i * UT];
-                acc[i].x = fmaf(z[i].x, s.x, fmaf(-z[i].y, s.y, acc[i].x));
-                acc[i].y = fmaf(z[i].x, s.y, fmaf(z[i].y, s.x, acc[i].y));
-            }
+                for (int e = 0; e < 8; e++) {
+                    float2& c = acc[8 * t + e];
+                    c.x = fmaf(out[e].x, xs[e].x, fmaf(-out[e].y, xs[e].y, c.x));
+                    c.y = fmaf(out[e].x, xs[e].y, fmaf(out[e].y, xs[e].x, c.y));
+                }
+                if (t + 1 < V / 8) {
+                    const float2* Xn = X + 8 * (t + 1) * UT + pinned_after(tau, out[7].x);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) xs[e] = Xn[e * UT];
+                }
+            });
         }
     }
     __syncthreads();  // the last transform's reads of both buffers are done
@@ -708,7 +898,7 @@ __global__ __launch_bounds__(2 * UT) void window_conv_kernel(WinArgs a) {
         const float2 o = bufs[1][i * UT + tau];
         acc[i] = make_float2(acc[i].x + o.x, acc[i].y + o.y);
     }
-    unit_inverse<V>(acc, bufs[0], a.tw, tau);
+    unit_inverse<V>(acc, bufs[0], sd, tau);
     const int ylen0 = (int)(o2 - o0), ylen1 = (int)(win_start(a, m0 + 3) - o1);
     float* y0 = a.ybuf + ((size_t)m0 * a.R + r) * a.ystride;
     float* y1 = y0 + (size_t)a.R * a.ystride;
@@ -805,6 +995,39 @@ int fft_tables_device(int H, const float2** tw, const float2** twN) {
     }
     *tw = f->second;
     *twN = f->second + std::max(H, 1);
+    return 0;
+}
+
+// Twiddles of a transform unit of N = 128 V points in the order its threads load them:
+// row k < V holds W_N^(tau k) for tau = 0..127, row V + c holds W_128^((tau mod 16) c), so that
+// every twiddle load of a wave reads 512 contiguous bytes.  Built once per (device, V).
+std::map<std::pair<int, int>, float2*> g_unit_tables;
+
+int unit_tables_device(int V, const float2** tw) {
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    auto key = std::make_pair(dev, V);
+    auto f = g_unit_tables.find(key);
+    if (f == g_unit_tables.end()) {
+        const int N = V * UT;
+        std::vector<float2> T((size_t)(V + 8) * UT);
+        for (int k = 0; k < V + 8; k++)
+            for (int t = 0; t < UT; t++) {
+                const long long m = k < V ? (long long)t * k : (long long)V * (t & 15) * (k - V);
+                const double ang = -2.0 * M_PI * (double)(m % N) / (double)N;
+                T[(size_t)k * UT + t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+            }
+        float2* d = nullptr;
+        if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(float2) * T.size()) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("out of device memory for the FFT tables");
+            return -3;
+        }
+        GPURIR_HIP(hipMemcpy(d, T.data(), sizeof(float2) * T.size(), hipMemcpyHostToDevice));
+        f = g_unit_tables.emplace(key, d).first;
+    }
+    *tw = f->second;
     return 0;
 }
 
@@ -950,8 +1173,8 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     int s_max = 0;
     for (int p = 0; p < P; p++) s_max = std::max(s_max, w_ini[p + 1] - w_ini[p]);
     const int V = 2 * s_max <= 8 * UT ? 8 : 2 * s_max <= 16 * UT ? 16 : 24;
-    const float2 *d_tw = nullptr, *d_twN = nullptr;
-    if (int rc = fft_tables_device(V * UT, &d_tw, &d_twN)) return rc;
+    const float2* d_tw = nullptr;
+    if (int rc = unit_tables_device(V, &d_tw)) return rc;
     auto o = [&](long long m) -> long long {
         return m <= P ? (long long)w_ini[m] : (long long)n_signal + (m - P) * (long long)s_max;
     };

@@ -1168,13 +1168,26 @@ int launch_window(WinArgs& a, int n_ola_wg, hipStream_t st) {
     return 0;
 }
 
-int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, int P, int R, int L,
-                      const int* w_ini, double* d_out, hipStream_t st) {
+// The device tables of one segmentation (P, R, L, n_signal, w_ini): w_ini | p_lo | ola_blk0 |
+// job_order, kept per device while calls repeat the segmentation, so that such a call plans
+// nothing on the host and uploads nothing.  A different segmentation replaces them (the old
+// buffer is freed, which waits for the work that still reads it).
+struct WindowPlan {
+    std::vector<int> key;
+    void* d = nullptr;
+    size_t bytes = 0;
+    hipEvent_t uploaded = nullptr;  // recorded on up_stream after the upload
+    hipStream_t up_stream = nullptr;
+    int V = 0, s_max = 0, n_jobs = 0, n_conv_wg = 0, n_ola_wg = 0, ystride = 0;
+    size_t off_j = 0, off_o = 0, off_q = 0;
+};
+std::mutex g_win_mu;
+std::map<int, WindowPlan> g_win_plans;
+
+int make_window_plan(WindowPlan& W, int n_signal, int P, int R, int L, const int* w_ini, hipStream_t st) {
     int s_max = 0;
     for (int p = 0; p < P; p++) s_max = std::max(s_max, w_ini[p + 1] - w_ini[p]);
     const int V = 2 * s_max <= 8 * UT ? 8 : 2 * s_max <= 16 * UT ? 16 : 24;
-    const float2* d_tw = nullptr;
-    if (int rc = unit_tables_device(V, &d_tw)) return rc;
     auto o = [&](long long m) -> long long {
         return m <= P ? (long long)w_ini[m] : (long long)n_signal + (m - P) * (long long)s_max;
     };
@@ -1225,43 +1238,99 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
         while (o(b + 1) <= t0) b++;
         blk0[(size_t)w] = (int)b;
     }
-    const int ystride = (2 * s_max + 63) / 64 * 64;
     const size_t b_w = al(sizeof(int) * (P + 1)), b_j = al(sizeof(int) * n_jobs), b_o = al(sizeof(int) * n_ola_wg);
     const size_t b_q = al(sizeof(int) * n_conv_wg);
-    const size_t b_spec = al(sizeof(float2) * (size_t)P * V * UT);
-    const size_t b_y = al(sizeof(float) * (size_t)(2 * n_jobs) * R * ystride);
-    void* ws = nullptr;
-    if (int rc = workspace(WS_CONV, b_w + b_j + b_o + b_q + b_spec + b_y, &ws)) return rc;
-    unsigned char* w = static_cast<unsigned char*>(ws);
-    std::vector<unsigned char> host(b_w + b_j + b_o + b_q, 0);
+    const size_t bytes = b_w + b_j + b_o + b_q;
+    if (W.bytes < bytes) {
+        if (W.d) GPURIR_HIP(hipFree(W.d));  // waits for the work that still reads the old tables
+        W.d = nullptr;
+        W.bytes = 0;
+        if (hipMalloc(&W.d, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            W.d = nullptr;
+            W.key.clear();
+            set_error("out of device memory for the trajectory tables");
+            return -3;
+        }
+        W.bytes = bytes;
+    }
+    std::vector<unsigned char> host(bytes, 0);
     std::memcpy(host.data(), w_ini, sizeof(int) * (P + 1));
     std::memcpy(host.data() + b_w, p_lo.data(), sizeof(int) * n_jobs);
     std::memcpy(host.data() + b_w + b_j, blk0.data(), sizeof(int) * n_ola_wg);
     std::memcpy(host.data() + b_w + b_j + b_o, order.data(), sizeof(int) * n_conv_wg);
-    if (int rc = upload_async(w, host.data(), host.size(), st)) return rc;
+    if (int rc = upload_async(W.d, host.data(), bytes, st)) {
+        W.key.clear();
+        return rc;
+    }
+    if (!W.uploaded) GPURIR_HIP(hipEventCreateWithFlags(&W.uploaded, hipEventDisableTiming));
+    GPURIR_HIP(hipEventRecord(W.uploaded, st));
+    W.up_stream = st;
+    W.V = V;
+    W.s_max = s_max;
+    W.n_jobs = (int)n_jobs;
+    W.n_conv_wg = n_conv_wg;
+    W.n_ola_wg = n_ola_wg;
+    W.ystride = (2 * s_max + 63) / 64 * 64;
+    W.off_j = b_w;
+    W.off_o = b_w + b_j;
+    W.off_q = b_w + b_j + b_o;
+    return 0;
+}
+
+int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, int P, int R, int L,
+                      const int* w_ini, double* d_out, hipStream_t st) {
+    int dev = 0;
+    GPURIR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_win_mu);
+    WindowPlan& W = g_win_plans[dev];
+    const size_t nk = 4 + (size_t)P + 1;
+    const bool hit = W.key.size() == nk && W.key[0] == P && W.key[1] == R && W.key[2] == L && W.key[3] == n_signal &&
+                     std::memcmp(W.key.data() + 4, w_ini, sizeof(int) * (P + 1)) == 0;
+    if (!hit) {
+        W.key.clear();
+        if (int rc = make_window_plan(W, n_signal, P, R, L, w_ini, st)) return rc;
+        W.key.resize(nk);
+        W.key[0] = P;
+        W.key[1] = R;
+        W.key[2] = L;
+        W.key[3] = n_signal;
+        std::memcpy(W.key.data() + 4, w_ini, sizeof(int) * (P + 1));
+    } else if (W.up_stream != st) {
+        GPURIR_HIP(hipStreamWaitEvent(st, W.uploaded, 0));
+    }
+    const int V = W.V;
+    const float2* d_tw = nullptr;
+    if (int rc = unit_tables_device(V, &d_tw)) return rc;
+    const size_t b_spec = al(sizeof(float2) * (size_t)P * V * UT);
+    const size_t b_y = al(sizeof(float) * (size_t)(2 * W.n_jobs) * R * W.ystride);
+    void* ws = nullptr;
+    if (int rc = workspace(WS_CONV, b_spec + b_y, &ws)) return rc;
+    unsigned char* w = static_cast<unsigned char*>(ws);
+    const unsigned char* t = static_cast<const unsigned char*>(W.d);
     WinArgs a;
     a.signal = d_signal;
-    a.w_ini = reinterpret_cast<const int*>(w);
+    a.w_ini = reinterpret_cast<const int*>(t);
     a.rirs = d_rirs;
     a.tw = d_tw;
-    a.p_lo = reinterpret_cast<const int*>(w + b_w);
-    a.ola_blk0 = reinterpret_cast<const int*>(w + b_w + b_j);
-    a.job_order = reinterpret_cast<const int*>(w + b_w + b_j + b_o);
-    a.spec = reinterpret_cast<float2*>(w + b_w + b_j + b_o + b_q);
-    a.ybuf = reinterpret_cast<float*>(w + b_w + b_j + b_o + b_q + b_spec);
+    a.p_lo = reinterpret_cast<const int*>(t + W.off_j);
+    a.ola_blk0 = reinterpret_cast<const int*>(t + W.off_o);
+    a.job_order = reinterpret_cast<const int*>(t + W.off_q);
+    a.spec = reinterpret_cast<float2*>(w);
+    a.ybuf = reinterpret_cast<float*>(w + b_spec);
     a.out = d_out;
     a.P = P;
     a.R = R;
     a.L = L;
     a.n_sig = n_signal;
-    a.s_max = s_max;
-    a.n_jobs = (int)n_jobs;
-    a.n_conv_wg = n_conv_wg;
-    a.ystride = ystride;
-    a.n_out = n_out;
-    if (V == 8) return launch_window<8>(a, n_ola_wg, st);
-    if (V == 16) return launch_window<16>(a, n_ola_wg, st);
-    return launch_window<24>(a, n_ola_wg, st);
+    a.s_max = W.s_max;
+    a.n_jobs = W.n_jobs;
+    a.n_conv_wg = W.n_conv_wg;
+    a.ystride = W.ystride;
+    a.n_out = (long long)n_signal + L - 1;
+    if (V == 8) return launch_window<8>(a, W.n_ola_wg, st);
+    if (V == 16) return launch_window<16>(a, W.n_ola_wg, st);
+    return launch_window<24>(a, W.n_ola_wg, st);
 }
 
 }  // namespace

@@ -13,6 +13,7 @@ current HIP stream, nothing copied to the host.
   trajectory         simulateTrajectory on device tensors (float64 output)
 """
 import ctypes
+import functools
 
 import numpy as np
 import torch
@@ -418,6 +419,15 @@ def simulate_rir_diff(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=N
     return views[0]
 
 
+@functools.lru_cache(maxsize=16)
+def _even_windows(n, P):
+    """Segment starts of simulateTrajectory without timestamps (gpuRIR/__init__.py:204-208):
+    int(p n / P), then n."""
+    w = np.ascontiguousarray(np.append((np.arange(P) * (n / P)).astype(int), n), dtype=np.int32)
+    w.setflags(write=False)
+    return w
+
+
 def trajectory(source_signal, rirs, timestamps=None, fs=None, device=None):
     """simulateTrajectory on device tensors; returns float64 [n + len - 1, n_rcv] on the GPU."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -426,9 +436,9 @@ def trajectory(source_signal, rirs, timestamps=None, fs=None, device=None):
     n = sig.shape[0]
     P, R, L = r.shape
     if timestamps is None:
-        fs = n / P
-        timestamps = np.arange(P)
-    w_ini = np.ascontiguousarray(np.append((np.asarray(timestamps) * fs).astype(int), n), dtype=np.int32)
+        w_ini = _even_windows(n, P)
+    else:
+        w_ini = np.ascontiguousarray(np.append((np.asarray(timestamps) * fs).astype(int), n), dtype=np.int32)
     out = torch.empty((n + L - 1, R), dtype=torch.float64, device=dev)
     _lib.check(_lib.load().gpurir_trajectory_device(sig.data_ptr(), n, r.data_ptr(), P, R, L, _lib.iptr(w_ini),
                                                     out.data_ptr(), _stream(dev)))

@@ -439,8 +439,8 @@ struct WinArgs {
     float2* spec;          // [P][V][UT] segment spectra / N, unit register layout
     const int* p_lo;       // [n_jobs] first point of window pair J
     const int* job_order;  // [n_conv_wg] job J * R + r of each workgroup, -1: none
-    const int* ola_blk0;   // [n_ola_wg] block of each OLA chunk's first sample
-    float* ybuf;           // [2 n_jobs][R][ystride] window results
+    float* yhead;          // [n_jobs][R][ystride] y_2J[0 : S_2J]
+    float* ytail;          // [n_jobs][R][ystride] y_2J+1[S_2J+1 : S_2J+1 + S_2J+2]
     double* out;           // [n_out][R]
     int P, R, L, n_sig, s_max, n_jobs, n_conv_wg, ystride;
     long long n_out;
@@ -899,35 +899,47 @@ __global__ __launch_bounds__(2 * UT) __attribute__((amdgpu_waves_per_eu(3))) voi
         acc[i] = make_float2(acc[i].x + o.x, acc[i].y + o.y);
     }
     unit_inverse<V>(acc, bufs[0], sd, tau);
-    const int ylen0 = (int)(o2 - o0), ylen1 = (int)(win_start(a, m0 + 3) - o1);
-    float* y0 = a.ybuf + ((size_t)m0 * a.R + r) * a.ystride;
-    float* y1 = y0 + (size_t)a.R * a.ystride;
+    // sample n = tau + 128 i of windows 2J (real part) and 2J + 1 (imaginary part).  Block
+    // 2J + 1 of the output lies under these two windows only: it is written here, as the FP64
+    // sum of the tail of window 2J and the head of window 2J + 1.  The head of window 2J and
+    // the tail of window 2J + 1 go to yhead / ytail for the seam blocks 2J and 2J + 2, which the
+    // neighbouring jobs share (window_seam_kernel).
+    const int S0 = (int)(o1 - o0), S1 = (int)(o2 - o1), S2 = (int)(win_start(a, m0 + 3) - o2);
+    float* head = a.yhead + ((size_t)J * a.R + r) * a.ystride;
+    float* tail = a.ytail + ((size_t)J * a.R + r) * a.ystride;
+    float* re = reinterpret_cast<float*>(bufs[1]);  // unit 1's buffer: read for the last time before the inverse
 #pragma unroll
     for (int i = 0; i < V; i++) {
         const int n = tau + UT * i;
-        if (n < ylen0) y0[n] = acc[i].x;
-        if (n < ylen1) y1[n] = acc[i].y;
+        re[n] = acc[i].x;
+        if (n < S0) head[n] = acc[i].x;
+        if (n >= S1 && n < S1 + S2) tail[n - S1] = acc[i].y;
+    }
+    __syncthreads();  // unit 0's two waves
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+        const int k = tau + UT * i;
+        if (k < S1 && o1 + k < a.n_out) a.out[(o1 + k) * a.R + r] = (double)re[S0 + k] + (double)acc[i].y;
     }
 }
 
-// out[t][r] = y_b[t - o_b] + y_{b-1}[t - o_{b-1}] in FP64 for the block b holding t.  A
-// workgroup owns WOLA_T consecutive samples (fewer than a segment, so at most two blocks:
-// ola_blk0[wg] and the next) and all receivers, eight at a time: the reads run along the
-// window rows, the sums go through LDS so that the [t][r] stores are contiguous.
-constexpr int WOLA_T = 128;
+// Seam block 2J (J = 0 .. n_jobs): out[o_2J + k][r] = head_J[k] + tail_{J-1}[k] in FP64, the
+// head of window 2J and the tail of window 2J - 1.  A workgroup owns WSEAM_T samples of one
+// block and all receivers, eight at a time: the reads run along the rows, the sums go through
+// LDS so that the [t][r] stores are contiguous.
+constexpr int WSEAM_T = 128;
 
-__global__ __launch_bounds__(256) void window_ola_kernel(WinArgs a) {
-    __shared__ double tile[WOLA_T * 9];
-    const long long t0 = (long long)blockIdx.x * WOLA_T;
-    const int nt = (int)min((long long)WOLA_T, a.n_out - t0);
-    const int n_win = 2 * a.n_jobs;
-    const int b0 = a.ola_blk0[blockIdx.x];
-    const long long om = b0 >= 1 ? win_start(a, b0 - 1) : 0, o0 = win_start(a, b0), o1 = win_start(a, b0 + 1);
-    const int i = threadIdx.x & (WOLA_T - 1), j0 = threadIdx.x >> 7;
-    const long long t = t0 + i;
-    const int b = t >= o1 ? b0 + 1 : b0;
-    const long long ob = b == b0 ? o0 : o1, op = b == b0 ? om : o0;  // o_b, o_{b-1}
-    const bool head = b < n_win && i < nt, tail = b >= 1 && b - 1 < n_win && i < nt;
+__global__ __launch_bounds__(256) void window_seam_kernel(WinArgs a) {
+    __shared__ double tile[WSEAM_T * 9];
+    const int chunks = (a.s_max + WSEAM_T - 1) / WSEAM_T;
+    const int J = blockIdx.x / chunks, c = blockIdx.x - J * chunks;
+    const long long ob = win_start(a, 2 * J);
+    const int k0 = c * WSEAM_T;
+    const long long len = min(win_start(a, 2 * J + 1) - ob, a.n_out - ob);  // samples of the block
+    const int nt = (int)max(0LL, min((long long)WSEAM_T, len - k0));
+    if (nt == 0) return;  // whole workgroup
+    const int i = threadIdx.x & (WSEAM_T - 1), j0 = threadIdx.x >> 7;
+    const bool has_head = J < a.n_jobs && i < nt, has_tail = J >= 1 && i < nt;
     for (int r0 = 0; r0 < a.R; r0 += 8) {
         const int rc = min(8, a.R - r0);
 #pragma unroll
@@ -936,15 +948,15 @@ __global__ __launch_bounds__(256) void window_ola_kernel(WinArgs a) {
             if (j < rc) {
                 const int r = r0 + j;
                 double v = 0.0;
-                if (head) v = (double)a.ybuf[((size_t)b * a.R + r) * a.ystride + (t - ob)];
-                if (tail) v += (double)a.ybuf[((size_t)(b - 1) * a.R + r) * a.ystride + (t - op)];
+                if (has_head) v = (double)a.yhead[((size_t)J * a.R + r) * a.ystride + k0 + i];
+                if (has_tail) v += (double)a.ytail[((size_t)(J - 1) * a.R + r) * a.ystride + k0 + i];
                 tile[i * 9 + j] = v;
             }
         }
         __syncthreads();
         for (int e = threadIdx.x; e < nt * rc; e += 256) {
             const int ii = e / rc, jj = e - ii * rc;
-            a.out[(t0 + ii) * a.R + r0 + jj] = tile[ii * 9 + jj];
+            a.out[(ob + k0 + ii) * a.R + r0 + jj] = tile[ii * 9 + jj];
         }
         __syncthreads();
     }
@@ -1158,18 +1170,17 @@ bool window_path_applies(const int* w_ini, int P) {
 }
 
 template <int V>
-int launch_window(WinArgs& a, int n_ola_wg, hipStream_t st) {
+int launch_window(WinArgs& a, int n_seam_wg, hipStream_t st) {
     hipLaunchKernelGGL(window_spectrum_kernel<V>, dim3((unsigned)a.P), dim3(UT), 0, st, a);
     GPURIR_HIP(hipGetLastError());
     hipLaunchKernelGGL(window_conv_kernel<V>, dim3((unsigned)a.n_conv_wg), dim3(2 * UT), 0, st, a);
     GPURIR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(window_ola_kernel, dim3((unsigned)n_ola_wg), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(window_seam_kernel, dim3((unsigned)n_seam_wg), dim3(256), 0, st, a);
     GPURIR_HIP(hipGetLastError());
     return 0;
 }
 
-// The device tables of one segmentation (P, R, L, n_signal, w_ini): w_ini | p_lo | ola_blk0 |
-// job_order, kept per device while calls repeat the segmentation, so that such a call plans
+// The device tables of one segmentation (P, R, L, n_signal, w_ini): w_ini | p_lo | job_order, kept per device while calls repeat the segmentation, so that such a call plans
 // nothing on the host and uploads nothing.  A different segmentation replaces them (the old
 // buffer is freed, which waits for the work that still reads it).
 struct WindowPlan {
@@ -1178,8 +1189,8 @@ struct WindowPlan {
     size_t bytes = 0;
     hipEvent_t uploaded = nullptr;  // recorded on up_stream after the upload
     hipStream_t up_stream = nullptr;
-    int V = 0, s_max = 0, n_jobs = 0, n_conv_wg = 0, n_ola_wg = 0, ystride = 0;
-    size_t off_j = 0, off_o = 0, off_q = 0;
+    int V = 0, s_max = 0, n_jobs = 0, n_conv_wg = 0, n_seam_wg = 0, ystride = 0;
+    size_t off_j = 0, off_q = 0;
 };
 std::mutex g_win_mu;
 std::map<int, WindowPlan> g_win_plans;
@@ -1195,7 +1206,8 @@ int make_window_plan(WindowPlan& W, int n_signal, int P, int R, int L, const int
     long long M = P;  // windows 0 .. M-1 receive products
     while (o(M) - o(P - 1) < L) M++;
     const long long n_jobs = (M + 1) / 2;
-    GPURIR_CHECK(n_jobs * R < (1LL << 30) && n_out / WOLA_T < (1LL << 30), "trajectory too long");
+    GPURIR_CHECK(n_jobs * R < (1LL << 30) && (n_jobs + 1) * ((s_max + WSEAM_T - 1) / WSEAM_T) < (1LL << 30),
+                 "trajectory too long");
     std::vector<int> p_lo((size_t)n_jobs);
     for (long long J = 0, p = 0; J < n_jobs; J++) {
         while (p < P && o(2 * J) - o(p) >= L) p++;
@@ -1231,16 +1243,8 @@ int make_window_plan(WindowPlan& W, int n_signal, int P, int R, int L, const int
     for (size_t x = 0; x < 8; x++)
         for (size_t k = 0; k < per_xcd[x].size(); k++) order[8 * k + x] = per_xcd[x][k];
     const int n_conv_wg = (int)order.size();
-    const int n_ola_wg = (int)((n_out + WOLA_T - 1) / WOLA_T);
-    std::vector<int> blk0((size_t)n_ola_wg);
-    for (long long w = 0, b = 0; w < n_ola_wg; w++) {
-        const long long t0 = w * WOLA_T;
-        while (o(b + 1) <= t0) b++;
-        blk0[(size_t)w] = (int)b;
-    }
-    const size_t b_w = al(sizeof(int) * (P + 1)), b_j = al(sizeof(int) * n_jobs), b_o = al(sizeof(int) * n_ola_wg);
-    const size_t b_q = al(sizeof(int) * n_conv_wg);
-    const size_t bytes = b_w + b_j + b_o + b_q;
+    const size_t b_w = al(sizeof(int) * (P + 1)), b_j = al(sizeof(int) * n_jobs), b_q = al(sizeof(int) * n_conv_wg);
+    const size_t bytes = b_w + b_j + b_q;
     if (W.bytes < bytes) {
         if (W.d) GPURIR_HIP(hipFree(W.d));  // waits for the work that still reads the old tables
         W.d = nullptr;
@@ -1257,8 +1261,7 @@ int make_window_plan(WindowPlan& W, int n_signal, int P, int R, int L, const int
     std::vector<unsigned char> host(bytes, 0);
     std::memcpy(host.data(), w_ini, sizeof(int) * (P + 1));
     std::memcpy(host.data() + b_w, p_lo.data(), sizeof(int) * n_jobs);
-    std::memcpy(host.data() + b_w + b_j, blk0.data(), sizeof(int) * n_ola_wg);
-    std::memcpy(host.data() + b_w + b_j + b_o, order.data(), sizeof(int) * n_conv_wg);
+    std::memcpy(host.data() + b_w + b_j, order.data(), sizeof(int) * n_conv_wg);
     if (int rc = upload_async(W.d, host.data(), bytes, st)) {
         W.key.clear();
         return rc;
@@ -1270,11 +1273,10 @@ int make_window_plan(WindowPlan& W, int n_signal, int P, int R, int L, const int
     W.s_max = s_max;
     W.n_jobs = (int)n_jobs;
     W.n_conv_wg = n_conv_wg;
-    W.n_ola_wg = n_ola_wg;
-    W.ystride = (2 * s_max + 63) / 64 * 64;
+    W.n_seam_wg = (int)((n_jobs + 1) * ((s_max + WSEAM_T - 1) / WSEAM_T));
+    W.ystride = (s_max + 63) / 64 * 64;
     W.off_j = b_w;
-    W.off_o = b_w + b_j;
-    W.off_q = b_w + b_j + b_o;
+    W.off_q = b_w + b_j;
     return 0;
 }
 
@@ -1303,9 +1305,9 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     const float2* d_tw = nullptr;
     if (int rc = unit_tables_device(V, &d_tw)) return rc;
     const size_t b_spec = al(sizeof(float2) * (size_t)P * V * UT);
-    const size_t b_y = al(sizeof(float) * (size_t)(2 * W.n_jobs) * R * W.ystride);
+    const size_t b_y = al(sizeof(float) * (size_t)W.n_jobs * R * W.ystride);
     void* ws = nullptr;
-    if (int rc = workspace(WS_CONV, b_spec + b_y, &ws)) return rc;
+    if (int rc = workspace(WS_CONV, b_spec + 2 * b_y, &ws)) return rc;
     unsigned char* w = static_cast<unsigned char*>(ws);
     const unsigned char* t = static_cast<const unsigned char*>(W.d);
     WinArgs a;
@@ -1314,10 +1316,10 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     a.rirs = d_rirs;
     a.tw = d_tw;
     a.p_lo = reinterpret_cast<const int*>(t + W.off_j);
-    a.ola_blk0 = reinterpret_cast<const int*>(t + W.off_o);
     a.job_order = reinterpret_cast<const int*>(t + W.off_q);
     a.spec = reinterpret_cast<float2*>(w);
-    a.ybuf = reinterpret_cast<float*>(w + b_spec);
+    a.yhead = reinterpret_cast<float*>(w + b_spec);
+    a.ytail = reinterpret_cast<float*>(w + b_spec + b_y);
     a.out = d_out;
     a.P = P;
     a.R = R;
@@ -1328,9 +1330,9 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     a.n_conv_wg = W.n_conv_wg;
     a.ystride = W.ystride;
     a.n_out = (long long)n_signal + L - 1;
-    if (V == 8) return launch_window<8>(a, W.n_ola_wg, st);
-    if (V == 16) return launch_window<16>(a, W.n_ola_wg, st);
-    return launch_window<24>(a, W.n_ola_wg, st);
+    if (V == 8) return launch_window<8>(a, W.n_seam_wg, st);
+    if (V == 16) return launch_window<16>(a, W.n_seam_wg, st);
+    return launch_window<24>(a, W.n_seam_wg, st);
 }
 
 }  // namespace

@@ -90,6 +90,11 @@ _SIG = {
     "gpurir_trajectory_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
+    "gpurir_trajectory_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_void_p,
+                                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "gpurir_trajectory_plan_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gpurir_trajectory_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 SYMBOLS = tuple(_SIG)

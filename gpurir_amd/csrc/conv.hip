@@ -1280,6 +1280,11 @@ int make_window_plan(WindowPlan& W, int n_signal, int P, int R, int L, const int
     return 0;
 }
 
+size_t window_workspace_bytes(const WindowPlan& W, int P, int R);
+int window_args(const WindowPlan& W, const float* d_signal, int n_signal, const float* d_rirs, int P, int R, int L,
+                double* d_out, void* ws, WinArgs& a);
+int launch_window(WinArgs& a, const WindowPlan& W, hipStream_t st);
+
 int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, int P, int R, int L,
                       const int* w_ini, double* d_out, hipStream_t st) {
     int dev = 0;
@@ -1301,16 +1306,26 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     } else if (W.up_stream != st) {
         GPURIR_HIP(hipStreamWaitEvent(st, W.uploaded, 0));
     }
+    void* ws = nullptr;
+    if (int rc = workspace(WS_CONV, window_workspace_bytes(W, P, R), &ws)) return rc;
+    WinArgs a;
+    if (int rc = window_args(W, d_signal, n_signal, d_rirs, P, R, L, d_out, ws, a)) return rc;
+    return launch_window(a, W, st);
+}
+
+size_t window_workspace_bytes(const WindowPlan& W, int P, int R) {
+    return al(sizeof(float2) * (size_t)P * W.V * UT) + 2 * al(sizeof(float) * (size_t)W.n_jobs * R * W.ystride);
+}
+
+int window_args(const WindowPlan& W, const float* d_signal, int n_signal, const float* d_rirs, int P, int R, int L,
+                double* d_out, void* ws, WinArgs& a) {
     const int V = W.V;
     const float2* d_tw = nullptr;
     if (int rc = unit_tables_device(V, &d_tw)) return rc;
     const size_t b_spec = al(sizeof(float2) * (size_t)P * V * UT);
     const size_t b_y = al(sizeof(float) * (size_t)W.n_jobs * R * W.ystride);
-    void* ws = nullptr;
-    if (int rc = workspace(WS_CONV, b_spec + 2 * b_y, &ws)) return rc;
     unsigned char* w = static_cast<unsigned char*>(ws);
     const unsigned char* t = static_cast<const unsigned char*>(W.d);
-    WinArgs a;
     a.signal = d_signal;
     a.w_ini = reinterpret_cast<const int*>(t);
     a.rirs = d_rirs;
@@ -1330,13 +1345,32 @@ int window_trajectory(const float* d_signal, int n_signal, const float* d_rirs, 
     a.n_conv_wg = W.n_conv_wg;
     a.ystride = W.ystride;
     a.n_out = (long long)n_signal + L - 1;
-    if (V == 8) return launch_window<8>(a, W.n_seam_wg, st);
-    if (V == 16) return launch_window<16>(a, W.n_seam_wg, st);
+    return 0;
+}
+
+int launch_window(WinArgs& a, const WindowPlan& W, hipStream_t st) {
+    if (W.V == 8) return launch_window<8>(a, W.n_seam_wg, st);
+    if (W.V == 16) return launch_window<16>(a, W.n_seam_wg, st);
     return launch_window<24>(a, W.n_seam_wg, st);
 }
 
 }  // namespace
 }  // namespace gpurir
+
+struct gpurir_traj_plan {
+    const float* signal;
+    const float* rirs;
+    double* out;
+    int n_signal, P, R, L;
+    std::vector<int> w_ini;
+    bool window = false;
+    gpurir::WindowPlan tables;  // own tables and workspace: the captured graph reads them
+    void* ws = nullptr;
+    gpurir::WinArgs args;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
 
 using namespace gpurir;
 
@@ -1365,20 +1399,115 @@ int gpurir_conv(const float* segments, int M_src, int seg_len, const float* rirs
     return 0;
 }
 
-int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv,
-                             int rir_len, const int* w_ini, double* d_out, void* stream) {
+namespace {
+int check_trajectory_args(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv, int rir_len,
+                          const int* w_ini, double* d_out) {
     GPURIR_CHECK(d_signal && d_rirs && w_ini && d_out, "null pointer argument");
     GPURIR_CHECK(n_signal > 0 && n_pts > 0 && n_rcv > 0 && rir_len > 0, "sizes must be positive");
     GPURIR_CHECK(w_ini[0] >= 0 && w_ini[n_pts] == n_signal, "w_ini must start at >= 0 and end at n_signal");
-    std::vector<int> ss(n_pts), sl(n_pts);
-    for (int p = 0; p < n_pts; p++) {
-        GPURIR_CHECK(w_ini[p + 1] >= w_ini[p], "segment starts must be non-decreasing");
-        ss[p] = w_ini[p];
-        sl[p] = w_ini[p + 1] - w_ini[p];
+    for (int p = 0; p < n_pts; p++) GPURIR_CHECK(w_ini[p + 1] >= w_ini[p], "segment starts must be non-decreasing");
+    return 0;
+}
+
+void free_traj_plan(gpurir_traj_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->tables.uploaded) (void)hipEventDestroy(p->tables.uploaded);
+    if (p->tables.d) (void)hipFree(p->tables.d);
+    if (p->ws) (void)hipFree(p->ws);
+    delete p;
+}
+}  // namespace
+
+int gpurir_trajectory_plan_create(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv,
+                                  int rir_len, const int* w_ini, double* d_out, gpurir_traj_plan_t* plan) {
+    GPURIR_CHECK(plan != nullptr, "null plan pointer");
+    *plan = nullptr;
+    if (int rc = check_trajectory_args(d_signal, n_signal, d_rirs, n_pts, n_rcv, rir_len, w_ini, d_out)) return rc;
+    gpurir_traj_plan* p = new gpurir_traj_plan();
+    p->signal = d_signal;
+    p->rirs = d_rirs;
+    p->out = d_out;
+    p->n_signal = n_signal;
+    p->P = n_pts;
+    p->R = n_rcv;
+    p->L = rir_len;
+    p->w_ini.assign(w_ini, w_ini + n_pts + 1);
+    p->window = window_path_applies(w_ini, n_pts);
+    if (p->window) {
+        if (int rc = make_window_plan(p->tables, n_signal, n_pts, n_rcv, rir_len, w_ini, nullptr)) {
+            free_traj_plan(p);
+            return rc;
+        }
+        if (hipMalloc(&p->ws, window_workspace_bytes(p->tables, n_pts, n_rcv)) != hipSuccess) {
+            (void)hipGetLastError();
+            p->ws = nullptr;
+            free_traj_plan(p);
+            set_error("out of device memory for the trajectory plan");
+            return -3;
+        }
+        if (int rc = window_args(p->tables, d_signal, n_signal, d_rirs, n_pts, n_rcv, rir_len, d_out, p->ws, p->args)) {
+            free_traj_plan(p);
+            return rc;
+        }
+        if (hipStreamSynchronize(nullptr) != hipSuccess) {  // the tables are on the device
+            free_traj_plan(p);
+            GPURIR_HIP(hipGetLastError());
+        }
     }
+    *plan = p;
+    return 0;
+}
+
+int gpurir_trajectory_plan_run(gpurir_traj_plan_t p, void* stream) {
+    GPURIR_CHECK(p != nullptr, "null plan");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!p->window)
+        return gpurir_trajectory_device(p->signal, p->n_signal, p->rirs, p->P, p->R, p->L, p->w_ini.data(), p->out,
+                                        stream);
+    if (!p->exec) {
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+        if (!p->cap_stream) GPURIR_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        GPURIR_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_window(p->args, p->tables, p->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            GPURIR_HIP(e);
+        }
+        p->graph = g;
+        hipError_t ie = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            p->exec = nullptr;
+            GPURIR_HIP(ie);
+        }
+    }
+    GPURIR_HIP(hipGraphLaunch(p->exec, st));
+    return 0;
+}
+
+int gpurir_trajectory_plan_destroy(gpurir_traj_plan_t p) {
+    if (!p) return 0;
+    (void)hipDeviceSynchronize();  // no replay may still read the plan's buffers
+    free_traj_plan(p);
+    return 0;
+}
+
+int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv,
+                             int rir_len, const int* w_ini, double* d_out, void* stream) {
+    if (int rc = check_trajectory_args(d_signal, n_signal, d_rirs, n_pts, n_rcv, rir_len, w_ini, d_out)) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (window_path_applies(w_ini, n_pts))
         return window_trajectory(d_signal, n_signal, d_rirs, n_pts, n_rcv, rir_len, w_ini, d_out, st);
+    std::vector<int> ss(n_pts), sl(n_pts);
+    for (int p = 0; p < n_pts; p++) {
+        ss[p] = w_ini[p];
+        sl[p] = w_ini[p + 1] - w_ini[p];
+    }
     float* conv = nullptr;
     int stride = 0;
     void* extra = nullptr;

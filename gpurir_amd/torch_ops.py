@@ -443,3 +443,48 @@ def trajectory(source_signal, rirs, timestamps=None, fs=None, device=None):
     _lib.check(_lib.load().gpurir_trajectory_device(sig.data_ptr(), n, r.data_ptr(), P, R, L, _lib.iptr(w_ini),
                                                     out.data_ptr(), _stream(dev)))
     return out
+
+
+class TrajectoryPlan:
+    """simulateTrajectory bound to device buffers and a segmentation, for repeated calls.
+
+    signal [n] and rirs [P, R, L] are float32 device tensors that the plan keeps and reads at
+    every run (refill them in place between runs); out [n + L - 1, R] float64 is the plan's
+    result buffer.  With simulateTrajectory's own segmentation (no timestamps) the plan replays
+    one HIP graph of its kernels, so a run costs one graph launch on the host; other
+    segmentations run the eager path."""
+
+    def __init__(self, signal, rirs, timestamps=None, fs=None):
+        if not (isinstance(signal, torch.Tensor) and isinstance(rirs, torch.Tensor)):
+            raise TypeError("signal and rirs must be device tensors")
+        if not (signal.is_cuda and rirs.is_cuda and signal.dtype == torch.float32 and rirs.dtype == torch.float32
+                and signal.is_contiguous() and rirs.is_contiguous() and rirs.dim() == 3 and signal.dim() == 1):
+            raise ValueError("signal [n] and rirs [P, R, L] must be contiguous float32 tensors on the GPU")
+        self.device = rirs.device
+        self.signal, self.rirs = signal, rirs
+        n = signal.shape[0]
+        P, R, L = rirs.shape
+        if timestamps is None:
+            w_ini = _even_windows(n, P)
+        else:
+            w_ini = np.ascontiguousarray(np.append((np.asarray(timestamps) * fs).astype(int), n), dtype=np.int32)
+        self.out = torch.empty((n + L - 1, R), dtype=torch.float64, device=self.device)
+        self._h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.load().gpurir_trajectory_plan_create(signal.data_ptr(), n, rirs.data_ptr(), P, R, L,
+                                                                 _lib.iptr(w_ini), self.out.data_ptr(),
+                                                                 ctypes.byref(self._h)))
+
+    def run(self):
+        """Enqueue one simulateTrajectory of the buffers' contents on the current stream; returns out."""
+        _lib.check(_lib.load().gpurir_trajectory_plan_run(self._h, _stream(self.device)))
+        return self.out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().gpurir_trajectory_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None

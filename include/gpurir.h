@@ -231,6 +231,20 @@ int gpurir_conv(const float* segments, int M_src, int seg_len, const float* rirs
 int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv,
                              int rir_len, const int* w_ini, double* d_out, void* stream);
 
+/* A trajectory plan: gpurir_trajectory_device bound to fixed device buffers (d_signal,
+ * d_rirs, d_out, as above) and one segmentation w_ini (host, copied).  The plan owns its
+ * tables and workspace; each run enqueues the simulateTrajectory of the buffers' current
+ * contents on `stream`.  Segments of S or S + 1 samples (256 <= S <= 1536, the segmentation
+ * of simulateTrajectory without timestamps, gpuRIR/__init__.py:204-208) replay one HIP graph
+ * captured on the first run; any other segmentation runs gpurir_trajectory_device eagerly.
+ * No counterpart in the reference: it serves device-resident pipelines that call
+ * simulateTrajectory repeatedly on the same shapes. */
+typedef struct gpurir_traj_plan* gpurir_traj_plan_t;
+int gpurir_trajectory_plan_create(const float* d_signal, int n_signal, const float* d_rirs, int n_pts, int n_rcv,
+                                  int rir_len, const int* w_ini, double* d_out, gpurir_traj_plan_t* plan);
+int gpurir_trajectory_plan_run(gpurir_traj_plan_t plan, void* stream);
+int gpurir_trajectory_plan_destroy(gpurir_traj_plan_t plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,36 @@ def test_trajectory_segment_aligned(n, P, R, L):
     assert rel_l2(got.cpu().numpy(), want) < 1e-5
 
 
+def test_trajectory_plan_graph_replays():
+    # the plan's captured graph against the eager path: bit-identical on every replay, with eager
+    # work of another segmentation (which replaces the library's cached tables and workspace)
+    # and synchronisations between replays, and after the RIR buffer is refilled in place
+    rng = np.random.default_rng(11)
+    n, P, R, L = 16000, 16, 3, 2000
+    sig = torch.as_tensor(rng.standard_normal(n).astype(np.float32), device="cuda")
+    rirs = torch.as_tensor((rng.standard_normal((P, R, L)) * np.exp(-np.arange(L) / 500.0)).astype(np.float32),
+                           device="cuda")
+    plan = T.TrajectoryPlan(sig, rirs)
+    want = T.trajectory(sig, rirs).clone()
+    assert rel_l2(want.cpu().numpy(), orc.simulate_trajectory(sig.cpu().numpy(), rirs.cpu().numpy())) < 1e-5
+    other_sig = torch.as_tensor(rng.standard_normal(3 * n).astype(np.float32), device="cuda")
+    other_rirs = torch.as_tensor(rng.standard_normal((24, 2, 3000)).astype(np.float32), device="cuda")
+    for _ in range(3):
+        got = plan.run()
+        torch.cuda.synchronize()
+        assert torch.equal(got, want)
+        T.trajectory(other_sig, other_rirs)
+        torch.cuda.synchronize()
+    rirs.mul_(-0.5)
+    got = plan.run().clone()
+    assert torch.equal(got, T.trajectory(sig, rirs))
+    # a segmentation outside the segment-aligned path runs eagerly inside the plan
+    ts = np.array([0.0, 0.1, 0.15, 0.5, 0.55, 0.6, 0.7, 0.8, 0.82, 0.84, 0.9, 0.92, 0.94, 0.96, 0.98, 0.99])
+    plan2 = T.TrajectoryPlan(sig, rirs, timestamps=ts, fs=n)
+    got2 = plan2.run().clone()
+    assert torch.equal(got2, T.trajectory(sig, rirs, timestamps=ts, fs=n))
+
+
 def test_trajectory_uneven_custom_timestamps():
     rng = np.random.default_rng(5)
     fs = 16000.0

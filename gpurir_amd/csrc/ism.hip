@@ -715,9 +715,13 @@ __global__ __launch_bounds__(WG) void ism_accumulate_persistent_kernel(AccArgs a
 //     entries inside a distance bound are contiguous runs of m found in closed form:
 //     the rows (x, y) with x^2 + y^2 inside the tile's outer radius are one run of y
 //     per parity for each x, and a row's z entries in the shell at most two runs per
-//     parity.  Rows are taken WGB at a time (one per thread), their images written
-//     to an LDS queue by a block-wide prefix sum, and the queue is evaluated one image
-//     per lane with the same FP64 operations as image_eval (identical n0 and f).
+//     parity.  Rows are taken WGB at a time (one per thread) and their images are
+//     placed by a block-wide prefix sum into rounds of QBK queue entries.  A row marks
+//     each of its runs in the round's LDS queue (a key at the run's first entry and at
+//     every wave segment start it covers); a wave prefix maximum of the keys gives every
+//     entry its row and z entry.  A thread takes QBK / WGB consecutive entries: the delays
+//     and amplitudes of all of them first (independent FP64 chains), the sums after.  The
+//     delays use the same FP64 operations as image_eval (identical n0 and f).
 //   * Each live image adds amp scale T_m(u), m < NB, into D_m (64-bit fixed point,
 //     ds_add_u64: exact and order-independent), T_m by the three-term recurrence.
 //   * D is converted to FP32 in place (rows padded by 4 floats per 16, so that the
@@ -774,6 +778,13 @@ __device__ inline int wave_max_nonneg(int x) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
     return x;
+}
+
+// Largest of non-negative integers over the lanes below this one (0 in lane 0): the prefix
+// maximum, moved up one lane by wave_shr:1.
+__device__ inline int wave_excl_max_nonneg(int x) {
+    x = wave_max_nonneg(x);
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);  // wave_shr:1
 }
 
 // Exclusive prefix sum of one int per thread over the WGB threads; *total gets the sum.
@@ -856,6 +867,9 @@ template <int NB, int DLC, int H16, bool LUT>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
     constexpr int QBK = LUT ? QB_LUT : QB;
+    constexpr int QEPT = QBK / WGB;   // consecutive queue entries per thread
+    constexpr int QSEG = 64 * QEPT;   // entries per wave
+    static_assert(QEPT * WGB == QBK, "whole entries per thread");
     const Item it = a.items[item];
     const RoomK R = load_room(a.rooms, it.room, a.prm);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -880,7 +894,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
-    int* queue = rxy + WGB;                                           // [QBK] slot << 12 | nz
+    int* queue = rxy + WGB;                                           // [QBK] run marks of a round
     int* misc = queue + QBK;                                          // [16] scan scratch, att maxima
     float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
     float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
@@ -896,6 +910,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         if (LUT) D32[e] = 0;
         else D[e] = 0;
     }
+    for (int e = tid; e < QBK; e += WGB) queue[e] = 0;  // run marks: zero between rounds
     if (H16) {
         for (int e = tid; e < 4 * NB * (CL + 4); e += WGB) {
             const int rho = e / (NB * (CL + 4)), rm = e - rho * (NB * (CL + 4));
@@ -1016,18 +1031,47 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             int tot;
             const int ex = block_scan_excl(cnt, misc, &tot);
             for (int q0 = 0; q0 < tot; q0 += QBK) {
-                int e = ex;
+                // Entry p of the round (run g of row slot, run start s) has nz = base + 2 p with
+                // base = zf[g] - 2 (s - q0) in [-4094, 4095].  Each run writes the key
+                // (slot, g, base + 4096) at its first entry in the round and at every wave
+                // segment start it covers; keys grow with the position, so an entry's key is
+                // the largest one at or before it.
+                {
+                    int s = ex;
 #pragma unroll
-                for (int g = 0; g < 4; g++) {
-                    const int e0 = max(e, q0), e1 = min(e + zc[g], q0 + QBK);
-                    for (int p = e0; p < e1; p++) queue[p - q0] = (tid << 12) | (zf[g] + 2 * (p - e));
-                    e += zc[g];
+                    for (int g = 0; g < 4; g++) {
+                        const int a0 = max(s, q0) - q0, b0 = min(s + zc[g], q0 + QBK) - q0;
+                        if (a0 < b0) {
+                            const int key = ((tid * 4 + g) << 13) | (zf[g] - 2 * (s - q0) + 4096);
+                            queue[a0] = key;
+                            for (int bnd = (a0 / QSEG + 1) * QSEG; bnd < b0; bnd += QSEG) queue[bnd] = key;
+                        }
+                        s += zc[g];
+                    }
                 }
                 __syncthreads();
                 const int nq = min(QBK, tot - q0);
-                for (int k = tid; k < nq; k += WGB) {
-                    const int qe = queue[k];
-                    const int slot = qe >> 12, nz = qe & 4095;
+                int keys[QEPT];
+                int kmax = 0;
+#pragma unroll
+                for (int i = 0; i < QEPT; i++) {
+                    keys[i] = queue[QEPT * tid + i];
+                    kmax = max(kmax, keys[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < QEPT; i++) queue[QEPT * tid + i] = 0;
+                int key = wave_excl_max_nonneg(kmax);
+                // delays and amplitudes of the thread's entries first (independent chains), the
+                // sums after; an entry outside the round or the tile's delays gets v = 0
+                int en0[QEPT], eph[QEPT];
+                float ev[QEPT], eu[QEPT];
+#pragma unroll
+                for (int i = 0; i < QEPT; i++) {
+                    key = max(key, keys[i]);
+                    const int pos = QEPT * tid + i;
+                    const bool inq = pos < nq;
+                    const int slot = inq ? key >> 15 : 0;
+                    const int nz = inq ? (key & 8191) - 4096 + 2 * pos : 0;
                     const double vz = offs[Nx + Ny + nz];
                     double d2 = rb2[slot];
                     d2 = d2 + vz * vz;
@@ -1035,7 +1079,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     const double tau = dist * fsc;
                     const double fl = floor(tau);
                     const int n0 = (int)fl;
-                    if (n0 < lo_n0 || n0 > hi_n0) continue;
+                    const bool live = inq && n0 >= lo_n0 && n0 <= hi_n0;
                     const float f = (float)(tau - fl);
                     const float distf = (float)dist;
                     float amp = ratt[slot] * att[Nx + Ny + nz];
@@ -1046,22 +1090,34 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                         amp = apply_patterns(R, amp, offs[nx], offs[Nx + ny], vz, distf, nx, ny, nz, a.spkr, a.mic,
                                              os, orr);
                     }
-                    if (amp == 0.0f) continue;
-                    const float v = amp * scale;
+                    ev[i] = live ? amp * scale : 0.0f;
+                    en0[i] = n0 - lo_n0;
                     if (LUT) {
                         // 16 (tau - n0) = p + w on the table's grid (exact in FP64)
                         const double g16 = (tau - fl) * 16.0;
                         const double gp = floor(g16);
-                        const int ph = (int)gp;
-                        const float w = (float)(g16 - gp);
-                        int* dp = D32 + ph * DL + (n0 - lo_n0);
+                        eph[i] = (int)gp;
+                        eu[i] = (float)(g16 - gp);
+                    } else {
+                        eph[i] = 0;
+                        eu[i] = fmaf(2.0f, f, -1.0f);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < QEPT; i++) {
+                    const float v = ev[i];
+                    if (v == 0.0f) continue;
+                    if (LUT) {
+                        const int ph = eph[i];
+                        const float w = eu[i];
+                        int* dp = D32 + ph * DL + en0[i];
                         lds_add32(dp, fmaf(-v, w, v));
                         lds_add32(ph < LUT_NB - 1 ? dp + DL : dp - (LUT_NB - 1) * DL + 1, v * w);
                         continue;
                     }
-                    const float u = fmaf(2.0f, f, -1.0f);
+                    const float u = eu[i];
                     const float u2 = u + u;
-                    long long* dp = D + (n0 - lo_n0);
+                    long long* dp = D + en0[i];
                     float b0 = v, b1 = v * u;
                     lds_add(dp, b0);
                     lds_add(dp + DL, b1);
@@ -1201,15 +1257,19 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     }
 }
 
+// The table mode's kernels with the row length compiled in are held to 128 VGPRs (two
+// workgroups per CU); the same cap on the other kernels schedules them worse.
 template <int NB, int DLC, int H16, bool LUT>
-__global__ __launch_bounds__(WGB) void ism_basis_kernel(AccArgs a) {
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && LUT ? 4 : 1)))
+void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     basis_item<NB, DLC, H16, LUT>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
 template <int NB, int DLC, int H16, bool LUT>
-__global__ __launch_bounds__(WGB) void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && LUT ? 4 : 1)))
+void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
     for (;;) {

@@ -168,6 +168,39 @@ __device__ inline void lds_add(long long* p, float v) {
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Two fixed-point sums in one 64-bit word: hv in the high 64 - S bits, lv in the low S
+// bits.  The word is the two's-complement sum of (qh << S) + ql over the adds, so as long
+// as every low sum stays inside (-2^(S-1), 2^(S-1)) and every high sum inside its field,
+// both are recovered exactly (unpack_pair): the low field as the sign-extended low S bits,
+// the high field as the rest shifted down.  One ds_add_u64 carries two basis rows.
+template <int S>
+__device__ inline void lds_add_pair(unsigned long long* p, float hv, float lv) {
+    int qh, ql;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(qh) : "v"(hv));
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(ql) : "v"(lv));
+    const unsigned long long w = ((unsigned long long)(long long)qh << S) + (unsigned long long)(long long)ql;
+    __hip_atomic_fetch_add(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int K>
+__device__ constexpr float exp2_neg() {
+    static_assert(K >= 0 && K < 64, "power of two");
+    return 1.0f / (float)(1ULL << K);
+}
+
+// hi is returned as the high field times 2^S (the word minus its sign-extended low field).
+template <int S>
+__device__ inline void unpack_pair(unsigned long long w, long long& hi, long long& lo) {
+    lo = (long long)((int)((unsigned)w << (32 - S)) >> (32 - S));
+    hi = (long long)w - lo;
+}
+
+// A 64-bit fixed-point value times 2^-E in FP32 (inv = 2^-E): the two 32-bit halves
+// converted separately, within 2 ulp of the correctly rounded value.
+__device__ inline float fix64_to_f32(long long v, float inv) {
+    return fmaf((float)(int)(v >> 32), inv * 4294967296.0f, (float)(unsigned)v * inv);
+}
+
 // Mixed-precision mode: 32-bit fixed point (ds_add_u32, half the LDS traffic of the
 // 64-bit sums); the workgroup moves these sums into an FP32 accumulator before
 // they can overflow.
@@ -739,6 +772,20 @@ static_assert(NWB == 4 * NOT, "four basis quarters per output block");
 constexpr int BASIS_NB = 8; // Chebyshev terms (FP32: within 5e-7 of each tap)
 constexpr int BASIS_NB_FP16 = 6;  // mixed precision: 6 terms, within 6e-5 (1e-4 relative L2)
 constexpr int BCS = 2;      // columns per thread of the in-place conversion: DL <= BCS * WGB
+// Packed basis sums: word w < NB/2 of a column holds basis row w in its high 64 - S bits
+// and row NB - 1 - w in its low S bits (lds_add_pair).  The low fields take the rows with
+// the smallest filters (coefficient L2 norms 0.83, 0.75, 0.23, 5.4e-2, 9.0e-3, 1.4e-3,
+// 1.5e-4, 1.8e-5 for rows 0..7), so each field is as wide as its row's filter needs.  Every
+// field's sum is bounded per tile by (images in the tile's shell) x (their largest
+// amplitude), and the tile's scale keeps that bound plus the rounding of every image
+// inside the narrowest field.  RIRs of PACK_MAX_IMAGES images or more use 64-bit rows.
+__host__ __device__ constexpr int pack_lo_bits(int nb, int w) {
+    return nb == 8 ? (w == 0 ? 24 : w == 1 ? 25 : w == 2 ? 28 : 30) : (w == 0 ? 26 : w == 1 ? 28 : 30);
+}
+__host__ __device__ constexpr int pack_field_bits(int nb, int m) {
+    return m < nb / 2 ? 64 - pack_lo_bits(nb, m) : pack_lo_bits(nb, nb - 1 - m);
+}
+constexpr long long PACK_MAX_IMAGES = 1LL << 21;
 // Lookup-table mode on the same machinery: the table's taps are linear in the weight of
 // the 1/16-sample grid, so an image at 16 tau = 16 n0 + p + a (phase p < 16, 0 <= a < 1)
 // adds amp (1 - a) to phase row p and amp a to row p + 1 (row 0 of column n0 + 1 for
@@ -863,9 +910,11 @@ __device__ inline unsigned short half_bits(float x) {
 // the filters take 16-bit inputs (D scaled per tile by a power of two, the coefficients
 // rounded) with FP32 accumulation: FP16 (1, v_mfma_f32_16x16x16_f16) or BF16 (2,
 // v_mfma_f32_16x16x16_bf16).
-template <int NB, int DLC, int H16, bool LUT>
+// PK: the sums of two basis rows share a 64-bit word (pack_lo_bits), with a scale per tile.
+template <int NB, int DLC, int H16, bool LUT, bool PK>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
+    static_assert(!PK || (!LUT && NB % 2 == 0), "packed sums pair the Chebyshev rows");
     constexpr int QBK = LUT ? QB_LUT : QB;
     constexpr int QEPT = QBK / WGB;   // consecutive queue entries per thread
     constexpr int QSEG = 64 * QEPT;   // entries per wave
@@ -883,9 +932,11 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4] rows before x entry n
     long long* D = reinterpret_cast<long long*>(rowStart + ntab + 4);  // [NB][DL] fixed point
     int* D32 = reinterpret_cast<int*>(D);                              // LUT: [NB][DL] 32-bit sums
+    unsigned long long* Dw = reinterpret_cast<unsigned long long*>(D);  // PK: [NB/2][DL] row pairs
     // the rows as FP32 [NB][DLP] after the conversion fit the 64-bit rows; the LUT's 32-bit
-    // rows are given the larger of the two
-    const int dreg = LUT ? max(NB * DL, NB * DLP) : 2 * NB * DL;     // floats of the row region
+    // rows and the packed pairs are given the larger of the two (16-bit rows fit either)
+    const int dreg = (LUT || PK) ? max(NB * DL, H16 ? 0 : NB * DLP) : 2 * NB * DL;  // floats of the row region
+    const int nwords = PK ? NB / 2 * DL : NB * DL;  // 64-bit (or LUT 32-bit) words of the sums
     float* ctab = reinterpret_cast<float*>(D) + dreg;                 // [NB][CL] (H16: see chs)
     // H16: four copies of the coefficients in FP16, copy rho shifted by rho entries, so that a
     // lane's four A operands start at a multiple of 4 halves: [4][NB][CL + 4]
@@ -906,7 +957,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     if (tid < 4) misc[8 + tid] = 0;
     if (tid >= 4 && tid < 7) misc[9 + tid] = 0x7f800000;  // misc[13..15]: per-axis min |offset| bits
     build_axis_tables(R, ps, pr, offs, att, tid, WGB);
-    for (int e = tid; e < NB * DL; e += WGB) {
+    for (int e = tid; e < nwords; e += WGB) {
         if (LUT) D32[e] = 0;
         else D[e] = 0;
     }
@@ -935,17 +986,23 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     const AxisRuns AY = axis_runs(Ny, R.L1, ps[1], pr[1]);
     const AxisRuns AZ = axis_runs(Nz, R.L2, ps[2], pr[2]);
     __syncthreads();
+    // |amp T_m| <= max|att| / (4 pi d) for an image at distance d >= d_min, d_min from the
+    // smallest offset per axis (the direct path when the source and receiver are inside
+    // the room)
+    const float att_max = __int_as_float(misc[9]) * __int_as_float(misc[10]) * __int_as_float(misc[11]);
+    float dmin;
+    {
+        const float dx = __int_as_float(misc[13]), dy = __int_as_float(misc[14]), dz = __int_as_float(misc[15]);
+        dmin = sqrtf(dx * dx + dy * dy + dz * dz);
+    }
     float scale, inv_scale;
     {
-        // |amp T_m| <= max|att| / (4 pi d_min), d_min from the smallest offset per axis (the
-        // direct path when the source and receiver are inside the room): < 2^30 after
-        // scaling, 64-bit sums
-        const float dx = __int_as_float(misc[13]), dy = __int_as_float(misc[14]), dz = __int_as_float(misc[15]);
-        const float bound = __int_as_float(misc[9]) * __int_as_float(misc[10]) * __int_as_float(misc[11]) *
-                            __builtin_amdgcn_rcpf(12.566370614359172f * sqrtf(dx * dx + dy * dy + dz * dz));
-        const int top = LUT ? 24 : 29;  // LUT: 32-bit sums with room for 128 images of the bound
+        // one scale per item: < 2^30 after scaling, 64-bit sums; the LUT's 32-bit sums have
+        // room for 128 images of the bound.  Packed sums take a scale per tile (below).
+        const float bound = att_max * __builtin_amdgcn_rcpf(12.566370614359172f * dmin);
+        const int top = LUT ? 24 : 29;
         int E = (bound > 0.0f && bound < 3.0e38f) ? top - ilogbf(bound) : 0;
-        E = min(max(E, -120), 120);
+        E = min(max(E, -90), 120);  // 2^32 / scale stays finite in the conversion
         scale = ldexpf(1.0f, E);
         inv_scale = ldexpf(1.0f, -E);
     }
@@ -989,47 +1046,93 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         if (tid == 0) rowStart[0] = 0;
         __syncthreads();
         const int nrows = carry;
+        // row `row` of the tile: its x and y entries, x^2 + y^2, and its z runs (first
+        // entry, count) in the distance shell: the image count of the row
+        auto row_runs = [&](int row, int* zf, int* zc, double& b2, int& nx, int& ny) -> int {
+            int lo = 0, hi = Nx;  // rowStart[lo] <= row < rowStart[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (rowStart[mid] <= row) lo = mid;
+                else hi = mid;
+            }
+            nx = lo;
+            const int j = row - rowStart[lo];
+            const int c0 = xrun[(ntab + 4) + nx];
+            ny = j < c0 ? xrun[nx] + 2 * j : xrun[2 * (ntab + 4) + nx] + 2 * (j - c0);
+            const double vx = offs[nx], vy = offs[Nx + ny];
+            b2 = vx * vx;
+            b2 = b2 + vy * vy;
+            // z entries with d2lo - b2 <= z^2 <= d2hi - b2
+            const double V2 = d2hi - b2, U2 = d2lo - b2;
+            if (V2 >= 0.0) {
+                const double V = sqrt(V2);
+                if (U2 <= 0.0) {
+                    axis_range(AZ, 0, -V, V, zf[0], zc[0], BIG);
+                    axis_range(AZ, 1, -V, V, zf[2], zc[2], BIG);
+                } else {
+                    const double U = sqrt(U2);
+                    // positive run first, then the negative run capped below it
+                    // (disjoint even where the widened bounds meet near 0)
+                    axis_range(AZ, 0, U, V, zf[1], zc[1], BIG);
+                    axis_range(AZ, 0, -V, -U, zf[0], zc[0], (zf[1] - AZ.half) / 2 - 1);
+                    axis_range(AZ, 1, U, V, zf[3], zc[3], BIG);
+                    axis_range(AZ, 1, -V, -U, zf[2], zc[2], (zf[3] - AZ.half + 1) / 2 - 1);
+                }
+            }
+            return zc[0] + zc[1] + zc[2] + zc[3];
+        };
+        // PK: the tile's image count bounds every field's sum, so it is counted before the
+        // first add (by a separate pass only when the rows take several rounds)
+        int tile_images = 0;
+        if (PK && nrows > WGB) {
+            for (int r0 = 0; r0 < nrows; r0 += WGB) {
+                int zf[4] = {0, 0, 0, 0}, zc[4] = {0, 0, 0, 0};
+                double b2;
+                int nx, ny;
+                const int cnt = r0 + tid < nrows ? row_runs(r0 + tid, zf, zc, b2, nx, ny) : 0;
+                int tot;
+                block_scan_excl(cnt, misc, &tot);
+                tile_images += tot;
+            }
+        }
+        float pscale = 1.0f;  // PK: the scale of row 0 (row m: pscale 2^-(F_0 - F_m))
         for (int r0 = 0; r0 < nrows; r0 += WGB) {
             const int row = r0 + tid;
             int cnt = 0;
             int zf[4] = {0, 0, 0, 0}, zc[4] = {0, 0, 0, 0};
             if (row < nrows) {
-                int lo = 0, hi = Nx;  // rowStart[lo] <= row < rowStart[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (rowStart[mid] <= row) lo = mid;
-                    else hi = mid;
-                }
-                const int nx = lo, j = row - rowStart[lo];
-                const int c0 = xrun[(ntab + 4) + nx];
-                const int ny = j < c0 ? xrun[nx] + 2 * j : xrun[2 * (ntab + 4) + nx] + 2 * (j - c0);
-                const double vx = offs[nx], vy = offs[Nx + ny];
-                double b2 = vx * vx;
-                b2 = b2 + vy * vy;
-                // z entries with d2lo - b2 <= z^2 <= d2hi - b2
-                const double V2 = d2hi - b2, U2 = d2lo - b2;
-                if (V2 >= 0.0) {
-                    const double V = sqrt(V2);
-                    if (U2 <= 0.0) {
-                        axis_range(AZ, 0, -V, V, zf[0], zc[0], BIG);
-                        axis_range(AZ, 1, -V, V, zf[2], zc[2], BIG);
-                    } else {
-                        const double U = sqrt(U2);
-                        // positive run first, then the negative run capped below it
-                        // (disjoint even where the widened bounds meet near 0)
-                        axis_range(AZ, 0, U, V, zf[1], zc[1], BIG);
-                        axis_range(AZ, 0, -V, -U, zf[0], zc[0], (zf[1] - AZ.half) / 2 - 1);
-                        axis_range(AZ, 1, U, V, zf[3], zc[3], BIG);
-                        axis_range(AZ, 1, -V, -U, zf[2], zc[2], (zf[3] - AZ.half + 1) / 2 - 1);
-                    }
-                }
-                cnt = zc[0] + zc[1] + zc[2] + zc[3];
+                double b2;
+                int nx, ny;
+                cnt = row_runs(row, zf, zc, b2, nx, ny);
                 rb2[tid] = b2;
                 ratt[tid] = att[nx] * att[Nx + ny];
                 rxy[tid] = nx | (ny << 16);
             }
             int tot;
             const int ex = block_scan_excl(cnt, misc, &tot);
+            if (PK && r0 == 0) {
+                if (nrows <= WGB) tile_images = tot;
+                // every image of the tile lies at d >= max(d_min, rlo): |amp| <= amax, and a
+                // field's sum is at most tile_images (amax scale + 1/2).  The scale of the
+                // narrowest field F_min keeps that below 2^(F_min - 1); a wider field's scale
+                // is larger by 2^(F - F_min); row 0 (widest) also stays below 2^30 per image
+                // for the int32 conversion.  The factor 1 + 2^-10 covers the FP32 rounding of
+                // the amplitudes, the recurrence and the bound itself.
+                constexpr int F0 = pack_field_bits(NB, 0), FMIN = pack_lo_bits(NB, 0);
+                const float rt = fmaxf(dmin, (float)rlo);
+                const float amax = att_max * __builtin_amdgcn_rcpf(12.566370614359172f * rt) * (1.0f + 0x1p-10f);
+                const float tb = (float)tile_images * amax * (1.0f + 0x1p-10f);
+                const float room_f = 0x1p-1f * ldexpf(1.0f, FMIN) - (float)tile_images - 2.0f;
+                int E = 0;
+                if (tile_images > 0 && room_f > 0.0f && tb > 0.0f && amax < 3.0e38f) {
+                    const float q = room_f / tb;
+                    E = q < 3.0e38f ? ilogbf(q) + (F0 - FMIN) : 120;
+                    E = min(E, 29 - ilogbf(amax));
+                }
+                E = min(max(E, -90), 120);
+                pscale = ldexpf(1.0f, E);
+                inv_scale = ldexpf(1.0f, -E);
+            }
             for (int q0 = 0; q0 < tot; q0 += QBK) {
                 // Entry p of the round (run g of row slot, run start s) has nz = base + 2 p with
                 // base = zf[g] - 2 (s - q0) in [-4094, 4095].  Each run writes the key
@@ -1051,6 +1154,9 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 }
                 __syncthreads();
                 const int nq = min(QBK, tot - q0);
+                // a wave whose entries all lie past the round's end has nothing to do (its
+                // queue positions hold no marks)
+                if (QSEG * wave < nq) {
                 int keys[QEPT];
                 int kmax = 0;
 #pragma unroll
@@ -1090,7 +1196,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                         amp = apply_patterns(R, amp, offs[nx], offs[Nx + ny], vz, distf, nx, ny, nz, a.spkr, a.mic,
                                              os, orr);
                     }
-                    ev[i] = live ? amp * scale : 0.0f;
+                    ev[i] = live ? amp * (PK ? pscale : scale) : 0.0f;
                     en0[i] = n0 - lo_n0;
                     if (LUT) {
                         // 16 (tau - n0) = p + w on the table's grid (exact in FP64)
@@ -1117,6 +1223,27 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     }
                     const float u = eu[i];
                     const float u2 = u + u;
+                    if (PK) {
+                        // T_m by the three-term recurrence at row 0's scale; row m's field
+                        // takes 2^-(F_0 - F_m) of it (pack_field_bits)
+                        float b[NB];
+                        b[0] = v;
+                        b[1] = v * u;
+#pragma unroll
+                        for (int m = 2; m < NB; m++) b[m] = fmaf(u2, b[m - 1], -b[m - 2]);
+                        unsigned long long* wp = Dw + en0[i];
+                        constexpr int F0 = pack_field_bits(NB, 0);
+                        lds_add_pair<pack_lo_bits(NB, 0)>(wp, b[0], b[NB - 1] * exp2_neg<F0 - pack_field_bits(NB, NB - 1)>());
+                        lds_add_pair<pack_lo_bits(NB, 1)>(wp + DL, b[1] * exp2_neg<F0 - pack_field_bits(NB, 1)>(),
+                                                          b[NB - 2] * exp2_neg<F0 - pack_field_bits(NB, NB - 2)>());
+                        lds_add_pair<pack_lo_bits(NB, 2)>(wp + 2 * DL, b[2] * exp2_neg<F0 - pack_field_bits(NB, 2)>(),
+                                                          b[NB - 3] * exp2_neg<F0 - pack_field_bits(NB, NB - 3)>());
+                        if (NB == 8)
+                            lds_add_pair<pack_lo_bits(NB, 3)>(wp + 3 * DL,
+                                                              b[3] * exp2_neg<F0 - pack_field_bits(NB, 3)>(),
+                                                              b[4] * exp2_neg<F0 - pack_field_bits(NB, 4)>());
+                        continue;
+                    }
                     long long* dp = D + en0[i];
                     float b0 = v, b1 = v * u;
                     lds_add(dp, b0);
@@ -1128,6 +1255,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                         b0 = b1;
                         b1 = b2;
                     }
+                }
                 }
                 __syncthreads();
             }
@@ -1142,14 +1270,39 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             float fv[NB][BCS];
             float mx = 0.0f;
 #pragma unroll
-            for (int m = 0; m < NB; m++)
+            for (int sl = 0; sl < BCS; sl++) {
+                const int d = min(tid + sl * WGB, DL - 1);  // columns past the row are not written
+                if (PK) {
+                    // row m's scale is 2^-(F_0 - F_m) of row 0's; a high field is the word
+                    // minus its sign-extended low field, a multiple of 2^S
+                    constexpr int F0 = pack_field_bits(NB, 0);
 #pragma unroll
-                for (int sl = 0; sl < BCS; sl++) {
-                    const int d = tid + sl * WGB;
-                    if (LUT) fv[m][sl] = d < DL ? (float)D32[m * DL + d] * inv_scale : 0.0f;
-                    else fv[m][sl] = d < DL ? (float)((double)D[m * DL + d] * (double)inv_scale) : 0.0f;
-                    mx = fmaxf(mx, fabsf(fv[m][sl]));
+                    for (int w = 0; w < NB / 2; w++) {
+                        const unsigned long long v = Dw[w * DL + d];
+                        const float ih = inv_scale * (float)(1LL << (F0 - pack_field_bits(NB, w))) *
+                                         (w == 0   ? exp2_neg<pack_lo_bits(NB, 0)>()
+                                          : w == 1 ? exp2_neg<pack_lo_bits(NB, 1)>()
+                                          : w == 2 ? exp2_neg<pack_lo_bits(NB, 2)>()
+                                                   : exp2_neg<pack_lo_bits(NB, 3)>());
+                        const float il = inv_scale * (float)(1LL << (F0 - pack_field_bits(NB, NB - 1 - w)));
+                        long long h, l;
+                        if (w == 0) unpack_pair<pack_lo_bits(NB, 0)>(v, h, l);
+                        else if (w == 1) unpack_pair<pack_lo_bits(NB, 1)>(v, h, l);
+                        else if (w == 2) unpack_pair<pack_lo_bits(NB, 2)>(v, h, l);
+                        else unpack_pair<pack_lo_bits(NB, 3)>(v, h, l);
+                        fv[w][sl] = fix64_to_f32(h, ih);
+                        fv[NB - 1 - w][sl] = (float)l * il;
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NB; m++) {
+                        if (LUT) fv[m][sl] = (float)D32[m * DL + d] * inv_scale;
+                        else fv[m][sl] = fix64_to_f32(D[m * DL + d], inv_scale);
+                    }
                 }
+#pragma unroll
+                for (int m = 0; m < NB; m++) mx = fmaxf(mx, fabsf(fv[m][sl]));
+            }
             if (H16) {
                 // non-negative floats order as ints; one atomic per wave (atomics of the lanes
                 // of a wave on one LDS word run one lane at a time: 512 per tile had cost ~1 ms
@@ -1248,7 +1401,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         for (int t = tid; t < th - tl; t += WGB)
             orow[tl + t] = ((ybuf[t] + ybuf[BT + t]) + ybuf[2 * BT + t]) + ybuf[3 * BT + t];
         __syncthreads();
-        for (int e = tid; e < NB * DL; e += WGB) {
+        for (int e = tid; e < nwords; e += WGB) {
             if (LUT) D32[e] = 0;
             else D[e] = 0;
         }
@@ -1257,18 +1410,18 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     }
 }
 
-// The table mode's kernels with the row length compiled in are held to 128 VGPRs (two
-// workgroups per CU); the same cap on the other kernels schedules them worse.
-template <int NB, int DLC, int H16, bool LUT>
-__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && LUT ? 4 : 1)))
+// The table mode's and the packed sums' kernels with the row length compiled in are held
+// to 128 VGPRs (two workgroups per CU); the same cap on the 64-bit rows schedules them worse.
+template <int NB, int DLC, int H16, bool LUT, bool PK>
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (LUT || PK) ? 4 : 1)))
 void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    basis_item<NB, DLC, H16, LUT>(a, blockIdx.x, smem_raw);
+    basis_item<NB, DLC, H16, LUT, PK>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
-template <int NB, int DLC, int H16, bool LUT>
-__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && LUT ? 4 : 1)))
+template <int NB, int DLC, int H16, bool LUT, bool PK>
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (LUT || PK) ? 4 : 1)))
 void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -1278,7 +1431,7 @@ void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
         const int item = next;
         __syncthreads();
         if (item >= n_items) break;
-        basis_item<NB, DLC, H16, LUT>(a, item, smem_raw);
+        basis_item<NB, DLC, H16, LUT, PK>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -2631,9 +2784,11 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB,
     return c;
 }
 
-size_t basis_lds_bytes(int ntab, const BasisTables& B, bool h16, int nb, bool lut) {
+size_t basis_lds_bytes(int ntab, const BasisTables& B, bool h16, int nb, bool lut, bool pack) {
     const size_t coef = h16 ? 8 * (size_t)nb * (B.cl + 4) : 4 * (size_t)nb * B.cl;
-    const size_t rows = lut ? 4 * (size_t)nb * (B.dl + B.dl / 4) : 8 * (size_t)nb * B.dl;
+    // the region holds the sums and, after the conversion, the FP32 (or 16-bit) rows
+    const size_t f32rows = 4 * (size_t)nb * (B.dl + B.dl / 4);
+    const size_t rows = lut ? f32rows : pack ? std::max(4 * (size_t)nb * B.dl, h16 ? 0 : f32rows) : 8 * (size_t)nb * B.dl;
     return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + rows + coef + (8 + 4 + 4) * (size_t)WGB +
            4 * (size_t)(lut ? QB_LUT : QB) + 4 * 16;
 }
@@ -2726,6 +2881,7 @@ struct SimLaunch {
     bool basis = false;  // accumulation by the basis method (FP32, or FP16 filter inputs)
     int h16 = 0;         // mixed precision: 16-bit inputs of the basis filters (1 FP16, 2 BF16)
     bool lut = false;    // table mode on the basis machinery (16 phase rows)
+    bool pack = false;   // two basis rows per 64-bit sum (RIRs of fewer than PACK_MAX_IMAGES images)
     int nb = 8;          // Chebyshev terms of the basis method
     int prec = PREC_FP32;
     int Tw = 0;
@@ -2795,7 +2951,10 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         }
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
-        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
+        long long most_images = 0;
+        for (const RoomHost& h : rooms) most_images = std::max(most_images, (long long)h.nb[0] * h.nb[1] * h.nb[2]);
+        S.pack = !S.lut && S.h16 == 0 && most_images < PACK_MAX_IMAGES;
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut, S.pack) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
             // scatter takes the launch
@@ -2835,7 +2994,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
             (size_t)(QCAP + QPAD) * 5 * 4 + 32 + (T.n_pass <= 12 ? sizeof(float2) * T.lut.size() : 0);
-    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut);
+    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut, S.pack);
     GPURIR_CHECK(T.n_pass <= 48, "sampling frequency too high: at most 3072 window taps (fs 384 kHz) are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -2936,17 +3095,20 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         AccArgs a = acc_args(S, d_plan, d_part, d_pos_src, d_pos_rcv, d_orv_src, d_orv_rcv, d_out);
         // the 16 kHz window (128 taps) has its row length compiled in
         const bool dl640 = S.Bt.dl == 640;
-#define GPURIR_BASIS_FN(K, N, H, U) \
-    (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H, U>) : reinterpret_cast<const void*>(&K<N, 0, H, U>))
-        const void* fn_once = S.lut        ? GPURIR_BASIS_FN(ism_basis_kernel, LUT_NB, 0, true)
-                              : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2, false)
-                              : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1, false)
-                                           : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0, false);
-        const void* fn_pers = S.lut        ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, LUT_NB, 0, true)
-                              : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2, false)
-                              : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1, false)
-                                           : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 0, false);
+#define GPURIR_BASIS_FN1(K, N, H, U, P) \
+    (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H, U, P>) : reinterpret_cast<const void*>(&K<N, 0, H, U, P>))
+#define GPURIR_BASIS_FN(K, N, H) \
+    (S.pack ? GPURIR_BASIS_FN1(K, N, H, false, true) : GPURIR_BASIS_FN1(K, N, H, false, false))
+        const void* fn_once = S.lut        ? GPURIR_BASIS_FN1(ism_basis_kernel, LUT_NB, 0, true, false)
+                              : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2)
+                              : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1)
+                                           : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0);
+        const void* fn_pers = S.lut        ? GPURIR_BASIS_FN1(ism_basis_persistent_kernel, LUT_NB, 0, true, false)
+                              : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2)
+                              : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1)
+                                           : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 0);
 #undef GPURIR_BASIS_FN
+#undef GPURIR_BASIS_FN1
         int n_items = (int)S.P.items.size();
         int resident = 0;
         if (S.counters) {

@@ -914,6 +914,11 @@ __device__ inline unsigned short half_bits(float x) {
 template <int NB, int DLC, int H16, bool LUT, bool PK>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
+    // SPL: FP32 filters from FP16 pairs (x = hi + lo 2^-11 for the coefficients and the rows,
+    // hi lo' + lo hi' accumulated apart): three v_mfma_f32_16x16x16_f16 per 16 K values
+    // instead of four v_mfma_f32_16x16x4_f32 per 4 K values
+    constexpr bool SPL = H16 == 3;
+    static_assert(!SPL || (NB == 8 && !LUT), "FP16 pairs carry the 8-term FP32 expansion");
     static_assert(!PK || (!LUT && NB % 2 == 0), "packed sums pair the Chebyshev rows");
     constexpr int QBK = LUT ? QB_LUT : QB;
     constexpr int QEPT = QBK / WGB;   // consecutive queue entries per thread
@@ -935,13 +940,13 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     unsigned long long* Dw = reinterpret_cast<unsigned long long*>(D);  // PK: [NB/2][DL] row pairs
     // the rows as FP32 [NB][DLP] after the conversion fit the 64-bit rows; the LUT's 32-bit
     // rows and the packed pairs are given the larger of the two (16-bit rows fit either)
-    const int dreg = (LUT || PK) ? max(NB * DL, H16 ? 0 : NB * DLP) : 2 * NB * DL;  // floats of the row region
+    const int dreg = (LUT || PK) ? max(NB * DL, (H16 == 1 || H16 == 2) ? 0 : NB * DLP) : 2 * NB * DL;  // floats
     const int nwords = PK ? NB / 2 * DL : NB * DL;  // 64-bit (or LUT 32-bit) words of the sums
     float* ctab = reinterpret_cast<float*>(D) + dreg;                 // [NB][CL] (H16: see chs)
     // H16: four copies of the coefficients in FP16, copy rho shifted by rho entries, so that a
     // lane's four A operands start at a multiple of 4 halves: [4][NB][CL + 4]
     unsigned short* chs = reinterpret_cast<unsigned short*>(ctab);
-    const int crw = H16 ? 2 * NB * (CL + 4) : NB * CL;                 // floats of the region
+    const int crw = H16 ? (SPL ? 4 : 2) * NB * (CL + 4) : NB * CL;    // floats of the region (SPL: hi, lo)
     double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
@@ -966,7 +971,16 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         for (int e = tid; e < 4 * NB * (CL + 4); e += WGB) {
             const int rho = e / (NB * (CL + 4)), rm = e - rho * (NB * (CL + 4));
             const int m = rm / (CL + 4), z = rm - m * (CL + 4), src = z - rho;
-            chs[e] = half_bits<H16>((src >= 0 && src < CL) ? a.ctab[m * CL + src] : 0.0f);
+            const float cv = (src >= 0 && src < CL) ? a.ctab[m * CL + src] : 0.0f;
+            if (SPL) {
+                // |C| < 1: times 2^14 inside the FP16 range; lo is the rounding error times 2^11
+                const float vs = cv * 16384.0f;
+                const _Float16 hi = (_Float16)vs;
+                chs[e] = __builtin_bit_cast(unsigned short, hi);
+                chs[e + 4 * NB * (CL + 4)] = __builtin_bit_cast(unsigned short, (_Float16)((vs - (float)hi) * 2048.0f));
+            } else {
+                chs[e] = half_bits<H16>(cv);
+            }
         }
         if (tid == 0) misc[12] = 0;
     } else {
@@ -1291,7 +1305,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                         else if (w == 2) unpack_pair<pack_lo_bits(NB, 2)>(v, h, l);
                         else unpack_pair<pack_lo_bits(NB, 3)>(v, h, l);
                         fv[w][sl] = fix64_to_f32(h, ih);
-                        fv[NB - 1 - w][sl] = (float)l * il;
+                        fv[NB - 1 - w][sl] = (float)(int)l * il;
                     }
                 } else {
 #pragma unroll
@@ -1317,7 +1331,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 const float big = __int_as_float(misc[12]);
                 const int e = (big > 0.0f && big < 3.0e38f) ? 14 - ilogbf(big) : 0;
                 sc = ldexpf(1.0f, min(max(e, -120), 120));
-                out_scale = ldexpf(1.0f, -min(max(e, -120), 120));
+                out_scale = ldexpf(1.0f, -min(max(e, -120), 120) - (SPL ? 14 : 0));
             }
 #pragma unroll
             for (int m = 0; m < NB; m++)
@@ -1325,7 +1339,15 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 for (int sl = 0; sl < BCS; sl++) {
                     const int d = tid + sl * WGB;
                     if (d < DL) {
-                        if (H16) Dh[m * DLP + d + 4 * (d >> 4)] = half_bits<H16>(fv[m][sl] * sc);
+                        if (SPL) {
+                            const float xs = fv[m][sl] * sc;
+                            const _Float16 hi = (_Float16)xs;
+                            Dh[m * DLP + d + 4 * (d >> 4)] = __builtin_bit_cast(unsigned short, hi);
+                            Dh[(NB + m) * DLP + d + 4 * (d >> 4)] =
+                                __builtin_bit_cast(unsigned short, (_Float16)((xs - (float)hi) * 2048.0f));
+                        } else if (H16) {
+                            Dh[m * DLP + d + 4 * (d >> 4)] = half_bits<H16>(fv[m][sl] * sc);
+                        }
                         else Df[m * DLP + d + 4 * (d >> 4)] = fv[m][sl];
                     }
                 }
@@ -1339,6 +1361,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             const int ot = wave % NOT, mq = wave / NOT;
             const int c = lane & 15, g = lane >> 4;
             floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            floatx4 accx = {0.0f, 0.0f, 0.0f, 0.0f};  // SPL: the cross terms, 2^-11 of acc's units
             if (H16) {
                 // four k per lane and instruction: K steps of 4 per lane group, KS / 4 products
                 const int rho = c & 3;
@@ -1352,7 +1375,15 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     const int kb = 256 * ot + 16 * c + g * KS;
                     for (int t4 = 0; t4 < KS / 4; t4++) {
                         const int dl = kb + 4 * t4;
-                        if (H16 == 2) {
+                        if (SPL) {
+                            const half4_t xh = *reinterpret_cast<const half4_t*>(Bp + dl + 4 * (dl >> 4));
+                            const half4_t xl = *reinterpret_cast<const half4_t*>(Bp + NB * DLP + dl + 4 * (dl >> 4));
+                            const half4_t ah = *reinterpret_cast<const half4_t*>(Ap + 4 * t4);
+                            const half4_t al = *reinterpret_cast<const half4_t*>(Ap + 4 * NB * (CL + 4) + 4 * t4);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xh, acc, 0, 0, 0);
+                            accx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xl, accx, 0, 0, 0);
+                            accx = __builtin_amdgcn_mfma_f32_16x16x16f16(al, xh, accx, 0, 0, 0);
+                        } else if (H16 == 2) {
                             const short4_t xv = *reinterpret_cast<const short4_t*>(Bp + dl + 4 * (dl >> 4));
                             const short4_t av = *reinterpret_cast<const short4_t*>(Ap + 4 * t4);
                             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, xv, acc, 0, 0, 0);
@@ -1363,7 +1394,46 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                         }
                     }
                 }
+                if (SPL) acc = acc + accx * (1.0f / 2048.0f);
                 acc = acc * out_scale;
+            } else if constexpr (DLC == 640 && NB % 4 == 0) {
+                // The 16 kHz row length pins KS = 36: the wave's NB/4 rows are nine K steps each,
+                // taken in groups of three.  The operands of group q + 1 (three X vectors and 12
+                // coefficients per lane) are read before the 12 products of group q, so each
+                // read's latency hides behind 384 cycles of products.
+                constexpr int KSC = 36, GQ = 3, NGR = KSC / 4 / GQ, NR = NB / 4;
+                const int m0 = mq * NR;
+                const float* Ap0 = ctab + m0 * CL + g * KSC + 16 - c;
+                const float* Bp0 = Df + m0 * DLP;
+                const int kb = 256 * ot + 16 * c + g * KSC;
+                float4 xb[2][GQ];
+                float ab[2][4 * GQ];
+                auto load_group = [&](int q, float4* xv, float* av) {
+                    const int r = q / NGR, t0 = (q % NGR) * GQ;
+#pragma unroll
+                    for (int t = 0; t < GQ; t++) {
+                        const int dl = kb + 4 * (t0 + t);
+                        xv[t] = *reinterpret_cast<const float4*>(Bp0 + r * DLP + dl + 4 * (dl >> 4));
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4 * GQ; i++) av[i] = Ap0[r * CL + 4 * t0 + i];
+                };
+                load_group(0, xb[0], ab[0]);
+#pragma unroll
+                for (int q = 0; q < NR * NGR; q++) {
+                    if (q + 1 < NR * NGR) load_group(q + 1, xb[(q + 1) & 1], ab[(q + 1) & 1]);
+                    __builtin_amdgcn_sched_barrier(0);  // the next group's reads stay ahead
+#pragma unroll
+                    for (int t = 0; t < GQ; t++) {
+                        const float4 x = xb[q & 1][t];
+                        const float* av = ab[q & 1] + 4 * t;
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], x.x, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], x.y, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], x.z, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], x.w, acc, 0, 0, 0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             } else
 #pragma unroll
             for (int mm = 0; mm < (NB + 3) / 4; mm++) {
@@ -1410,10 +1480,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     }
 }
 
-// The table mode's and the packed sums' kernels with the row length compiled in are held
-// to 128 VGPRs (two workgroups per CU); the same cap on the 64-bit rows schedules them worse.
+// The FP32 and table-mode kernels with the row length compiled in are held to 128 VGPRs
+// (two workgroups per CU); the same cap on the 16-bit-input kernels schedules them worse.
 template <int NB, int DLC, int H16, bool LUT, bool PK>
-__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (LUT || PK) ? 4 : 1)))
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3) ? 4 : 1)))
 void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     basis_item<NB, DLC, H16, LUT, PK>(a, blockIdx.x, smem_raw);
@@ -1421,7 +1491,7 @@ void ism_basis_kernel(AccArgs a) {
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
 template <int NB, int DLC, int H16, bool LUT, bool PK>
-__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (LUT || PK) ? 4 : 1)))
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3) ? 4 : 1)))
 void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -2784,11 +2854,14 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB,
     return c;
 }
 
-size_t basis_lds_bytes(int ntab, const BasisTables& B, bool h16, int nb, bool lut, bool pack) {
-    const size_t coef = h16 ? 8 * (size_t)nb * (B.cl + 4) : 4 * (size_t)nb * B.cl;
+size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut, bool pack) {
+    // 16-bit coefficients in four shifted copies (FP16 pairs: hi and lo), or FP32
+    const size_t coef = h16 ? (h16 == 3 ? 16 : 8) * (size_t)nb * (B.cl + 4) : 4 * (size_t)nb * B.cl;
     // the region holds the sums and, after the conversion, the FP32 (or 16-bit) rows
     const size_t f32rows = 4 * (size_t)nb * (B.dl + B.dl / 4);
-    const size_t rows = lut ? f32rows : pack ? std::max(4 * (size_t)nb * B.dl, h16 ? 0 : f32rows) : 8 * (size_t)nb * B.dl;
+    const size_t rows = lut    ? f32rows
+                        : pack ? std::max(4 * (size_t)nb * B.dl, (h16 == 1 || h16 == 2) ? 0 : f32rows)
+                               : 8 * (size_t)nb * B.dl;
     return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + rows + coef + (8 + 4 + 4) * (size_t)WGB +
            4 * (size_t)(lut ? QB_LUT : QB) + 4 * 16;
 }
@@ -2879,7 +2952,7 @@ struct SimLaunch {
     TapTables T;
     BasisTables Bt;
     bool basis = false;  // accumulation by the basis method (FP32, or FP16 filter inputs)
-    int h16 = 0;         // mixed precision: 16-bit inputs of the basis filters (1 FP16, 2 BF16)
+    int h16 = 0;         // basis filter inputs: 16-bit (1 FP16, 2 BF16) or FP16 pairs for FP32 (3)
     bool lut = false;    // table mode on the basis machinery (16 phase rows)
     bool pack = false;   // two basis rows per 64-bit sum (RIRs of fewer than PACK_MAX_IMAGES images)
     int nb = 8;          // Chebyshev terms of the basis method
@@ -2926,9 +2999,11 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     GPURIR_CHECK(S.prec != PREC_BF16 || opt.method != METHOD_TAPS,
                  "the BF16 mode runs on the matrix cores only (method basis or auto)");
     S.basis = opt.method != METHOD_TAPS;
-    S.h16 = S.prec == PREC_FP16 ? 1 : (S.prec == PREC_BF16 ? 2 : 0);
+    // basis filters: FP16 (1) or BF16 (2) inputs in the mixed-precision modes, FP16 pairs (3)
+    // for FP32
+    S.h16 = S.prec == PREC_FP16 ? 1 : (S.prec == PREC_BF16 ? 2 : (S.prec == PREC_FP32 ? 3 : 0));
     S.lut = S.prec == PREC_LUT;
-    S.nb = S.lut ? LUT_NB : (S.h16 ? BASIS_NB_FP16 : BASIS_NB);
+    S.nb = S.lut ? LUT_NB : ((S.h16 == 1 || S.h16 == 2) ? BASIS_NB_FP16 : BASIS_NB);
     if (S.basis && opt.method == METHOD_AUTO && S.prec != PREC_BF16) {
         // a launch of fewer tiles than the device has CUs (e.g. one room with two
         // receivers) is faster as a tap scatter, whose image chunks fill the device
@@ -2953,7 +3028,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         long long most_images = 0;
         for (const RoomHost& h : rooms) most_images = std::max(most_images, (long long)h.nb[0] * h.nb[1] * h.nb[2]);
-        S.pack = !S.lut && S.h16 == 0 && most_images < PACK_MAX_IMAGES;
+        S.pack = !S.lut && (S.h16 == 0 || S.h16 == 3) && most_images < PACK_MAX_IMAGES;
         if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut, S.pack) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
@@ -3102,10 +3177,12 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
         const void* fn_once = S.lut        ? GPURIR_BASIS_FN1(ism_basis_kernel, LUT_NB, 0, true, false)
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1)
+                              : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 3)
                                            : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0);
         const void* fn_pers = S.lut        ? GPURIR_BASIS_FN1(ism_basis_persistent_kernel, LUT_NB, 0, true, false)
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1)
+                              : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 3)
                                            : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 0);
 #undef GPURIR_BASIS_FN
 #undef GPURIR_BASIS_FN1

@@ -1362,7 +1362,44 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             const int c = lane & 15, g = lane >> 4;
             floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
             floatx4 accx = {0.0f, 0.0f, 0.0f, 0.0f};  // SPL: the cross terms, 2^-11 of acc's units
-            if (H16) {
+            if constexpr (SPL && DLC == 640) {
+                // FP16 pairs at 16 kHz (KS = 36, CL = 160): the wave's two rows are nine K
+                // steps each; the four operand reads of step q + 1 are issued before the three
+                // products of step q
+                constexpr int KSC = 36, CLC = 4 * KSC + 16, GQ = 1, NGR = KSC / 4 / GQ, NR = NB / 4;
+                const int rho = c & 3, m0 = mq * NR;
+                const unsigned short* Ap0 = chs + (rho * NB + m0) * (CLC + 4) + g * KSC + 16 - c + rho;
+                const unsigned short* Bp0 = Dh + m0 * DLP;
+                const int kb = 256 * ot + 16 * c + g * KSC;
+                half4_t xh[2][GQ], xl[2][GQ], ah[2][GQ], al[2][GQ];
+                auto load_group = [&](int q, int b) {
+                    const int r = q / NGR, t0 = (q % NGR) * GQ;
+#pragma unroll
+                    for (int t = 0; t < GQ; t++) {
+                        const int dl = kb + 4 * (t0 + t);
+                        const unsigned short* xp = Bp0 + r * DLP + dl + 4 * (dl >> 4);
+                        const unsigned short* ap = Ap0 + r * (CLC + 4) + 4 * (t0 + t);
+                        xh[b][t] = *reinterpret_cast<const half4_t*>(xp);
+                        xl[b][t] = *reinterpret_cast<const half4_t*>(xp + NB * DLP);
+                        ah[b][t] = *reinterpret_cast<const half4_t*>(ap);
+                        al[b][t] = *reinterpret_cast<const half4_t*>(ap + 4 * NB * (CLC + 4));
+                    }
+                };
+                load_group(0, 0);
+#pragma unroll
+                for (int q = 0; q < NR * NGR; q++) {
+                    if (q + 1 < NR * NGR) load_group(q + 1, (q + 1) & 1);
+                    __builtin_amdgcn_sched_barrier(0);  // the next group's reads stay ahead
+#pragma unroll
+                    for (int t = 0; t < GQ; t++) {
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1][t], xh[q & 1][t], acc, 0, 0, 0);
+                        accx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1][t], xl[q & 1][t], accx, 0, 0, 0);
+                        accx = __builtin_amdgcn_mfma_f32_16x16x16f16(al[q & 1][t], xh[q & 1][t], accx, 0, 0, 0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                acc = (acc + accx * (1.0f / 2048.0f)) * out_scale;
+            } else if (H16) {
                 // four k per lane and instruction: K steps of 4 per lane group, KS / 4 products
                 const int rho = c & 3;
 #pragma unroll

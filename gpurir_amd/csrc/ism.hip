@@ -968,20 +968,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     }
     for (int e = tid; e < QBK; e += WGB) queue[e] = 0;  // run marks: zero between rounds
     if (H16) {
-        for (int e = tid; e < 4 * NB * (CL + 4); e += WGB) {
-            const int rho = e / (NB * (CL + 4)), rm = e - rho * (NB * (CL + 4));
-            const int m = rm / (CL + 4), z = rm - m * (CL + 4), src = z - rho;
-            const float cv = (src >= 0 && src < CL) ? a.ctab[m * CL + src] : 0.0f;
-            if (SPL) {
-                // |C| < 1: times 2^14 inside the FP16 range; lo is the rounding error times 2^11
-                const float vs = cv * 16384.0f;
-                const _Float16 hi = (_Float16)vs;
-                chs[e] = __builtin_bit_cast(unsigned short, hi);
-                chs[e + 4 * NB * (CL + 4)] = __builtin_bit_cast(unsigned short, (_Float16)((vs - (float)hi) * 2048.0f));
-            } else {
-                chs[e] = half_bits<H16>(cv);
-            }
-        }
+        // the plan holds the copies' LDS image (make_half_copies): 16-byte copies
+        const uint4* src = reinterpret_cast<const uint4*>(a.ctab);
+        uint4* dst = reinterpret_cast<uint4*>(chs);
+        for (int e = tid; e < crw / 4; e += WGB) dst[e] = src[e];
         if (tid == 0) misc[12] = 0;
     } else {
         for (int e = tid; e < NB * CL; e += WGB) ctab[e] = a.ctab[e];
@@ -2872,6 +2862,36 @@ BasisTables make_lut_basis_tables(const TapTables& T) {
     return B;
 }
 
+// LDS image of the 16-bit coefficient copies of the basis kernels with 16-bit filter inputs:
+// copy rho < 4 of row m holds C[m][z - rho] at z < cl + 4 (zero outside [0, cl)), so that a
+// lane's four operands start at a multiple of four halves.  h16 = 1: FP16; 2: BF16 (rounded
+// to nearest even); 3: FP16 pairs of 2^14 C, the hi copies followed by the lo copies (the
+// rounding error of hi times 2^11).  The kernel copies the image into LDS unchanged.
+std::vector<unsigned short> make_half_copies(const BasisTables& B, int nb, int h16) {
+    const size_t per = 4 * (size_t)nb * (B.cl + 4);
+    std::vector<unsigned short> img((h16 == 3 ? 2 : 1) * per, 0);
+    for (size_t e = 0; e < per; e++) {
+        const int rho = (int)(e / ((size_t)nb * (B.cl + 4)));
+        const int rm = (int)(e - (size_t)rho * nb * (B.cl + 4));
+        const int m = rm / (B.cl + 4), z = rm - m * (B.cl + 4), src = z - rho;
+        const float cv = (src >= 0 && src < B.cl) ? B.ctab[(size_t)m * B.cl + src] : 0.0f;
+        if (h16 == 3) {
+            const float vs = cv * 16384.0f;
+            const _Float16 hi = (_Float16)vs;
+            img[e] = __builtin_bit_cast(unsigned short, hi);
+            img[e + per] = __builtin_bit_cast(unsigned short, (_Float16)((vs - (float)hi) * 2048.0f));
+        } else if (h16 == 2) {
+            unsigned u;
+            std::memcpy(&u, &cv, 4);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            img[e] = (unsigned short)(u >> 16);
+        } else {
+            img[e] = __builtin_bit_cast(unsigned short, (_Float16)cv);
+        }
+    }
+    return img;
+}
+
 // Chebyshev interpolation coefficients of every tap offset for the basis adjoint: row m
 // holds 16 zeros, then C[m][k] for k = j - J0 at 16 + k, then zeros up to cl (FP64 on the
 // host), the Toeplitz operand of its correlation products.
@@ -2992,6 +3012,7 @@ struct SimLaunch {
     int h16 = 0;         // basis filter inputs: 16-bit (1 FP16, 2 BF16) or FP16 pairs for FP32 (3)
     bool lut = false;    // table mode on the basis machinery (16 phase rows)
     bool pack = false;   // two basis rows per 64-bit sum (RIRs of fewer than PACK_MAX_IMAGES images)
+    std::vector<unsigned short> chs;  // h16: LDS image of the 16-bit coefficient copies (make_half_copies)
     int nb = 8;          // Chebyshev terms of the basis method
     int prec = PREC_FP32;
     int Tw = 0;
@@ -3077,8 +3098,18 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     }
     if (S.basis) {
         S.P = make_plan_basis(rooms, opt, S.T);
+        if (S.h16) {
+            static std::mutex mu;
+            static std::map<std::tuple<int, int, int>, std::vector<unsigned short>> cache;
+            std::lock_guard<std::mutex> lk(mu);
+            const auto key = std::make_tuple(S.Tw, S.nb, S.h16);
+            auto f = cache.find(key);
+            if (f == cache.end()) f = cache.emplace(key, make_half_copies(S.Bt, S.nb, S.h16)).first;
+            S.chs = f->second;
+        }
     } else {
         S.Bt = BasisTables();
+        S.chs.clear();
         S.P = make_plan(rooms, opt, S.T, true);
     }
     const Plan& P = S.P;
@@ -3090,7 +3121,9 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.off_lut = align_up(S.off_jt + sizeof(float4) * T.jtab.size(), 256);
     S.off_sb = align_up(S.off_lut + sizeof(float2) * (T.lut.size() + 1), 256);
     S.off_ct = align_up(S.off_sb + sizeof(int2) * (P.sum_blocks.size() + 1), 256);
-    S.plan_bytes = align_up(S.off_ct + sizeof(float) * (S.Bt.ctab.size() + 1), 256);
+    // the coefficients: FP32 rows, or the LDS image of their 16-bit copies
+    const size_t ct_bytes = S.chs.empty() ? sizeof(float) * S.Bt.ctab.size() : sizeof(unsigned short) * S.chs.size();
+    S.plan_bytes = align_up(S.off_ct + ct_bytes + sizeof(float), 256);
     S.blob.assign(S.plan_bytes, 0);
     std::memcpy(S.blob.data() + S.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(S.blob.data() + S.off_items, P.items.data(), sizeof(Item) * P.items.size());
@@ -3099,7 +3132,9 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     if (!T.lut.empty()) std::memcpy(S.blob.data() + S.off_lut, T.lut.data(), sizeof(float2) * T.lut.size());
     if (!P.sum_blocks.empty())
         std::memcpy(S.blob.data() + S.off_sb, P.sum_blocks.data(), sizeof(int2) * P.sum_blocks.size());
-    if (!S.Bt.ctab.empty())
+    if (!S.chs.empty())
+        std::memcpy(S.blob.data() + S.off_ct, S.chs.data(), ct_bytes);
+    else if (!S.Bt.ctab.empty())
         std::memcpy(S.blob.data() + S.off_ct, S.Bt.ctab.data(), sizeof(float) * S.Bt.ctab.size());
     S.ntab = (P.ntab_max + 3) / 4 * 4;  // keeps every LDS array after the tables 16-byte aligned
     // acc index = n0 - t_lo + J1 + (j - J0), n0 <= t_hi - 1 - J0, j - J0 < 64 n_pass

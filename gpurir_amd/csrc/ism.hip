@@ -1270,7 +1270,63 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         // d = tid + s WGB (s < BCS) of every row into registers, and after the barrier
         // writes them to m DLP + d + 4 (d >> 4)
         float out_scale = 1.0f;  // H16: the power of two that undoes the tile's FP16 scale
-        {
+        if constexpr (PK && DLC == 640 && NB == 8) {
+            // 16 kHz packed sums: word e = tid + 512 k (k < 5) of the [4][640] words, so every
+            // thread converts five words; e / 640 is the same across a wave (512 and 640 are
+            // multiples of 64).  High fields: the word minus its sign-extended low field
+            // (row 0's scale 2^-24 of it for every word); low fields: 2^(F_0 - S) of row 0's.
+            constexpr int F0 = pack_field_bits(NB, 0), KW = NB / 2 * 640 / WGB;
+            static_assert(KW * WGB == NB / 2 * 640, "whole words per thread");
+            float fh[KW], fl[KW];
+            float mx = 0.0f;
+#pragma unroll
+            for (int k = 0; k < KW; k++) {
+                const int e = tid + WGB * k;
+                const int w = __builtin_amdgcn_readfirstlane(e / 640);
+                const int S = w == 0 ? pack_lo_bits(NB, 0) : w == 1 ? pack_lo_bits(NB, 1) : w == 2 ? pack_lo_bits(NB, 2) : pack_lo_bits(NB, 3);
+                const unsigned long long v = Dw[e];
+                const int lo = __builtin_amdgcn_sbfe((int)(unsigned)v, 0, S);
+                const long long h = (long long)v - (long long)lo;
+                fh[k] = fix64_to_f32(h, inv_scale * exp2_neg<64 - F0>());
+                fl[k] = (float)lo * (inv_scale * (float)(1 << (F0 - S)));
+                mx = fmaxf(mx, fmaxf(fabsf(fh[k]), fabsf(fl[k])));
+            }
+            if (H16) {
+                const int wm = wave_max_nonneg(__float_as_int(mx));
+                if (lane == 63) atomicMax(misc + 12, wm);
+            }
+            __syncthreads();
+            float sc = 1.0f;
+            if (H16) {
+                const float big = __int_as_float(misc[12]);
+                const int ex = (big > 0.0f && big < 3.0e38f) ? 14 - ilogbf(big) : 0;
+                sc = ldexpf(1.0f, min(max(ex, -120), 120));
+                out_scale = ldexpf(1.0f, -min(max(ex, -120), 120) - (SPL ? 14 : 0));
+            }
+#pragma unroll
+            for (int k = 0; k < KW; k++) {
+                const int e = tid + WGB * k;
+                const int w = __builtin_amdgcn_readfirstlane(e / 640);
+                const int d = e - 640 * w;
+                const int col = d + 4 * (d >> 4);
+#pragma unroll
+                for (int half = 0; half < 2; half++) {
+                    const int m = half ? NB - 1 - w : w;
+                    const float x = half ? fl[k] : fh[k];
+                    if (SPL) {
+                        const float xs = x * sc;
+                        const _Float16 hi = (_Float16)xs;
+                        Dh[m * DLP + col] = __builtin_bit_cast(unsigned short, hi);
+                        Dh[(NB + m) * DLP + col] = __builtin_bit_cast(unsigned short, (_Float16)((xs - (float)hi) * 2048.0f));
+                    } else if (H16) {
+                        Dh[m * DLP + col] = half_bits<H16>(x * sc);
+                    } else {
+                        Df[m * DLP + col] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        } else {
             float fv[NB][BCS];
             float mx = 0.0f;
 #pragma unroll

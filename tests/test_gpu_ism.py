@@ -573,3 +573,36 @@ def test_eager_plan_cache_reuse_and_invalidation():
     want = orc.simulate_rir(**args)
     check_parity(first.cpu().numpy(), want, orc.n_samples(0.1, kw["fs"]))
     assert torch.equal(first, second)
+
+
+def test_dense_reflective_room_packed_sums():
+    """A 2 x 1.6 x 1.4 m room with beta 0.99 and a 0.25 s early part: ~1.1e6 images per RIR
+    (below the 2^21 that selects packed basis sums), up to ~2.5e5 of them in one 512-sample
+    tile's distance shell.  The per-tile scale of the packed sums (two basis rows per 64-bit
+    LDS word, the narrowest field 24 bits) must leave every field room for the shell's whole
+    image count without losing the 1e-5 bar."""
+    L = [2.0, 1.6, 1.4]
+    ps = np.array([[0.3, 0.4, 0.5]], np.float32)
+    pr = np.array([[1.7, 1.2, 0.9], [1.0, 0.8, 0.7]], np.float32)
+    nb = gpuRIR.t2n(0.25, L)
+    assert 5e5 < np.prod(nb) < 2 ** 21
+    kw = dict(room_sz=L, beta=np.full(6, 0.99, np.float32), pos_src=ps, pos_rcv=pr, nb_img=nb, Tmax=0.27,
+              fs=16000.0, Tdiff=0.25)
+    g = T.simulate_rir(**kw, method="basis").cpu().numpy()
+    o = orc.simulate_rir(**kw)
+    check_parity(g, o, orc.n_samples(0.25, 16000.0))
+
+
+def test_more_images_than_packed_sums_take():
+    """nb_img with a product above 2^21: the basis kernel keeps 64-bit sums per row (no packing);
+    parity with the oracle as for every other room."""
+    L = [2.0, 2.0, 2.0]
+    ps = np.array([[0.5, 0.6, 0.7]], np.float32)
+    pr = np.array([[1.5, 1.3, 1.1]], np.float32)
+    nb = gpuRIR.t2n(0.4, L)
+    assert np.prod(nb) >= 2 ** 21
+    kw = dict(room_sz=L, beta=np.full(6, 0.95, np.float32), pos_src=ps, pos_rcv=pr, nb_img=nb, Tmax=0.42,
+              fs=16000.0, Tdiff=0.4)
+    g = T.simulate_rir(**kw, method="basis").cpu().numpy()
+    o = orc.simulate_rir(**kw)
+    check_parity(g, o, orc.n_samples(0.4, 16000.0))

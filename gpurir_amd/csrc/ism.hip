@@ -853,6 +853,29 @@ __device__ inline int block_scan_excl(int v, int* tmp, int* total) {
     return before + x - v;
 }
 
+// block_scan_excl with one barrier: the caller alternates between two tmp arrays, so a
+// scan's writes never meet the reads of the scan before it (whose successor's barrier lies
+// between them).  Every thread must call it; tmp holds NWB ints.
+__device__ inline int block_scan_excl1(int v, int* tmp, int* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = wave_scan_incl(v);
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < NWB; i++) {
+        const int t = tmp[i];
+        before += i < w ? t : 0;
+        all += t;
+    }
+    *total = all;
+    return before + x - v;
+}
+
+// sqrt of a non-negative double for the enumeration's distance bounds (widened by 1e-7 of
+// 2L, so values below 1e-200 may read as 0)
+__device__ inline double sqrt_bound(double x) { return x > 1e-200 ? sqrt_normal(x) : 0.0; }
+
 // Image entries of one axis in closed form: entry n has k = n - N/2 and offset
 // 2 m L + c_q - (rounding) with m = ceil(k / 2), q = k & 1, c_0 = s - r, c_1 = -s - r
 // (axis_entry), increasing in m for each parity.  The entries with
@@ -951,7 +974,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
     int* queue = rxy + WGB;                                           // [QBK] run marks of a round
-    int* misc = queue + QBK;                                          // [16] scan scratch, att maxima
+    int* misc = queue + QBK;  // [32]: 8..15 att maxima, min offsets, tile max; 16..31 scan scratch
     float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
     float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
     unsigned short* Dh = reinterpret_cast<unsigned short*>(D);        // H16: [NB][DLP] (aliases D)
@@ -1015,6 +1038,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     const bool pattern = a.spkr != 0 || a.mic != 0;
     float* orow = a.out + R.out_off + (long long)it.rir * R.nSamples;
     constexpr int BIG = 1 << 30;
+    int scan_par = 0;  // block_scan_excl1 buffer parity (uniform)
 
     for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
         const int th = min(tl + BT, it.t_hi);
@@ -1033,7 +1057,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 const double rem = d2hi - X * X;
                 int f0 = 0, c0 = 0, f1 = 0, c1 = 0;
                 if (rem >= 0.0) {
-                    const double Y = sqrt(rem);
+                    const double Y = sqrt_bound(rem);
                     axis_range(AY, 0, -Y, Y, f0, c0, BIG);
                     axis_range(AY, 1, -Y, Y, f1, c1, BIG);
                 }
@@ -1043,7 +1067,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 c = c0 + c1;
             }
             int tot;
-            const int ex = block_scan_excl(c, misc, &tot);
+            const int ex = block_scan_excl1(c, misc + 16 + 8 * (scan_par++ & 1), &tot);
             if (nx < Nx) rowStart[nx + 1] = carry + ex + c;
             carry += tot;
         }
@@ -1069,12 +1093,12 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             // z entries with d2lo - b2 <= z^2 <= d2hi - b2
             const double V2 = d2hi - b2, U2 = d2lo - b2;
             if (V2 >= 0.0) {
-                const double V = sqrt(V2);
+                const double V = sqrt_bound(V2);
                 if (U2 <= 0.0) {
                     axis_range(AZ, 0, -V, V, zf[0], zc[0], BIG);
                     axis_range(AZ, 1, -V, V, zf[2], zc[2], BIG);
                 } else {
-                    const double U = sqrt(U2);
+                    const double U = sqrt_bound(U2);
                     // positive run first, then the negative run capped below it
                     // (disjoint even where the widened bounds meet near 0)
                     axis_range(AZ, 0, U, V, zf[1], zc[1], BIG);
@@ -1095,7 +1119,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 int nx, ny;
                 const int cnt = r0 + tid < nrows ? row_runs(r0 + tid, zf, zc, b2, nx, ny) : 0;
                 int tot;
-                block_scan_excl(cnt, misc, &tot);
+                block_scan_excl1(cnt, misc + 16 + 8 * (scan_par++ & 1), &tot);
                 tile_images += tot;
             }
         }
@@ -1113,7 +1137,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 rxy[tid] = nx | (ny << 16);
             }
             int tot;
-            const int ex = block_scan_excl(cnt, misc, &tot);
+            const int ex = block_scan_excl1(cnt, misc + 16 + 8 * (scan_par++ & 1), &tot);
             if (PK && r0 == 0) {
                 if (nrows <= WGB) tile_images = tot;
                 // every image of the tile lies at d >= max(d_min, rlo): |amp| <= amax, and a
@@ -1553,7 +1577,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         __syncthreads();
         for (int t = tid; t < th - tl; t += WGB)
             orow[tl + t] = ((ybuf[t] + ybuf[BT + t]) + ybuf[2 * BT + t]) + ybuf[3 * BT + t];
-        __syncthreads();
+        // no barrier: the sums' region was last read by the products (before the barrier
+        // above), and ybuf (the row arrays) is next written after the next tile's row scan
         for (int e = tid; e < nwords; e += WGB) {
             if (LUT) D32[e] = 0;
             else D[e] = 0;
@@ -2976,7 +3001,7 @@ size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut
                         : pack ? std::max(4 * (size_t)nb * B.dl, (h16 == 1 || h16 == 2) ? 0 : f32rows)
                                : 8 * (size_t)nb * B.dl;
     return 8 * (size_t)ntab + 4 * (size_t)ntab + 16 * ((size_t)ntab + 4) + rows + coef + (8 + 4 + 4) * (size_t)WGB +
-           4 * (size_t)(lut ? QB_LUT : QB) + 4 * 16;
+           4 * (size_t)(lut ? QB_LUT : QB) + 4 * 32;
 }
 
 // Basis plan: one item per RIR and range of BT-sample tiles (tile_samples rounded up

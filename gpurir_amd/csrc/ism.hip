@@ -786,6 +786,7 @@ __host__ __device__ constexpr int pack_field_bits(int nb, int m) {
     return m < nb / 2 ? 64 - pack_lo_bits(nb, m) : pack_lo_bits(nb, nb - 1 - m);
 }
 constexpr long long PACK_MAX_IMAGES = 1LL << 21;
+constexpr long long LONG_TILE_MIN_TILES = 4096;  // launches with fewer 2 BT tiles keep BT
 // Lookup-table mode on the same machinery: the table's taps are linear in the weight of
 // the 1/16-sample grid, so an image at 16 tau = 16 n0 + p + a (phase p < 16, 0 <= a < 1)
 // adds amp (1 - a) to phase row p and amp a to row p + 1 (row 0 of column n0 + 1 for
@@ -934,7 +935,10 @@ __device__ inline unsigned short half_bits(float x) {
 // rounded) with FP32 accumulation: FP16 (1, v_mfma_f32_16x16x16_f16) or BF16 (2,
 // v_mfma_f32_16x16x16_bf16).
 // PK: the sums of two basis rows share a 64-bit word (pack_lo_bits), with a scale per tile.
-template <int NB, int DLC, int H16, bool LUT, bool PK>
+// BTK: samples per tile, BT or (FP16 pairs on packed sums at 16 kHz) 2 BT: half the tiles,
+// so half the per-tile enumeration, conversion set-up and barriers, and an image's window
+// overlaps a second tile half as often.
+template <int NB, int DLC, int H16, bool LUT, bool PK, int BTK = BT>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
     static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
     // SPL: FP32 filters from FP16 pairs (x = hi + lo 2^-11 for the coefficients and the rows,
@@ -943,6 +947,13 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     constexpr bool SPL = H16 == 3;
     static_assert(!SPL || (NB == 8 && !LUT), "FP16 pairs carry the 8-term FP32 expansion");
     static_assert(!PK || (!LUT && NB % 2 == 0), "packed sums pair the Chebyshev rows");
+    static_assert(BTK == BT || (BTK == 2 * BT && SPL && PK && DLC == 1152), "long tiles: FP16 pairs at 16 kHz");
+    constexpr int NOTK = BTK / 256;        // 256-sample output blocks of a tile
+    constexpr int UPW = 4 * NOTK / NWB;    // (basis quarter, block) products per wave
+    static_assert(UPW * NWB == 4 * NOTK, "whole products per wave");
+    // 16-bit coefficient copies: NCP copies, copy rho shifted by rho entries (four: the four
+    // operands of a lane start at a multiple of 4 halves, 8-byte reads; two: 4-byte reads)
+    constexpr int NCP = BTK == BT ? 4 : 2;
     constexpr int QBK = LUT ? QB_LUT : QB;
     constexpr int QEPT = QBK / WGB;   // consecutive queue entries per thread
     constexpr int QSEG = 64 * QEPT;   // entries per wave
@@ -966,16 +977,16 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     const int dreg = (LUT || PK) ? max(NB * DL, (H16 == 1 || H16 == 2) ? 0 : NB * DLP) : 2 * NB * DL;  // floats
     const int nwords = PK ? NB / 2 * DL : NB * DL;  // 64-bit (or LUT 32-bit) words of the sums
     float* ctab = reinterpret_cast<float*>(D) + dreg;                 // [NB][CL] (H16: see chs)
-    // H16: four copies of the coefficients in FP16, copy rho shifted by rho entries, so that a
-    // lane's four A operands start at a multiple of 4 halves: [4][NB][CL + 4]
+    // H16: NCP copies of the coefficients in 16 bits, [NCP][NB][CL + NCP] (SPL: hi, then lo)
     unsigned short* chs = reinterpret_cast<unsigned short*>(ctab);
-    const int crw = H16 ? (SPL ? 4 : 2) * NB * (CL + 4) : NB * CL;    // floats of the region (SPL: hi, lo)
+    const int crw = H16 ? (SPL ? 2 : 1) * NCP * NB * (CL + NCP) / 2 : NB * CL;  // floats of the region
     double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
     int* queue = rxy + WGB;                                           // [QBK] run marks of a round
     int* misc = queue + QBK;  // [32]: 8..15 att maxima, min offsets, tile max; 16..31 scan scratch
-    float* ybuf = reinterpret_cast<float*>(rb2);                      // [4][BT] (aliases the rows)
+    float* ybuf = reinterpret_cast<float*>(rb2);  // [4][BTK] (aliases the rows and, for 2 BT, the queue)
+    static_assert(4 * BTK * 4 <= WGB * (8 + 4 + 4) + QBK * 4, "ybuf fits the row arrays and the queue");
     float* Df = reinterpret_cast<float*>(D);                          // [NB][DLP] (aliases D)
     unsigned short* Dh = reinterpret_cast<unsigned short*>(D);        // H16: [NB][DLP] (aliases D)
 
@@ -1040,8 +1051,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     constexpr int BIG = 1 << 30;
     int scan_par = 0;  // block_scan_excl1 buffer parity (uniform)
 
-    for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
-        const int th = min(tl + BT, it.t_hi);
+    for (int tl = it.t_lo; tl < it.t_hi; tl += BTK) {
+        const int th = min(tl + BTK, it.t_hi);
         const int lo_n0 = tl - J1, hi_n0 = th - 1 - J0;
         // distance shell of those delays, one sample wider on each side (the exact
         // n0 test below decides)
@@ -1294,19 +1305,19 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
         // d = tid + s WGB (s < BCS) of every row into registers, and after the barrier
         // writes them to m DLP + d + 4 (d >> 4)
         float out_scale = 1.0f;  // H16: the power of two that undoes the tile's FP16 scale
-        if constexpr (PK && DLC == 640 && NB == 8) {
-            // 16 kHz packed sums: word e = tid + 512 k (k < 5) of the [4][640] words, so every
-            // thread converts five words; e / 640 is the same across a wave (512 and 640 are
-            // multiples of 64).  High fields: the word minus its sign-extended low field
+        if constexpr (PK && (DLC == 640 || DLC == 1152) && NB == 8) {
+            // 16 kHz packed sums: word e = tid + 512 k of the [4][DLC] words (DLC = 640 or
+            // 1152), so every thread converts DLC / 128 words; e / DLC is the same across a
+            // wave (512 and DLC are multiples of 64).  High fields: the word minus its sign-extended low field
             // (row 0's scale 2^-24 of it for every word); low fields: 2^(F_0 - S) of row 0's.
-            constexpr int F0 = pack_field_bits(NB, 0), KW = NB / 2 * 640 / WGB;
-            static_assert(KW * WGB == NB / 2 * 640, "whole words per thread");
+            constexpr int F0 = pack_field_bits(NB, 0), KW = NB / 2 * DLC / WGB;
+            static_assert(KW * WGB == NB / 2 * DLC && DLC % 64 == 0, "whole words per thread, rows per wave");
             float fh[KW], fl[KW];
             float mx = 0.0f;
 #pragma unroll
             for (int k = 0; k < KW; k++) {
                 const int e = tid + WGB * k;
-                const int w = __builtin_amdgcn_readfirstlane(e / 640);
+                const int w = __builtin_amdgcn_readfirstlane(e / DLC);
                 const int S = w == 0 ? pack_lo_bits(NB, 0) : w == 1 ? pack_lo_bits(NB, 1) : w == 2 ? pack_lo_bits(NB, 2) : pack_lo_bits(NB, 3);
                 const unsigned long long v = Dw[e];
                 const int lo = __builtin_amdgcn_sbfe((int)(unsigned)v, 0, S);
@@ -1330,8 +1341,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
 #pragma unroll
             for (int k = 0; k < KW; k++) {
                 const int e = tid + WGB * k;
-                const int w = __builtin_amdgcn_readfirstlane(e / 640);
-                const int d = e - 640 * w;
+                const int w = __builtin_amdgcn_readfirstlane(e / DLC);
+                const int d = e - DLC * w;
                 const int col = d + 4 * (d >> 4);
 #pragma unroll
                 for (int half = 0; half < 2; half++) {
@@ -1424,52 +1435,79 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             __syncthreads();
         }
 
-        // Toeplitz products: wave w takes samples [256 (w % NOT), +256) and basis rows
-        // (w / NOT) NB/4 .. +NB/4.  Lane (c, g) = (lane & 15, lane >> 4) supplies
-        // A[c][k] and X[k][c] for k = g KS + i, i < KS (one k per lane group and step)
+        // Toeplitz products.  BTK = BT: wave w takes samples [256 (w % NOT), +256) and basis
+        // rows (w / NOT) NB/4 .. +NB/4; BTK = 2 BT: basis quarter w % 4 and the 256-sample
+        // blocks w / 4 and w / 4 + 2 (UPW = 2 products, sharing the coefficient operands).
+        // Lane (c, g) = (lane & 15, lane >> 4) supplies A[c][k] and X[k][c] for k = g KS + i,
+        // i < KS (one k per lane group and step)
         {
-            const int ot = wave % NOT, mq = wave / NOT;
+            const int mq = BTK == BT ? wave / NOT : (wave & 3);
+            const int ot = BTK == BT ? wave % NOT : (wave >> 2);  // product u: block ot + 2 u
             const int c = lane & 15, g = lane >> 4;
             floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
             floatx4 accx = {0.0f, 0.0f, 0.0f, 0.0f};  // SPL: the cross terms, 2^-11 of acc's units
-            if constexpr (SPL && DLC == 640) {
+            floatx4 accu[UPW];
+            if constexpr (SPL && (DLC == 640 || DLC == 1152)) {
                 // FP16 pairs at 16 kHz (KS = 36, CL = 160): the wave's two rows are nine K
-                // steps each; the four operand reads of step q + 1 are issued before the three
-                // products of step q
-                constexpr int KSC = 36, CLC = 4 * KSC + 16, GQ = 1, NGR = KSC / 4 / GQ, NR = NB / 4;
-                const int rho = c & 3, m0 = mq * NR;
-                const unsigned short* Ap0 = chs + (rho * NB + m0) * (CLC + 4) + g * KSC + 16 - c + rho;
+                // steps each; the operand reads of step q + 1 are issued before the products
+                // of step q (three per block)
+                constexpr int KSC = 36, CLC = 4 * KSC + 16, NR = NB / 4, NSTEP = NR * KSC / 4;
+                constexpr int AHL = NCP * NB * (CLC + NCP);  // halves from the hi to the lo copies
+                const int rho = c & (NCP - 1), m0 = mq * NR;
+                const unsigned short* Ap0 = chs + (rho * NB + m0) * (CLC + NCP) + g * KSC + 16 - c + rho;
                 const unsigned short* Bp0 = Dh + m0 * DLP;
-                const int kb = 256 * ot + 16 * c + g * KSC;
-                half4_t xh[2][GQ], xl[2][GQ], ah[2][GQ], al[2][GQ];
-                auto load_group = [&](int q, int b) {
-                    const int r = q / NGR, t0 = (q % NGR) * GQ;
+                int kbu[UPW];
 #pragma unroll
-                    for (int t = 0; t < GQ; t++) {
-                        const int dl = kb + 4 * (t0 + t);
-                        const unsigned short* xp = Bp0 + r * DLP + dl + 4 * (dl >> 4);
-                        const unsigned short* ap = Ap0 + r * (CLC + 4) + 4 * (t0 + t);
-                        xh[b][t] = *reinterpret_cast<const half4_t*>(xp);
-                        xl[b][t] = *reinterpret_cast<const half4_t*>(xp + NB * DLP);
-                        ah[b][t] = *reinterpret_cast<const half4_t*>(ap);
-                        al[b][t] = *reinterpret_cast<const half4_t*>(ap + 4 * NB * (CLC + 4));
+                for (int u = 0; u < UPW; u++) kbu[u] = 256 * (ot + 2 * u) + 16 * c + g * KSC;
+                half4_t ah[2], al[2], xh[2][UPW], xl[2][UPW];
+                floatx4 ahh[UPW], axx[UPW];
+#pragma unroll
+                for (int u = 0; u < UPW; u++) {
+                    ahh[u] = acc;
+                    axx[u] = acc;
+                }
+                // four coefficients of a lane: 8-byte aligned with four copies, 4-byte with two
+                auto ld_coef = [&](const unsigned short* p) -> half4_t {
+                    if constexpr (NCP == 4) {
+                        return *reinterpret_cast<const half4_t*>(p);
+                    } else {
+                        const unsigned* q = reinterpret_cast<const unsigned*>(p);
+                        uint2 v;
+                        v.x = q[0];
+                        v.y = q[1];
+                        return __builtin_bit_cast(half4_t, v);
                     }
                 };
-                load_group(0, 0);
+                auto load_step = [&](int q, int b) {
+                    const int r = q / (KSC / 4), t4 = q % (KSC / 4);
+                    const unsigned short* ap = Ap0 + r * (CLC + NCP) + 4 * t4;
+                    ah[b] = ld_coef(ap);
+                    al[b] = ld_coef(ap + AHL);
 #pragma unroll
-                for (int q = 0; q < NR * NGR; q++) {
-                    if (q + 1 < NR * NGR) load_group(q + 1, (q + 1) & 1);
-                    __builtin_amdgcn_sched_barrier(0);  // the next group's reads stay ahead
+                    for (int u = 0; u < UPW; u++) {
+                        const int dl = kbu[u] + 4 * t4;
+                        const unsigned short* xp = Bp0 + r * DLP + dl + 4 * (dl >> 4);
+                        xh[b][u] = *reinterpret_cast<const half4_t*>(xp);
+                        xl[b][u] = *reinterpret_cast<const half4_t*>(xp + NB * DLP);
+                    }
+                };
+                load_step(0, 0);
 #pragma unroll
-                    for (int t = 0; t < GQ; t++) {
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1][t], xh[q & 1][t], acc, 0, 0, 0);
-                        accx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1][t], xl[q & 1][t], accx, 0, 0, 0);
-                        accx = __builtin_amdgcn_mfma_f32_16x16x16f16(al[q & 1][t], xh[q & 1][t], accx, 0, 0, 0);
+                for (int q = 0; q < NSTEP; q++) {
+                    if (q + 1 < NSTEP) load_step(q + 1, (q + 1) & 1);
+                    __builtin_amdgcn_sched_barrier(0);  // the next step's reads stay ahead
+#pragma unroll
+                    for (int u = 0; u < UPW; u++) {
+                        ahh[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1], xh[q & 1][u], ahh[u], 0, 0, 0);
+                        axx[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1], xl[q & 1][u], axx[u], 0, 0, 0);
+                        axx[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[q & 1], xh[q & 1][u], axx[u], 0, 0, 0);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                acc = (acc + accx * (1.0f / 2048.0f)) * out_scale;
-            } else if (H16) {
+#pragma unroll
+                for (int u = 0; u < UPW; u++) accu[u] = (ahh[u] + axx[u] * (1.0f / 2048.0f)) * out_scale;
+            } else {
+if (H16) {
                 // four k per lane and instruction: K steps of 4 per lane group, KS / 4 products
                 const int rho = c & 3;
 #pragma unroll
@@ -1570,13 +1608,24 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     a3 = n3;
                 }
             }
-            // result element (row 4 g + i, column c) is sample 16 c + 4 g + i of the half
-            *reinterpret_cast<float4*>(ybuf + mq * BT + 256 * ot + 16 * c + 4 * g) =
-                make_float4(acc[0], acc[1], acc[2], acc[3]);
+            accu[0] = acc;  // UPW = 1 on these paths
+            }
+            // result element (row 4 g + i, column c) of product u is sample 16 c + 4 g + i of
+            // block ot + 2 u
+#pragma unroll
+            for (int u = 0; u < UPW; u++)
+                *reinterpret_cast<float4*>(ybuf + mq * BTK + 256 * (ot + 2 * u) + 16 * c + 4 * g) =
+                    make_float4(accu[u][0], accu[u][1], accu[u][2], accu[u][3]);
         }
         __syncthreads();
         for (int t = tid; t < th - tl; t += WGB)
-            orow[tl + t] = ((ybuf[t] + ybuf[BT + t]) + ybuf[2 * BT + t]) + ybuf[3 * BT + t];
+            orow[tl + t] = ((ybuf[t] + ybuf[BTK + t]) + ybuf[2 * BTK + t]) + ybuf[3 * BTK + t];
+        if (BTK != BT) {
+            // ybuf covered the queue: its run marks start from zero again (the queue is next
+            // written after the next tile's row scan, whose barrier follows these reads)
+            __syncthreads();
+            for (int e = tid; e < QBK; e += WGB) queue[e] = 0;
+        }
         // no barrier: the sums' region was last read by the products (before the barrier
         // above), and ybuf (the row arrays) is next written after the next tile's row scan
         for (int e = tid; e < nwords; e += WGB) {
@@ -1590,15 +1639,15 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
 
 // The FP32 and table-mode kernels with the row length compiled in are held to 128 VGPRs
 // (two workgroups per CU); the same cap on the 16-bit-input kernels schedules them worse.
-template <int NB, int DLC, int H16, bool LUT, bool PK>
+template <int NB, int DLC, int H16, bool LUT, bool PK, int BTK = BT>
 __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3) ? 4 : 1)))
 void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    basis_item<NB, DLC, H16, LUT, PK>(a, blockIdx.x, smem_raw);
+    basis_item<NB, DLC, H16, LUT, PK, BTK>(a, blockIdx.x, smem_raw);
 }
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
-template <int NB, int DLC, int H16, bool LUT, bool PK>
+template <int NB, int DLC, int H16, bool LUT, bool PK, int BTK = BT>
 __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3) ? 4 : 1)))
 void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1609,7 +1658,7 @@ void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
         const int item = next;
         __syncthreads();
         if (item >= n_items) break;
-        basis_item<NB, DLC, H16, LUT, PK>(a, item, smem_raw);
+        basis_item<NB, DLC, H16, LUT, PK, BTK>(a, item, smem_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -2900,11 +2949,11 @@ double windowed_sinc(double x, int Tw) {
     return x == 0.0 ? w : w * std::sin(M_PI * x) / (M_PI * x);
 }
 
-BasisTables make_basis_tables(const TapTables& T, int Tw, int NB) {
+BasisTables make_basis_tables(const TapTables& T, int Tw, int NB, int bt = BT) {
     BasisTables B;
     B.ks = 4 * ((T.nJ + 15 + 15) / 16);
     B.cl = 4 * B.ks + 16;
-    B.dl = (int)align_up((size_t)std::max(BT + T.nJ - 1, BT - 16 + 4 * B.ks), 16);
+    B.dl = (int)align_up((size_t)std::max(bt + T.nJ - 1, bt - 16 + 4 * B.ks), 16);
     B.ctab.assign((size_t)NB * B.cl, 0.0f);
     const int J1 = T.J0 + T.nJ - 1;
     std::vector<double> fn(NB), val(NB);
@@ -2944,17 +2993,18 @@ BasisTables make_lut_basis_tables(const TapTables& T) {
 }
 
 // LDS image of the 16-bit coefficient copies of the basis kernels with 16-bit filter inputs:
-// copy rho < 4 of row m holds C[m][z - rho] at z < cl + 4 (zero outside [0, cl)), so that a
-// lane's four operands start at a multiple of four halves.  h16 = 1: FP16; 2: BF16 (rounded
-// to nearest even); 3: FP16 pairs of 2^14 C, the hi copies followed by the lo copies (the
-// rounding error of hi times 2^11).  The kernel copies the image into LDS unchanged.
-std::vector<unsigned short> make_half_copies(const BasisTables& B, int nb, int h16) {
-    const size_t per = 4 * (size_t)nb * (B.cl + 4);
+// copy rho < ncp of row m holds C[m][z - rho] at z < cl + ncp (zero outside [0, cl)), so that
+// a lane's four operands start at a multiple of ncp halves (ncp = 4: 8-byte reads; 2: 4-byte
+// reads).  h16 = 1: FP16; 2: BF16 (rounded to nearest even); 3: FP16 pairs of 2^14 C, the hi
+// copies followed by the lo copies (the rounding error of hi times 2^11).  The kernel copies
+// the image into LDS unchanged.
+std::vector<unsigned short> make_half_copies(const BasisTables& B, int nb, int h16, int ncp = 4) {
+    const size_t per = (size_t)ncp * nb * (B.cl + ncp);
     std::vector<unsigned short> img((h16 == 3 ? 2 : 1) * per, 0);
     for (size_t e = 0; e < per; e++) {
-        const int rho = (int)(e / ((size_t)nb * (B.cl + 4)));
-        const int rm = (int)(e - (size_t)rho * nb * (B.cl + 4));
-        const int m = rm / (B.cl + 4), z = rm - m * (B.cl + 4), src = z - rho;
+        const int rho = (int)(e / ((size_t)nb * (B.cl + ncp)));
+        const int rm = (int)(e - (size_t)rho * nb * (B.cl + ncp));
+        const int m = rm / (B.cl + ncp), z = rm - m * (B.cl + ncp), src = z - rho;
         const float cv = (src >= 0 && src < B.cl) ? B.ctab[(size_t)m * B.cl + src] : 0.0f;
         if (h16 == 3) {
             const float vs = cv * 16384.0f;
@@ -2992,9 +3042,9 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB,
     return c;
 }
 
-size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut, bool pack) {
-    // 16-bit coefficients in four shifted copies (FP16 pairs: hi and lo), or FP32
-    const size_t coef = h16 ? (h16 == 3 ? 16 : 8) * (size_t)nb * (B.cl + 4) : 4 * (size_t)nb * B.cl;
+size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut, bool pack, int ncp = 4) {
+    // 16-bit coefficients in ncp shifted copies (FP16 pairs: hi and lo), or FP32
+    const size_t coef = h16 ? (h16 == 3 ? 4 : 2) * (size_t)ncp * nb * (B.cl + ncp) : 4 * (size_t)nb * B.cl;
     // the region holds the sums and, after the conversion, the FP32 (or 16-bit) rows
     const size_t f32rows = 4 * (size_t)nb * (B.dl + B.dl / 4);
     const size_t rows = lut    ? f32rows
@@ -3009,16 +3059,16 @@ size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut
 // launch still spreads its tiles over the device), longest expected work first: the
 // images in a shell grow with the cube of its outer radius and with 1 / volume.
 Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, const TapTables& Tt,
-                     bool sort_items = true) {
+                     bool sort_items = true, int bt = BT) {
     Plan P;
     int ntab = 0;
     long long tiles_all = 0;
     for (const RoomHost& h : rh) {
         ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
-        tiles_all += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + BT - 1) / BT);
+        tiles_all += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + bt - 1) / bt);
     }
     P.ntab_max = ntab;
-    const long long per_default = (long long)BT * std::max<long long>(1, tiles_all / TARGET_ITEMS);
+    const long long per_default = (long long)bt * std::max<long long>(1, tiles_all / TARGET_ITEMS);
     std::vector<std::pair<double, Item>> work;
     long long rir_off = 0;
     for (size_t k = 0; k < rh.size(); k++) {
@@ -3046,9 +3096,9 @@ Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, con
         R.n_chunks = 1;
         const int N = h.nb[0] * h.nb[1] * h.nb[2];
         const int M = h.n_src * h.n_rcv;
-        int per = opt.tile_samples > 0 ? (int)align_up((size_t)opt.tile_samples, BT)
+        int per = opt.tile_samples > 0 ? (int)align_up((size_t)opt.tile_samples, bt)
                                        : (int)std::min<long long>(per_default, 1 << 30);
-        per = std::max(per, BT);
+        per = std::max(per, bt);
         const double vol = (double)h.L[0] * h.L[1] * h.L[2];
         for (int m = 0; m < M; m++) {
             P.rir_room.push_back((int)k);
@@ -3063,7 +3113,7 @@ Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, con
                 it.chunk = 0;
                 it.pad = 0;
                 const double r1 = (double)(it.t_hi + Tt.nJ), r0 = (double)std::max(it.t_lo - Tt.nJ, 0);
-                const double tiles = (double)((it.t_hi - it.t_lo + BT - 1) / BT);
+                const double tiles = (double)((it.t_hi - it.t_lo + bt - 1) / bt);
                 work.emplace_back((r1 * r1 * r1 - r0 * r0 * r0) / vol + 64.0 * tiles, it);
             }
         }
@@ -3076,7 +3126,7 @@ Plan make_plan_basis(const std::vector<RoomHost>& rh, const SimOptions& opt, con
         });
     P.items.reserve(work.size());
     for (const auto& w : work) P.items.push_back(w.second);
-    P.tt_max = BT;
+    P.tt_max = bt;
     return P;
 }
 
@@ -3094,6 +3144,7 @@ struct SimLaunch {
     bool lut = false;    // table mode on the basis machinery (16 phase rows)
     bool pack = false;   // two basis rows per 64-bit sum (RIRs of fewer than PACK_MAX_IMAGES images)
     std::vector<unsigned short> chs;  // h16: LDS image of the 16-bit coefficient copies (make_half_copies)
+    int bt = BT;         // samples per tile of the basis kernel (2 BT: FP16 pairs at 16 kHz, large launches)
     int nb = 8;          // Chebyshev terms of the basis method
     int prec = PREC_FP32;
     int Tw = 0;
@@ -3151,24 +3202,42 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
             tiles += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + BT - 1) / BT);
         if (tiles < AUTO_BASIS_MIN_TILES) S.basis = false;
     }
+    S.bt = BT;
     if (S.basis) {
-        // the tables depend on Tw only (about 10^4 transcendental calls): built once
+        // the tables depend on Tw (and the tile length) only (about 10^4 transcendental
+        // calls): built once
         static std::mutex mu;
-        static std::map<std::pair<int, int>, BasisTables> cache;
-        {
+        static std::map<std::tuple<int, int, int>, BasisTables> cache;
+        auto tables = [&](int bt) {
             std::lock_guard<std::mutex> lk(mu);
-            const auto key = std::make_pair(S.Tw, S.nb);
+            const auto key = std::make_tuple(S.Tw, S.nb, bt);
             auto f = cache.find(key);
             if (f == cache.end())
-                f = cache.emplace(key, S.lut ? make_lut_basis_tables(S.T) : make_basis_tables(S.T, S.Tw, S.nb)).first;
-            S.Bt = f->second;
-        }
+                f = cache.emplace(key, S.lut ? make_lut_basis_tables(S.T) : make_basis_tables(S.T, S.Tw, S.nb, bt)).first;
+            return f->second;
+        };
+        S.Bt = tables(BT);
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         long long most_images = 0;
         for (const RoomHost& h : rooms) most_images = std::max(most_images, (long long)h.nb[0] * h.nb[1] * h.nb[2]);
         S.pack = !S.lut && (S.h16 == 0 || S.h16 == 3) && most_images < PACK_MAX_IMAGES;
-        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut, S.pack) > 160 * 1024 || S.Bt.dl > BCS * WGB) {
+        if (S.pack && S.h16 == 3 && S.Bt.dl == 640) {
+            // FP16 pairs at 16 kHz: tiles of 2 BT samples when the launch has enough of them
+            // to fill the device and two workgroups still fit a CU's LDS
+            long long tiles2 = 0;
+            for (const RoomHost& h : rooms)
+                tiles2 += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + 2 * BT - 1) / (2 * BT));
+            const BasisTables B2 = tables(2 * BT);
+            if (B2.dl == 1152 && tiles2 >= LONG_TILE_MIN_TILES &&
+                2 * basis_lds_bytes((ntab + 3) / 4 * 4, B2, 3, S.nb, false, true, 2) <= 160 * 1024) {
+                S.bt = 2 * BT;
+                S.Bt = B2;
+            }
+        }
+        // (2 BT tiles convert with their own DLC / 128 words per thread)
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut, S.pack, S.bt == BT ? 4 : 2) > 160 * 1024 ||
+            (S.bt == BT && S.Bt.dl > BCS * WGB)) {
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
             // scatter takes the launch
@@ -3178,19 +3247,21 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         }
     }
     if (S.basis) {
-        S.P = make_plan_basis(rooms, opt, S.T);
+        S.P = make_plan_basis(rooms, opt, S.T, true, S.bt);
         if (S.h16) {
             static std::mutex mu;
-            static std::map<std::tuple<int, int, int>, std::vector<unsigned short>> cache;
+            static std::map<std::tuple<int, int, int, int>, std::vector<unsigned short>> cache;
             std::lock_guard<std::mutex> lk(mu);
-            const auto key = std::make_tuple(S.Tw, S.nb, S.h16);
+            const int ncp = S.bt == BT ? 4 : 2;
+            const auto key = std::make_tuple(S.Tw, S.nb, S.h16, ncp);
             auto f = cache.find(key);
-            if (f == cache.end()) f = cache.emplace(key, make_half_copies(S.Bt, S.nb, S.h16)).first;
+            if (f == cache.end()) f = cache.emplace(key, make_half_copies(S.Bt, S.nb, S.h16, ncp)).first;
             S.chs = f->second;
         }
     } else {
         S.Bt = BasisTables();
         S.chs.clear();
+        S.bt = BT;
         S.P = make_plan(rooms, opt, S.T, true);
     }
     const Plan& P = S.P;
@@ -3222,7 +3293,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
             (size_t)(QCAP + QPAD) * 5 * 4 + 32 + (T.n_pass <= 12 ? sizeof(float2) * T.lut.size() : 0);
-    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut, S.pack);
+    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut, S.pack, S.bt == BT ? 4 : 2);
     GPURIR_CHECK(T.n_pass <= 48, "sampling frequency too high: at most 3072 window taps (fs 384 kHz) are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -3327,12 +3398,14 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H, U, P>) : reinterpret_cast<const void*>(&K<N, 0, H, U, P>))
 #define GPURIR_BASIS_FN(K, N, H) \
     (S.pack ? GPURIR_BASIS_FN1(K, N, H, false, true) : GPURIR_BASIS_FN1(K, N, H, false, false))
-        const void* fn_once = S.lut        ? GPURIR_BASIS_FN1(ism_basis_kernel, LUT_NB, 0, true, false)
+        const void* fn_once = S.bt != BT   ? reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>)
+                              : S.lut      ? GPURIR_BASIS_FN1(ism_basis_kernel, LUT_NB, 0, true, false)
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1)
                               : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 3)
                                            : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0);
-        const void* fn_pers = S.lut        ? GPURIR_BASIS_FN1(ism_basis_persistent_kernel, LUT_NB, 0, true, false)
+        const void* fn_pers = S.bt != BT   ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>)
+                              : S.lut      ? GPURIR_BASIS_FN1(ism_basis_persistent_kernel, LUT_NB, 0, true, false)
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1)
                               : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 3)

@@ -18,12 +18,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (packed); non-packed issue caps at half of it
-FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix
 FLOP_PER_TAP = 12               # sub, x2pi, /Tw, cos, +1, x0.5, xpi, sin, /, xamp, xsinc, +acc
 FLOP_PER_TAP_BWD = 24           # value and derivative of the windowed sinc, both accumulated
-BASIS_TILE = 512                # samples per tile of the basis kernel
-BASIS_MFMA_PER_TILE = 576       # v_mfma_f32_16x16x4_f32 per tile at Tw = 128: 8 rows x 36 K steps x 2 blocks
-MFMA_16X16X4_FLOP = 2048
+# the FP32 basis kernel on a large 16 kHz batch: 1024-sample tiles, filters from FP16 pairs
+BASIS_TILE = 1024
+BASIS_MFMA_PER_TILE = 864       # v_mfma_f32_16x16x16_f16 per tile: 3 per 16 K values x 9 K steps x 2 rows x 2 blocks x 8 waves
+MFMA_16X16X16_FLOP = 8192
+FP16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 matrix (v_mfma_f32_16x16x16_f16 issues at half of it)
 
 
 def log(msg):
@@ -461,10 +462,10 @@ def main():
 
     traffic = read_traffic("traffic_ism_basis_uneven_rooms.txt")
     counters = read_counters("counters_ism_basis_uneven_rooms.txt")
-    # work the basis kernel executes: its matrix-core filters per 512-sample tile
+    # work the basis kernel executes: its matrix-core filters per 1024-sample tile
     tiles = sum(len(r["pos_src"]) * len(r["pos_rcv"]) * -(-T.n_samples(r["Tdiff"], fs) // BASIS_TILE)
                 for r in rooms)
-    mfma_tflops = tiles * BASIS_MFMA_PER_TILE * MFMA_16X16X4_FLOP / (ms_acc / 1e3) / 1e12
+    mfma_tflops = tiles * BASIS_MFMA_PER_TILE * MFMA_16X16X16_FLOP / (ms_acc / 1e3) / 1e12
     line = {
         "metric": "RIRs/s batched vs plain at <=1e-3 rel-L2 vs FP64; fwd+bwd grads/s",
         "value": rirs_per_s,
@@ -486,9 +487,12 @@ def main():
                      "kernel": "ism_basis_persistent_kernel" if (B["autotune"] or {}).get("persistent", True)
                      else "ism_basis_kernel", "kernel_ms": ms_acc,
                      "work": f"{taps} in-window taps x {FLOP_PER_TAP} flop (the tap formula's work; the basis "
-                             "method executes 8 fixed-point sums per image and matrix-core filters instead)",
-                     "executed": {"tiles": tiles, "mfma_tflops": mfma_tflops,
-                                  "mfma_frac": mfma_tflops / FP32_MFMA_PEAK_TFLOPS,
+                             "method executes 8 fixed-point sums per image (4 packed 64-bit LDS adds) and "
+                             "matrix-core filters instead)",
+                     "executed": {"tiles": tiles, "tile_samples": BASIS_TILE,
+                                  "mfma": "v_mfma_f32_16x16x16_f16, FP16 pairs (3 per 16 K values)",
+                                  "mfma_per_tile": BASIS_MFMA_PER_TILE, "mfma_tflops": mfma_tflops,
+                                  "mfma_frac_of_fp16_dense_peak": mfma_tflops / FP16_MFMA_PEAK_TFLOPS,
                                   "counters": counters}},
         "cpu_baseline": cpu,
         "accuracy": {"batch_rel_l2_max_3_rooms": max(errs), "plain_rel_l2": err_plain, "mid_room_rel_l2": err_mid,

@@ -134,6 +134,16 @@ __device__ inline double sqrt_normal(double x) {
     return fma(d, h, g);
 }
 
+// 1 / x for a positive normal double: the hardware estimate and two Newton steps (error
+// below 2 ulp)
+__device__ inline double rcp_normal(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 __device__ inline double ipow(double b, int e) {
     double r = 1.0;
     while (e > 0) {
@@ -1983,10 +1993,10 @@ struct AdjArgs {
     int J0, nJ;
     float halfTw, inv_tw2, pi_tw;  // Tw/2, 2/Tw, pi/Tw
     int ntab_max, gp_len;
-    // basis adjoint: coefficients [ADJ_NB][acl] (16 zeros, then offset k = j - J0 at 16 + k),
-    // row length of E, logical length of g' (stored with 4 floats of padding per 16), K steps
-    // per lane group of the correlation products
-    const float* actab;
+    // basis adjoint: LDS image of the coefficients as FP16 pairs (make_adjoint_half_copies),
+    // row length of E, logical length of g' (stored with 4 halves of padding per 16), K steps
+    // per lane group of the correlation products, coefficient row length
+    const unsigned short* achs;
     int adl, agl, aks, acl;
     RoomParams prm;
 };
@@ -2078,16 +2088,16 @@ __device__ __forceinline__ void adj_tap_near(float4 kc, float g, float f, float 
 __device__ __forceinline__ void adj_chain(const RoomK& R, int nx, int ny, int nz, const double* offs, const float* F,
                                           const float* Flo, const float* Fhi, double orx, double ory, double orz,
                                           double osx, double osy, double osz, int spkr, int mic, float G0, float G1,
-                                          double acc[12]) {
+                                          double acc[12], double d) {
     const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
     const int ey = Nx + ny, ez = Nx + Ny + nz;
     const double vx = offs[nx], vy = offs[ey], vz = offs[ez];
-    const double d = sqrt(vx * vx + vy * vy + vz * vz);
-    const double id = 1.0 / d;
+    const double id = rcp_normal(d);
     const double ux = vx * id, uy = vy * id, uz = vz * id;
     const double Fx = F[nx], Fy = F[ey], Fz = F[ez];
-    const double att = Fx * Fy * Fz;
-    const double a0 = id / 12.566370614359172;
+    const double Fxy = Fx * Fy;
+    const double att = Fxy * Fz;
+    const double a0 = id * 0.079577471545947668;  // 1 / (4 pi)
     const double sx = ((nx - Nx / 2) & 1) ? -1.0 : 1.0;
     const double sy = ((ny - Ny / 2) & 1) ? -1.0 : 1.0;
     const double sz = ((nz - Nz / 2) & 1) ? -1.0 : 1.0;
@@ -2105,12 +2115,13 @@ __device__ __forceinline__ void adj_chain(const RoomK& R, int nx, int ny, int nz
     const double amp = att * geo;
     const double g0 = (double)G0, g1 = (double)G1;
     const double gbeta = g0 * geo;
-    acc[0] += gbeta * (double)Flo[nx] * Fy * Fz;
-    acc[1] += gbeta * (double)Fhi[nx] * Fy * Fz;
-    acc[2] += gbeta * Fx * (double)Flo[ey] * Fz;
-    acc[3] += gbeta * Fx * (double)Fhi[ey] * Fz;
-    acc[4] += gbeta * Fx * Fy * (double)Flo[ez];
-    acc[5] += gbeta * Fx * Fy * (double)Fhi[ez];
+    const double gyz = gbeta * (Fy * Fz), gxz = gbeta * (Fx * Fz), gxy = gbeta * Fxy;
+    acc[0] += gyz * (double)Flo[nx];
+    acc[1] += gyz * (double)Fhi[nx];
+    acc[2] += gxz * (double)Flo[ey];
+    acc[3] += gxz * (double)Fhi[ey];
+    acc[4] += gxy * (double)Flo[ez];
+    acc[5] += gxy * (double)Fhi[ez];
     // d amp / d v = att a0 (-Ds Dr u/d + Ds' Dr dcs/dv + Ds Dr' dcr/dv),
     // dcr/dv = (o_r - cr u)/d, dcs/dv = (-o_s' - cs u)/d;  d tau / d v = (fs/c) u
     const double kr = att * a0 * Ds * Drp * id, ks = att * a0 * Dsp * Dr * id;
@@ -2234,7 +2245,9 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
         const int rem = i - nx * (Ny * Nz);
         const int ny = (int)(((double)rem + 0.5) * R.inv_nz);
         const int nz = rem - ny * Nz;
-        adj_chain(R, nx, ny, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, a.spkr, a.mic, G0, G1, acc);
+        const double dx = offs[nx], dy = offs[Nx + ny], dz = offs[Nx + Ny + nz];
+        adj_chain(R, nx, ny, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, a.spkr, a.mic, G0, G1, acc,
+                  sqrt_normal(dx * dx + dy * dy + dz * dz));
     };
 
     // image rounds of WG: phase A lists the images whose taps reach the tile into a
@@ -2316,12 +2329,16 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
 //   G1 = sum_t g'(t) h'(t - tau) = -2 sum_m m U_{m-1}(u) E_m[n0]
 // (h'(x) = -d h_j / df at x = j - f, du/df = 2, T_m' = m U_{m-1}).  Per 512-sample tile the
 // workgroup builds g' on the tile (zero outside it, so that every tile adds only its own
-// taps), correlates it with the coefficient rows (E, FP32 FMAs with the coefficients as
-// scalar operands), enumerates the images of the tile's delay range as the forward basis
+// taps), correlates it with the coefficient rows on the matrix cores (E; both operands as
+// FP16 pairs x = hi + lo 2^-11, three products hi hi, hi lo, lo hi per K step, 22 bits of
+// each operand), enumerates the images of the tile's delay range as the forward basis
 // kernel does, and sends each live image's G0, G1 through adj_chain into per-lane FP64 sums,
 // reduced per item in a fixed order.
 constexpr int ADJ_NB = 10;
 
+// PAT: a source or receiver directivity other than omni (without it, the orientations and
+// the pattern terms of the chain rule are compiled out).
+template <bool PAT>
 __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void ism_adjoint_basis_kernel(AdjArgs a) {
     constexpr int NB = ADJ_NB;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -2340,14 +2357,22 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     float* Fhi = Flo + ntab;
     int* xrun = reinterpret_cast<int*>(Fhi + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
     int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4]
-    float* gp = reinterpret_cast<float*>(rowStart + ntab + 4);  // [GL + GL/4] g'(lo_n0 + J0 + u) at u + 4 (u >> 4)
-    float* cpad = gp + GL + GL / 4;                      // [NB][CL] coefficients
-    float* E = cpad + NB * CL;                           // [NB][DL] E_m[lo_n0 + d]
+    const int GP = GL + GL / 4;
+    // [2][GP] g'(lo_n0 + J0 + u) 2^sg as FP16 pairs (hi, then lo) at u + 4 (u >> 4)
+    unsigned short* gph = reinterpret_cast<unsigned short*>(rowStart + ntab + 4);
+    constexpr int NCP = 2;                               // shifted coefficient copies
+    const int CLP = CL + NCP;
+    unsigned short* chs = gph + 2 * GP;                  // [2][NCP][NB][CLP] coefficient FP16 pairs
+    float* E = reinterpret_cast<float*>(chs + 2 * NCP * NB * CLP);  // [NB][DL] E_m[lo_n0 + d]
     double* rb2 = reinterpret_cast<double*>(E + NB * DL);  // [WGB] row x^2 + y^2
     int* rxy = reinterpret_cast<int*>(rb2 + WGB);        // [WGB] row nx | ny << 16
-    int* queue = rxy + WGB;                              // [QB] slot << 12 | nz
-    int* misc = queue + QB;                              // [16] scan scratch
-    for (int e = tid; e < NB * CL; e += WGB) cpad[e] = a.actab[e];
+    int* queue = rxy + WGB;                              // [QB] run marks
+    int* misc = queue + QB;                              // [32] scan scratch (two buffers at 16, 24)
+    constexpr int QEPT = QB / WGB;                       // consecutive queue entries per thread
+    constexpr int QSEG = 64 * QEPT;                      // entries per wave
+    for (int e = tid; e < NCP * NB * CLP / 2; e += WGB)  // 2 NCP NB CLP halves in 8-byte words
+        reinterpret_cast<uint2*>(chs)[e] = reinterpret_cast<const uint2*>(a.achs)[e];
+    for (int e = tid; e < QB; e += WGB) queue[e] = 0;    // run marks: zero between rounds
     const int s = it.rir / R.n_rcv, r = it.rir - s * R.n_rcv;
     const int g_rir = R.rir_off + it.rir;
     const float* ps = a.pos_src + 3 * (R.src_off + s);
@@ -2361,14 +2386,15 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
         axis_entry_adj(n, R.Nz, R.L2, ps[2], pr[2], R.b4, R.b5, offs + o, F + o, Flo + o, Fhi + o);
     }
     double orx = 0, ory = 0, orz = 0, osx = 0, osy = 0, osz = 0;
-    if (a.mic != 0) {
+    const int spkr = PAT ? a.spkr : 0, mic = PAT ? a.mic : 0;
+    if (mic != 0) {
         const float* o = a.orv_rcv + 3 * (R.rcv_off + r);
         const double n = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
         orx = o[0] / n;
         ory = o[1] / n;
         orz = o[2] / n;
     }
-    if (a.spkr != 0) {
+    if (spkr != 0) {
         const float* o = a.orv_src + 3 * (R.src_off + s);
         const double n = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
         osx = o[0] / n;
@@ -2383,6 +2409,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     const float cw = a.rir_cw[g_rir];
     const int ws = a.rir_ws[g_rir];
     constexpr int BIG = 1 << 30;
+    int scan_par = 0;  // block_scan_excl1 buffer parity (uniform)
     double acc[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) acc[k] = 0.0;
@@ -2391,54 +2418,96 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
         const int th = min(tl + BT, it.t_hi);
         const int lo_n0 = tl - J1, hi_n0 = th - 1 - J0;
-        // g' on the tile, zero outside it: dL/dout plus the envelope-window term
-        for (int u = tid; u < GL; u += WGB) {
+        // g' on the tile, zero outside it: dL/dout plus the envelope-window term (GL <= 4 WGB)
+        float gv[4];
+        float gm = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int u = tid + i * WGB;
             const int t = lo_n0 + J0 + u;
             float g = 0.0f;
-            if (t >= tl && t < th) {
+            if (u < GL && t >= tl && t < th) {
                 g = a.gy[row_off + t];
                 if (t >= ws) g = fmaf(cw, a.y[row_off + t], g);
             }
-            gp[u + 4 * (u >> 4)] = g;
+            gv[i] = g;
+            gm = fmaxf(gm, fabsf(g));
+        }
+        // the tile's largest |g'| (non-negative floats order as their bit patterns) scales
+        // g' into [2^14, 2^15) for the FP16 pairs
+        {
+            const int wm = wave_max_nonneg(__float_as_int(gm));
+            if (lane == 63) misc[wave] = wm;
+        }
+        __syncthreads();
+        int gmb = 0;
+#pragma unroll
+        for (int w = 0; w < NWB; w++) gmb = max(gmb, misc[w]);
+        const float gmax = __int_as_float(gmb);
+        const int sg = gmax > 0.0f ? min(max(14 - ilogbf(gmax), -100), 100) : 0;
+        {
+            const float gs = ldexpf(1.0f, sg);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int u = tid + i * WGB;
+                if (u < GL) {
+                    const float x = gv[i] * gs;
+                    const _Float16 hi = (_Float16)x;
+                    gph[u + 4 * (u >> 4)] = __builtin_bit_cast(unsigned short, hi);
+                    gph[GP + u + 4 * (u >> 4)] = __builtin_bit_cast(unsigned short, (_Float16)((x - (float)hi) * 2048.0f));
+                }
+            }
         }
         __syncthreads();
         // E_m[lo_n0 + d] = sum_k C[m][k] g'(lo_n0 + d + J0 + k): E[16 b + r] = sum_x A[r][x] X[x][b]
         // with A[r][x] = C[x - r] (Toeplitz) and X[x][b] = g'[16 b + x] (Hankel), the forward
         // basis kernel's products with the coefficients unreversed; one task per (row m,
-        // 256 positions of E), the waves taking tasks in turn
+        // 256 positions of E), the waves taking tasks in turn.  Lane (c, g) holds K indices
+        // g KS + 4 s + e (e < 4) of step s: coefficients C[g KS + 4 s + e - c] from copy
+        // c & 1 (4-byte aligned reads), g' at 16 c + g KS + 4 s + e.
         {
-            const int c = lane & 15, g = lane >> 4;
-            const int nota = (DL + 255) / 256;
+            const int c = lane & 15, g = lane >> 4, rho = c & (NCP - 1);
+            const int nota = (DL + 255) / 256, nst = KS / 4;
+            const float oscale = ldexpf(1.0f, -(14 + sg));  // coefficients carry 2^14, g' 2^sg
+            auto ld_coef = [&](const unsigned short* p) -> half4_t {
+                const unsigned* q = reinterpret_cast<const unsigned*>(p);
+                uint2 v;
+                v.x = q[0];
+                v.y = q[1];
+                return __builtin_bit_cast(half4_t, v);
+            };
             for (int task = wave; task < NB * nota; task += NWB) {
                 const int m = task / nota, ot = task - m * nota;
-                floatx4 acc4 = {0.0f, 0.0f, 0.0f, 0.0f};
-                const float* Ap = cpad + m * CL + g * KS + 16 - c;
+                const unsigned short* Ap = chs + (rho * NB + m) * CLP + g * KS + 16 - c + rho;
                 const int kb = 256 * ot + 16 * c + g * KS;
-                // operands of step t4 + 1 are read before the products of step t4; the read
-                // after the last step stays inside g' (its logical length covers it) and the
-                // arrays after the coefficients, and is not used
-                float4 xv = *reinterpret_cast<const float4*>(gp + kb + 4 * (kb >> 4));
-                float a0 = Ap[0], a1 = Ap[1], a2 = Ap[2], a3 = Ap[3];
-                for (int t4 = 0; t4 < KS / 4; t4++) {
-                    const int dn = kb + 4 * (t4 + 1);
-                    const float4 xn = *reinterpret_cast<const float4*>(gp + dn + 4 * (dn >> 4));
-                    const float* an = Ap + 4 * (t4 + 1);
-                    const float n0 = an[0], n1 = an[1], n2 = an[2], n3 = an[3];
-                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, xv.x, acc4, 0, 0, 0);
-                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, xv.y, acc4, 0, 0, 0);
-                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, xv.z, acc4, 0, 0, 0);
-                    acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, xv.w, acc4, 0, 0, 0);
-                    xv = xn;
-                    a0 = n0;
-                    a1 = n1;
-                    a2 = n2;
-                    a3 = n3;
+                floatx4 ahh = {0.0f, 0.0f, 0.0f, 0.0f}, axx = {0.0f, 0.0f, 0.0f, 0.0f};
+                half4_t ah = ld_coef(Ap), al = ld_coef(Ap + NCP * NB * CLP);
+                half4_t xh = *reinterpret_cast<const half4_t*>(gph + kb + 4 * (kb >> 4));
+                half4_t xl = *reinterpret_cast<const half4_t*>(gph + GP + kb + 4 * (kb >> 4));
+                for (int st = 0; st < nst; st++) {
+                    // operands of step st + 1 are read before the products of step st
+                    half4_t nah = ah, nal = al, nxh = xh, nxl = xl;
+                    if (st + 1 < nst) {
+                        const unsigned short* an = Ap + 4 * (st + 1);
+                        nah = ld_coef(an);
+                        nal = ld_coef(an + NCP * NB * CLP);
+                        const int dn = kb + 4 * (st + 1);
+                        nxh = *reinterpret_cast<const half4_t*>(gph + dn + 4 * (dn >> 4));
+                        nxl = *reinterpret_cast<const half4_t*>(gph + GP + dn + 4 * (dn >> 4));
+                    }
+                    ahh = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xh, ahh, 0, 0, 0);
+                    axx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xl, axx, 0, 0, 0);
+                    axx = __builtin_amdgcn_mfma_f32_16x16x16f16(al, xh, axx, 0, 0, 0);
+                    ah = nah;
+                    al = nal;
+                    xh = nxh;
+                    xl = nxl;
                 }
                 // element (row 4 g + i, column c) is position 256 ot + 16 c + 4 g + i
                 const int d0 = 256 * ot + 16 * c + 4 * g;
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    if (d0 + i < DL) E[m * DL + d0 + i] = acc4[i];
+                    if (d0 + i < DL) E[m * DL + d0 + i] = (ahh[i] + axx[i] * (1.0f / 2048.0f)) * oscale;
             }
         }
         // rows: per x entry, the y entries with x^2 + y^2 <= d2hi (one run per parity)
@@ -2453,7 +2522,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 const double rem = d2hi - X * X;
                 int f0 = 0, c0 = 0, f1 = 0, c1 = 0;
                 if (rem >= 0.0) {
-                    const double Y = sqrt(rem);
+                    const double Y = sqrt_bound(rem);
                     axis_range(AY, 0, -Y, Y, f0, c0, BIG);
                     axis_range(AY, 1, -Y, Y, f1, c1, BIG);
                 }
@@ -2463,7 +2532,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 c = c0 + c1;
             }
             int tot;
-            const int ex = block_scan_excl(c, misc, &tot);
+            const int ex = block_scan_excl1(c, misc + 16 + 8 * (scan_par++ & 1), &tot);
             if (nx < Nx) rowStart[nx + 1] = carry + ex + c;
             carry += tot;
         }
@@ -2489,12 +2558,12 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 b2 = b2 + vy * vy;
                 const double V2 = d2hi - b2, U2 = d2lo - b2;
                 if (V2 >= 0.0) {
-                    const double V = sqrt(V2);
+                    const double V = sqrt_bound(V2);
                     if (U2 <= 0.0) {
                         axis_range(AZ, 0, -V, V, zf[0], zc[0], BIG);
                         axis_range(AZ, 1, -V, V, zf[2], zc[2], BIG);
                     } else {
-                        const double U = sqrt(U2);
+                        const double U = sqrt_bound(U2);
                         axis_range(AZ, 0, U, V, zf[1], zc[1], BIG);
                         axis_range(AZ, 0, -V, -U, zf[0], zc[0], (zf[1] - AZ.half) / 2 - 1);
                         axis_range(AZ, 1, U, V, zf[3], zc[3], BIG);
@@ -2506,49 +2575,81 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 rxy[tid] = nx | (ny << 16);
             }
             int tot;
-            const int ex = block_scan_excl(cnt, misc, &tot);
+            const int ex = block_scan_excl1(cnt, misc + 16 + 8 * (scan_par++ & 1), &tot);
             for (int q0 = 0; q0 < tot; q0 += QB) {
-                int e = ex;
+                // run marks as in the forward basis kernel: entry p of the round (run g of row
+                // slot, run start s) has nz = base + 2 p, base = zf[g] - 2 (s - q0); each run
+                // writes its key (slot, g, base + 4096) at its first entry in the round and at
+                // every wave segment start it covers, and an entry's key is the largest one at
+                // or before it
+                {
+                    int s = ex;
 #pragma unroll
-                for (int g = 0; g < 4; g++) {
-                    const int e0 = max(e, q0), e1 = min(e + zc[g], q0 + QB);
-                    for (int p = e0; p < e1; p++) queue[p - q0] = (tid << 12) | (zf[g] + 2 * (p - e));
-                    e += zc[g];
+                    for (int g = 0; g < 4; g++) {
+                        const int a0 = max(s, q0) - q0, b0 = min(s + zc[g], q0 + QB) - q0;
+                        if (a0 < b0) {
+                            const int key = ((tid * 4 + g) << 13) | (zf[g] - 2 * (s - q0) + 4096);
+                            queue[a0] = key;
+                            for (int bnd = (a0 / QSEG + 1) * QSEG; bnd < b0; bnd += QSEG) queue[bnd] = key;
+                        }
+                        s += zc[g];
+                    }
                 }
                 __syncthreads();
                 const int nq = min(QB, tot - q0);
-                for (int k = tid; k < nq; k += WGB) {
-                    const int qe = queue[k];
-                    const int slot = qe >> 12, nz = qe & 4095;
-                    const double vz = offs[Nx + Ny + nz];
-                    double d2 = rb2[slot];
-                    d2 = d2 + vz * vz;
-                    const double tau = sqrt_normal(d2) * fsc;
-                    const double fl = floor(tau);
-                    const int n0 = (int)fl;
-                    if (n0 < lo_n0 || n0 > hi_n0) continue;
-                    const float u = fmaf(2.0f, (float)(tau - fl), -1.0f);
-                    const float u2 = u + u;
-                    const float* ep = E + (n0 - lo_n0);
-                    // T_m (first kind) and m U_{m-1} (derivative) by their recurrences
-                    float t0 = 1.0f, t1 = u, w0 = 1.0f, w1 = u2;  // T_0, T_1, U_0, U_1
-                    float G0 = fmaf(t1, ep[DL], ep[0]);
-                    float D1 = ep[DL];  // 1 * U_0 * E_1
+                if (QSEG * wave < nq) {
+                    static_assert(QEPT == 4, "entry keys are selected from four registers");
+                    int keys[QEPT];
+                    int kmax = 0;
 #pragma unroll
-                    for (int m = 2; m < NB; m++) {
-                        const float t2 = fmaf(u2, t1, -t0);
-                        const float em = ep[m * DL];
-                        G0 = fmaf(t2, em, G0);
-                        D1 = fmaf((float)m * w1, em, D1);
-                        const float w2 = fmaf(u2, w1, -w0);
-                        t0 = t1;
-                        t1 = t2;
-                        w0 = w1;
-                        w1 = w2;
+                    for (int i = 0; i < QEPT; i++) {
+                        keys[i] = queue[QEPT * tid + i];
+                        kmax = max(kmax, keys[i]);
                     }
-                    const int xy = rxy[slot];
-                    adj_chain(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, a.spkr,
-                              a.mic, G0, -2.0f * D1, acc);
+#pragma unroll
+                    for (int i = 0; i < QEPT; i++) queue[QEPT * tid + i] = 0;
+                    int key = wave_excl_max_nonneg(kmax);
+#pragma unroll
+                    for (int i = 0; i < QEPT; i++) {
+                        key = max(key, keys[i]);
+                        keys[i] = key;
+                    }
+                    for (int i = 0; i < QEPT; i++) {
+                        const int pos = QEPT * tid + i;
+                        if (pos >= nq) break;
+                        const int ki = i == 0 ? keys[0] : i == 1 ? keys[1] : i == 2 ? keys[2] : keys[3];
+                        const int slot = ki >> 15, nz = (ki & 8191) - 4096 + 2 * pos;
+                        const double vz = offs[Nx + Ny + nz];
+                        double d2 = rb2[slot];
+                        d2 = d2 + vz * vz;
+                        const double dist = sqrt_normal(d2);
+                        const double tau = dist * fsc;
+                        const double fl = floor(tau);
+                        const int n0 = (int)fl;
+                        if (n0 < lo_n0 || n0 > hi_n0) continue;
+                        const float u = fmaf(2.0f, (float)(tau - fl), -1.0f);
+                        const float u2 = u + u;
+                        const float* ep = E + (n0 - lo_n0);
+                        // T_m (first kind) and m U_{m-1} (derivative) by their recurrences
+                        float t0 = 1.0f, t1 = u, w0 = 1.0f, w1 = u2;  // T_0, T_1, U_0, U_1
+                        float G0 = fmaf(t1, ep[DL], ep[0]);
+                        float D1 = ep[DL];  // 1 * U_0 * E_1
+#pragma unroll
+                        for (int m = 2; m < NB; m++) {
+                            const float t2 = fmaf(u2, t1, -t0);
+                            const float em = ep[m * DL];
+                            G0 = fmaf(t2, em, G0);
+                            D1 = fmaf((float)m * w1, em, D1);
+                            const float w2 = fmaf(u2, w1, -w0);
+                            t0 = t1;
+                            t1 = t2;
+                            w0 = w1;
+                            w1 = w2;
+                        }
+                        const int xy = rxy[slot];
+                        adj_chain(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, spkr,
+                                  mic, G0, -2.0f * D1, acc, dist);
+                    }
                 }
                 __syncthreads();
             }
@@ -3040,6 +3141,24 @@ std::vector<float> make_adjoint_coefficients(const TapTables& T, int Tw, int NB,
         }
     }
     return c;
+}
+
+// LDS image of the basis adjoint's coefficients c ([nb][cl], make_adjoint_coefficients) as
+// FP16 pairs: x = 2^14 c, hi = x rounded to FP16, lo = (x - hi) 2^11 rounded to FP16; in
+// ncp copies, copy rho shifted right by rho halves, rows of cl + ncp: [hi, lo][ncp][nb][cl + ncp].
+std::vector<unsigned short> make_adjoint_half_copies(const std::vector<float>& c, int nb, int cl, int ncp) {
+    const size_t per = (size_t)ncp * nb * (cl + ncp);
+    std::vector<unsigned short> img(2 * per, 0);
+    for (size_t e = 0; e < per; e++) {
+        const int rho = (int)(e / ((size_t)nb * (cl + ncp)));
+        const int rm = (int)(e - (size_t)rho * nb * (cl + ncp));
+        const int m = rm / (cl + ncp), z = rm - m * (cl + ncp), src = z - rho;
+        const float x = (src >= 0 && src < cl) ? c[(size_t)m * cl + src] * 16384.0f : 0.0f;
+        const _Float16 hi = (_Float16)x;
+        img[e] = __builtin_bit_cast(unsigned short, hi);
+        img[e + per] = __builtin_bit_cast(unsigned short, (_Float16)((x - (float)hi) * 2048.0f));
+    }
+    return img;
 }
 
 size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut, bool pack, int ncp = 4) {
@@ -3811,7 +3930,7 @@ struct AdjLaunch {
     std::vector<int> rir_item_begin;
     std::vector<int> order;
     bool basis = false;      // basis adjoint (ism_adjoint_basis_kernel)
-    std::vector<float> actab;
+    std::vector<unsigned short> achs;  // basis: coefficient FP16 pairs (make_adjoint_half_copies)
     int adl = 0, agl = 0, aks = 0, acl = 0;
     size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, off_ord = 0, off_act = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
@@ -3869,10 +3988,12 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         const int nota = (A.adl + 255) / 256;
         A.agl = (int)align_up((size_t)std::max(A.adl + A.T.nJ - 1, 256 * nota - 16 + 4 * A.aks + 4), 16);
         const size_t nt = align_up((size_t)ntab, 4);
-        A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 4) + 4 * (size_t)ADJ_NB * A.acl +
-                4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB + 4 * (size_t)QB + 64;
-        if ((opt.method == METHOD_AUTO && tiles < AUTO_BASIS_MIN_TILES) || A.lds > 150 * 1024) {
-            GPURIR_CHECK(opt.method == METHOD_AUTO, "basis adjoint: LDS budget exceeded");
+        // axis tables, x runs and row starts, g' pairs, coefficient pairs (two copies), E,
+        // row arrays, queue, scan scratch
+        A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 4) + 8 * (size_t)ADJ_NB * (A.acl + 2) +
+                4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB + 4 * (size_t)QB + 128;
+        if ((opt.method == METHOD_AUTO && tiles < AUTO_BASIS_MIN_TILES) || A.lds > 150 * 1024 || A.agl > 4 * WGB) {
+            GPURIR_CHECK(opt.method == METHOD_AUTO, "basis adjoint: LDS budget exceeded or window too long");
             A.basis = false;
         }
     }
@@ -3882,7 +4003,7 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         A.opt = opt;
         A.mask = (A.Tw % 2) != 0;
         A.P = make_plan_basis(rooms, ob, A.T, false);
-        A.actab = make_adjoint_coefficients(A.T, A.Tw, ADJ_NB, A.acl);
+        A.achs = make_adjoint_half_copies(make_adjoint_coefficients(A.T, A.Tw, ADJ_NB, A.acl), ADJ_NB, A.acl, 2);
     } else {
         if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
         if (opt.chunk_images <= 0) {
@@ -3942,7 +4063,7 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
     A.off_rib = align_up(A.off_kt + sizeof(float4) * A.ktab.size(), 256);
     A.off_ord = align_up(A.off_rib + sizeof(int) * A.rir_item_begin.size(), 256);
     A.off_act = align_up(A.off_ord + sizeof(int) * (A.order.size() + 1), 256);
-    A.plan_bytes = align_up(A.off_act + sizeof(float) * (A.actab.size() + 1), 256);
+    A.plan_bytes = align_up(A.off_act + sizeof(unsigned short) * (A.achs.size() + 4), 256);
     A.blob.assign(A.plan_bytes, 0);
     std::memcpy(A.blob.data() + A.off_rooms, P.rooms.data(), sizeof(RoomDev) * P.rooms.size());
     if (!P.items.empty()) std::memcpy(A.blob.data() + A.off_items, P.items.data(), sizeof(Item) * P.items.size());
@@ -3950,7 +4071,8 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
     std::memcpy(A.blob.data() + A.off_kt, A.ktab.data(), sizeof(float4) * A.ktab.size());
     std::memcpy(A.blob.data() + A.off_rib, A.rir_item_begin.data(), sizeof(int) * A.rir_item_begin.size());
     if (!A.order.empty()) std::memcpy(A.blob.data() + A.off_ord, A.order.data(), sizeof(int) * A.order.size());
-    if (!A.actab.empty()) std::memcpy(A.blob.data() + A.off_act, A.actab.data(), sizeof(float) * A.actab.size());
+    if (!A.achs.empty())
+        std::memcpy(A.blob.data() + A.off_act, A.achs.data(), sizeof(unsigned short) * A.achs.size());
     A.ntab = (P.ntab_max + 3) / 4 * 4;
     A.gp_len = (int)align_up((size_t)P.tt_max + 2 * (size_t)(A.T.nJ - 1) + 4, 4);
     if (!A.basis)
@@ -4021,13 +4143,15 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.pi_tw = (float)(M_PI / (double)A.Tw);
         a.ntab_max = A.ntab;
         a.gp_len = A.gp_len;
-        a.actab = reinterpret_cast<const float*>(dp + A.off_act);
+        a.achs = reinterpret_cast<const unsigned short*>(dp + A.off_act);
         a.adl = A.adl;
         a.agl = A.agl;
         a.aks = A.aks;
         a.acl = A.acl;
         a.prm = A.prm;
-        const void* fn = A.basis  ? reinterpret_cast<const void*>(&ism_adjoint_basis_kernel)
+        const bool pat = a.spkr != 0 || a.mic != 0;
+        const void* fn = A.basis  ? (pat ? reinterpret_cast<const void*>(&ism_adjoint_basis_kernel<true>)
+                                         : reinterpret_cast<const void*>(&ism_adjoint_basis_kernel<false>))
                          : A.mask ? reinterpret_cast<const void*>(&ism_adjoint_kernel<true>)
                                   : reinterpret_cast<const void*>(&ism_adjoint_kernel<false>);
         if (int rc = allow_lds(fn, A.lds)) return rc;

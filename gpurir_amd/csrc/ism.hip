@@ -2368,8 +2368,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     int* rxy = reinterpret_cast<int*>(rb2 + WGB);        // [WGB] row nx | ny << 16
     int* queue = rxy + WGB;                              // [QB] run marks
     int* misc = queue + QB;                              // [32] scan scratch (two buffers at 16, 24)
-    constexpr int QEPT = QB / WGB;                       // consecutive queue entries per thread
-    constexpr int QSEG = 64 * QEPT;                      // entries per wave
+    constexpr int QEPT = QB / WGB;                       // most consecutive queue entries per thread
     for (int e = tid; e < NCP * NB * CLP / 2; e += WGB)  // 2 NCP NB CLP halves in 8-byte words
         reinterpret_cast<uint2*>(chs)[e] = reinterpret_cast<const uint2*>(a.achs)[e];
     for (int e = tid; e < QB; e += WGB) queue[e] = 0;    // run marks: zero between rounds
@@ -2581,7 +2580,10 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 // slot, run start s) has nz = base + 2 p, base = zf[g] - 2 (s - q0); each run
                 // writes its key (slot, g, base + 4096) at its first entry in the round and at
                 // every wave segment start it covers, and an entry's key is the largest one at
-                // or before it
+                // or before it.  A round gives each thread qe consecutive entries, as few as
+                // spread it over all the threads.
+                const int nq = min(QB, tot - q0);
+                const int qe = (nq + WGB - 1) / WGB, qseg = 64 * qe;
                 {
                     int s = ex;
 #pragma unroll
@@ -2590,32 +2592,32 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                         if (a0 < b0) {
                             const int key = ((tid * 4 + g) << 13) | (zf[g] - 2 * (s - q0) + 4096);
                             queue[a0] = key;
-                            for (int bnd = (a0 / QSEG + 1) * QSEG; bnd < b0; bnd += QSEG) queue[bnd] = key;
+                            for (int bnd = (a0 / qseg + 1) * qseg; bnd < b0; bnd += qseg) queue[bnd] = key;
                         }
                         s += zc[g];
                     }
                 }
                 __syncthreads();
-                const int nq = min(QB, tot - q0);
-                if (QSEG * wave < nq) {
+                if (qseg * wave < nq) {
                     static_assert(QEPT == 4, "entry keys are selected from four registers");
                     int keys[QEPT];
                     int kmax = 0;
 #pragma unroll
                     for (int i = 0; i < QEPT; i++) {
-                        keys[i] = queue[QEPT * tid + i];
+                        keys[i] = i < qe ? queue[qe * tid + i] : 0;
                         kmax = max(kmax, keys[i]);
                     }
 #pragma unroll
-                    for (int i = 0; i < QEPT; i++) queue[QEPT * tid + i] = 0;
+                    for (int i = 0; i < QEPT; i++)
+                        if (i < qe) queue[qe * tid + i] = 0;
                     int key = wave_excl_max_nonneg(kmax);
 #pragma unroll
                     for (int i = 0; i < QEPT; i++) {
                         key = max(key, keys[i]);
                         keys[i] = key;
                     }
-                    for (int i = 0; i < QEPT; i++) {
-                        const int pos = QEPT * tid + i;
+                    for (int i = 0; i < qe; i++) {
+                        const int pos = qe * tid + i;
                         if (pos >= nq) break;
                         const int ki = i == 0 ? keys[0] : i == 1 ? keys[1] : i == 2 ? keys[2] : keys[3];
                         const int slot = ki >> 15, nz = (ki & 8191) - 4096 + 2 * pos;

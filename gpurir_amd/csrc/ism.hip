@@ -796,7 +796,7 @@ __host__ __device__ constexpr int pack_field_bits(int nb, int m) {
     return m < nb / 2 ? 64 - pack_lo_bits(nb, m) : pack_lo_bits(nb, nb - 1 - m);
 }
 constexpr long long PACK_MAX_IMAGES = 1LL << 21;
-constexpr long long LONG_TILE_MIN_TILES = 4096;  // launches with fewer 2 BT tiles keep BT
+constexpr long long LONG_TILE_MIN_TILES = 2048;  // launches with fewer 2 BT tiles keep BT
 // Lookup-table mode on the same machinery: the table's taps are linear in the weight of
 // the 1/16-sample grid, so an image at 16 tau = 16 n0 + p + a (phase p < 16, 0 <= a < 1)
 // adds amp (1 - a) to phase row p and amp a to row p + 1 (row 0 of column n0 + 1 for
@@ -810,6 +810,7 @@ constexpr int QB_LUT = 1536;  // queue entries per round in the table mode (its 
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef short short4_t __attribute__((ext_vector_type(4)));
 
 // Inclusive prefix sum over the 64 lanes of a wave with DPP moves (VALU only):
@@ -1994,7 +1995,7 @@ struct AdjArgs {
     float halfTw, inv_tw2, pi_tw;  // Tw/2, 2/Tw, pi/Tw
     int ntab_max, gp_len;
     // basis adjoint: LDS image of the coefficients as FP16 pairs (make_adjoint_half_copies),
-    // row length of E, logical length of g' (stored with 4 halves of padding per 16), K steps
+    // row length of E, logical length of g' (stored with 8 halves of padding per 16), K steps
     // per lane group of the correlation products, coefficient row length
     const unsigned short* achs;
     int adl, agl, aks, acl;
@@ -2357,8 +2358,8 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     float* Fhi = Flo + ntab;
     int* xrun = reinterpret_cast<int*>(Fhi + ntab);      // [3][ntab + 4] per x entry: y runs (n0, count, n1)
     int* rowStart = xrun + 3 * (ntab + 4);               // [ntab + 4]
-    const int GP = GL + GL / 4;
-    // [2][GP] g'(lo_n0 + J0 + u) 2^sg as FP16 pairs (hi, then lo) at u + 4 (u >> 4)
+    const int GP = GL + GL / 2;
+    // [2][GP] g'(lo_n0 + J0 + u) 2^sg as FP16 pairs (hi, then lo) at u + 8 (u >> 4)
     unsigned short* gph = reinterpret_cast<unsigned short*>(rowStart + ntab + 4);
     constexpr int NCP = 2;                               // shifted coefficient copies
     const int CLP = CL + NCP;
@@ -2452,53 +2453,66 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 if (u < GL) {
                     const float x = gv[i] * gs;
                     const _Float16 hi = (_Float16)x;
-                    gph[u + 4 * (u >> 4)] = __builtin_bit_cast(unsigned short, hi);
-                    gph[GP + u + 4 * (u >> 4)] = __builtin_bit_cast(unsigned short, (_Float16)((x - (float)hi) * 2048.0f));
+                    gph[u + 8 * (u >> 4)] = __builtin_bit_cast(unsigned short, hi);
+                    gph[GP + u + 8 * (u >> 4)] = __builtin_bit_cast(unsigned short, (_Float16)((x - (float)hi) * 2048.0f));
                 }
             }
         }
         __syncthreads();
         // E_m[lo_n0 + d] = sum_k C[m][k] g'(lo_n0 + d + J0 + k): E[16 b + r] = sum_x A[r][x] X[x][b]
         // with A[r][x] = C[x - r] (Toeplitz) and X[x][b] = g'[16 b + x] (Hankel), the forward
-        // basis kernel's products with the coefficients unreversed; one task per (row m,
-        // 256 positions of E), the waves taking tasks in turn.  Lane (c, g) holds K indices
-        // g KS + 4 s + e (e < 4) of step s: coefficients C[g KS + 4 s + e - c] from copy
-        // c & 1 (4-byte aligned reads), g' at 16 c + g KS + 4 s + e.
+        // basis kernel's products with the coefficients unreversed, on v_mfma_f32_16x16x32_f16;
+        // one task per (rows m and m + NB/2, 256 positions of E), the waves taking tasks in
+        // turn.  Lane (c, g) holds K indices g KS + 8 s + e (e < 8) of step s in both operands:
+        // coefficients C[g KS + 8 s + e - c] from copy c & 1 (4-byte aligned reads), g' at
+        // 16 c + g KS + 8 s + e (one 16-byte read).
         {
             const int c = lane & 15, g = lane >> 4, rho = c & (NCP - 1);
-            const int nota = (DL + 255) / 256, nst = KS / 4;
+            const int nota = (DL + 255) / 256, nst = KS / 8;
+            constexpr int MH = NB / 2;
+            static_assert(NB % 2 == 0, "row pairs");
+            const int LO = NCP * NB * CLP;                   // halves from the hi to the lo copies
             const float oscale = ldexpf(1.0f, -(14 + sg));  // coefficients carry 2^14, g' 2^sg
-            auto ld_coef = [&](const unsigned short* p) -> half4_t {
+            auto ld_coef = [&](const unsigned short* p) -> half8_t {
                 const unsigned* q = reinterpret_cast<const unsigned*>(p);
-                uint2 v;
+                uint4 v;
                 v.x = q[0];
                 v.y = q[1];
-                return __builtin_bit_cast(half4_t, v);
+                v.z = q[2];
+                v.w = q[3];
+                return __builtin_bit_cast(half8_t, v);
             };
-            for (int task = wave; task < NB * nota; task += NWB) {
-                const int m = task / nota, ot = task - m * nota;
-                const unsigned short* Ap = chs + (rho * NB + m) * CLP + g * KS + 16 - c + rho;
+            for (int task = wave; task < MH * nota; task += NWB) {
+                const int mp = task / nota, ot = task - mp * nota;
+                const unsigned short* A0 = chs + (rho * NB + mp) * CLP + g * KS + 16 - c + rho;
+                const unsigned short* A1 = A0 + MH * CLP;
                 const int kb = 256 * ot + 16 * c + g * KS;
-                floatx4 ahh = {0.0f, 0.0f, 0.0f, 0.0f}, axx = {0.0f, 0.0f, 0.0f, 0.0f};
-                half4_t ah = ld_coef(Ap), al = ld_coef(Ap + NCP * NB * CLP);
-                half4_t xh = *reinterpret_cast<const half4_t*>(gph + kb + 4 * (kb >> 4));
-                half4_t xl = *reinterpret_cast<const half4_t*>(gph + GP + kb + 4 * (kb >> 4));
+                floatx4 h0 = {0.0f, 0.0f, 0.0f, 0.0f}, x0 = h0, h1 = h0, x1 = h0;
+                half8_t a0h = ld_coef(A0), a0l = ld_coef(A0 + LO), a1h = ld_coef(A1), a1l = ld_coef(A1 + LO);
+                half8_t xh = *reinterpret_cast<const half8_t*>(gph + kb + 8 * (kb >> 4));
+                half8_t xl = *reinterpret_cast<const half8_t*>(gph + GP + kb + 8 * (kb >> 4));
                 for (int st = 0; st < nst; st++) {
                     // operands of step st + 1 are read before the products of step st
-                    half4_t nah = ah, nal = al, nxh = xh, nxl = xl;
+                    half8_t n0h = a0h, n0l = a0l, n1h = a1h, n1l = a1l, nxh = xh, nxl = xl;
                     if (st + 1 < nst) {
-                        const unsigned short* an = Ap + 4 * (st + 1);
-                        nah = ld_coef(an);
-                        nal = ld_coef(an + NCP * NB * CLP);
-                        const int dn = kb + 4 * (st + 1);
-                        nxh = *reinterpret_cast<const half4_t*>(gph + dn + 4 * (dn >> 4));
-                        nxl = *reinterpret_cast<const half4_t*>(gph + GP + dn + 4 * (dn >> 4));
+                        const int o = 8 * (st + 1), dn = kb + o;
+                        n0h = ld_coef(A0 + o);
+                        n0l = ld_coef(A0 + LO + o);
+                        n1h = ld_coef(A1 + o);
+                        n1l = ld_coef(A1 + LO + o);
+                        nxh = *reinterpret_cast<const half8_t*>(gph + dn + 8 * (dn >> 4));
+                        nxl = *reinterpret_cast<const half8_t*>(gph + GP + dn + 8 * (dn >> 4));
                     }
-                    ahh = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xh, ahh, 0, 0, 0);
-                    axx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xl, axx, 0, 0, 0);
-                    axx = __builtin_amdgcn_mfma_f32_16x16x16f16(al, xh, axx, 0, 0, 0);
-                    ah = nah;
-                    al = nal;
+                    h0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, xh, h0, 0, 0, 0);
+                    x0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, xl, x0, 0, 0, 0);
+                    h1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, xh, h1, 0, 0, 0);
+                    x1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, xl, x1, 0, 0, 0);
+                    x0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0l, xh, x0, 0, 0, 0);
+                    x1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, xh, x1, 0, 0, 0);
+                    a0h = n0h;
+                    a0l = n0l;
+                    a1h = n1h;
+                    a1l = n1l;
                     xh = nxh;
                     xl = nxl;
                 }
@@ -2506,7 +2520,10 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 const int d0 = 256 * ot + 16 * c + 4 * g;
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    if (d0 + i < DL) E[m * DL + d0 + i] = (ahh[i] + axx[i] * (1.0f / 2048.0f)) * oscale;
+                    if (d0 + i < DL) {
+                        E[mp * DL + d0 + i] = (h0[i] + x0[i] * (1.0f / 2048.0f)) * oscale;
+                        E[(mp + MH) * DL + d0 + i] = (h1[i] + x1[i] * (1.0f / 2048.0f)) * oscale;
+                    }
             }
         }
         // rows: per x entry, the y entries with x^2 + y^2 <= d2hi (one run per parity)
@@ -3985,14 +4002,14 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
             ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         }
         A.adl = (int)align_up((size_t)(BT + A.T.nJ - 1), 16);
-        A.aks = 4 * ((A.T.nJ + 15 + 15) / 16);
+        A.aks = 8 * ((A.T.nJ + 15 + 31) / 32);  // K per lane group: 4 aks >= nJ + 15, steps of 8
         A.acl = 4 * A.aks + 16;
         const int nota = (A.adl + 255) / 256;
         A.agl = (int)align_up((size_t)std::max(A.adl + A.T.nJ - 1, 256 * nota - 16 + 4 * A.aks + 4), 16);
         const size_t nt = align_up((size_t)ntab, 4);
         // axis tables, x runs and row starts, g' pairs, coefficient pairs (two copies), E,
         // row arrays, queue, scan scratch
-        A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 4) + 8 * (size_t)ADJ_NB * (A.acl + 2) +
+        A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 2) + 8 * (size_t)ADJ_NB * (A.acl + 2) +
                 4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB + 4 * (size_t)QB + 128;
         if ((opt.method == METHOD_AUTO && tiles < AUTO_BASIS_MIN_TILES) || A.lds > 150 * 1024 || A.agl > 4 * WGB) {
             GPURIR_CHECK(opt.method == METHOD_AUTO, "basis adjoint: LDS budget exceeded or window too long");

@@ -2139,6 +2139,38 @@ __device__ __forceinline__ void adj_chain(const RoomK& R, int nx, int ny, int nz
     acc[11] -= gvz;
 }
 
+// adj_chain with omnidirectional source and receiver (no directivity, no orientation terms):
+// with amp = att / (4 pi d), d amp / d v = -amp v / d^2 and d tau / d v = (fs/c) v / d, the
+// position terms are g_v = -amp (G0 / d + G1 fs/c) v / d.
+__device__ __forceinline__ void adj_chain_omni(const RoomK& R, int nx, int ny, int nz, const double* offs,
+                                               const float* F, const float* Flo, const float* Fhi, float G0, float G1,
+                                               double acc[12], double d) {
+    const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+    const int ey = Nx + ny, ez = Nx + Ny + nz;
+    const double id = rcp_normal(d);
+    const double Fx = F[nx], Fy = F[ey], Fz = F[ez];
+    const double g0 = (double)G0, g1 = (double)G1;
+    const double a0 = id * 0.079577471545947668;  // 1 / (4 pi d)
+    const double gbeta = g0 * a0;
+    const double Fxy = Fx * Fy;
+    const double gyz = gbeta * (Fy * Fz), gxz = gbeta * (Fx * Fz), gxy = gbeta * Fxy;
+    acc[0] += gyz * (double)Flo[nx];
+    acc[1] += gyz * (double)Fhi[nx];
+    acc[2] += gxz * (double)Flo[ey];
+    acc[3] += gxz * (double)Fhi[ey];
+    acc[4] += gxy * (double)Flo[ez];
+    acc[5] += gxy * (double)Fhi[ez];
+    const double amp = Fxy * Fz * a0;
+    const double k = -amp * fma(g1, R.fs_over_c, g0 * id) * id;
+    const double gvx = k * offs[nx], gvy = k * offs[ey], gvz = k * offs[ez];
+    acc[6] += ((nx - Nx / 2) & 1) ? -gvx : gvx;
+    acc[7] += ((ny - Ny / 2) & 1) ? -gvy : gvy;
+    acc[8] += ((nz - Nz / 2) & 1) ? -gvz : gvz;
+    acc[9] -= gvx;
+    acc[10] -= gvy;
+    acc[11] -= gvz;
+}
+
 struct AdjImg {
     double ox, oy, oz;  // unit source / receiver orientation (receiver: or*, source: os*)
 };
@@ -2328,9 +2360,9 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
 // 5e-7), an image at tau = n0 + f has
 //   G0 = sum_t g'(t) h(t - tau)  = sum_m T_m(u) E_m[n0],   E_m[n] = sum_j C[m][j] g'(n + j),
 //   G1 = sum_t g'(t) h'(t - tau) = -2 sum_m m U_{m-1}(u) E_m[n0]
-// (h'(x) = -d h_j / df at x = j - f, du/df = 2, T_m' = m U_{m-1}).  Per 512-sample tile the
-// workgroup builds g' on the tile (zero outside it, so that every tile adds only its own
-// taps), correlates it with the coefficient rows on the matrix cores (E; both operands as
+// (h'(x) = -d h_j / df at x = j - f, du/df = 2, T_m' = m U_{m-1}).  Per tile of 512 delays
+// the workgroup builds g' around the tile (the images of the tile see their whole window),
+// correlates it with the coefficient rows on the matrix cores (E; both operands as
 // FP16 pairs x = hi + lo 2^-11, three products hi hi, hi lo, lo hi per K step, 22 bits of
 // each operand), enumerates the images of the tile's delay range as the forward basis
 // kernel does, and sends each live image's G0, G1 through adj_chain into per-lane FP64 sums,
@@ -2415,10 +2447,16 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     for (int k = 0; k < 12; k++) acc[k] = 0.0;
     __syncthreads();
 
+    // The item's output samples [t_lo, t_hi) name its delays: n0 in [t_lo - J1, t_hi - J1), the
+    // RIR's last item through nISM - 1 - J0 (the last delay with a tap in the ISM part), so
+    // the items of a RIR partition its images and each image is evaluated once, in the tile
+    // holding its n0, against g' over its whole window.
+    const int n0_end = it.t_hi >= R.nISM ? R.nISM - J0 : it.t_hi - J1;  // exclusive
     for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
-        const int th = min(tl + BT, it.t_hi);
-        const int lo_n0 = tl - J1, hi_n0 = th - 1 - J0;
-        // g' on the tile, zero outside it: dL/dout plus the envelope-window term (GL <= 4 WGB)
+        const int lo_n0 = tl - J1, hi_n0 = tl + BT >= it.t_hi ? n0_end - 1 : tl + BT - J1 - 1;
+        const int nE = hi_n0 - lo_n0 + 1;  // E positions of the tile (<= BT + nJ - 1 <= DL)
+        // g' around the tile: dL/dout plus the envelope-window term on the ISM part, zero
+        // outside it (GL <= 4 WGB)
         float gv[4];
         float gm = 0.0f;
 #pragma unroll
@@ -2426,7 +2464,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
             const int u = tid + i * WGB;
             const int t = lo_n0 + J0 + u;
             float g = 0.0f;
-            if (u < GL && t >= tl && t < th) {
+            if (u < GL && t >= 0 && t < R.nISM) {
                 g = a.gy[row_off + t];
                 if (t >= ws) g = fmaf(cw, a.y[row_off + t], g);
             }
@@ -2468,7 +2506,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
         // 16 c + g KS + 8 s + e (one 16-byte read).
         {
             const int c = lane & 15, g = lane >> 4, rho = c & (NCP - 1);
-            const int nota = (DL + 255) / 256, nst = KS / 8;
+            const int nota = (nE + 255) / 256, nst = KS / 8;
             constexpr int MH = NB / 2;
             static_assert(NB % 2 == 0, "row pairs");
             const int LO = NCP * NB * CLP;                   // halves from the hi to the lo copies
@@ -2520,7 +2558,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 const int d0 = 256 * ot + 16 * c + 4 * g;
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    if (d0 + i < DL) {
+                    if (d0 + i < nE) {
                         E[mp * DL + d0 + i] = (h0[i] + x0[i] * (1.0f / 2048.0f)) * oscale;
                         E[(mp + MH) * DL + d0 + i] = (h1[i] + x1[i] * (1.0f / 2048.0f)) * oscale;
                     }
@@ -2666,8 +2704,11 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                             w1 = w2;
                         }
                         const int xy = rxy[slot];
-                        adj_chain(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz, spkr,
-                                  mic, G0, -2.0f * D1, acc, dist);
+                        if constexpr (PAT)
+                            adj_chain(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz,
+                                      spkr, mic, G0, -2.0f * D1, acc, dist);
+                        else
+                            adj_chain_omni(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, G0, -2.0f * D1, acc, dist);
                     }
                 }
                 __syncthreads();

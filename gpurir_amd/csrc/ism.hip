@@ -1998,7 +1998,7 @@ struct AdjArgs {
     // row length of E, logical length of g' (stored with 8 halves of padding per 16), K steps
     // per lane group of the correlation products, coefficient row length
     const unsigned short* achs;
-    int adl, agl, aks, acl;
+    int abt, adl, agl, aks, acl;  // abt: delays per tile
     RoomParams prm;
 };
 
@@ -2360,8 +2360,9 @@ __global__ __launch_bounds__(WG) void ism_adjoint_kernel(AdjArgs a) {
 // 5e-7), an image at tau = n0 + f has
 //   G0 = sum_t g'(t) h(t - tau)  = sum_m T_m(u) E_m[n0],   E_m[n] = sum_j C[m][j] g'(n + j),
 //   G1 = sum_t g'(t) h'(t - tau) = -2 sum_m m U_{m-1}(u) E_m[n0]
-// (h'(x) = -d h_j / df at x = j - f, du/df = 2, T_m' = m U_{m-1}).  Per tile of 512 delays
-// the workgroup builds g' around the tile (the images of the tile see their whole window),
+// (h'(x) = -d h_j / df at x = j - f, du/df = 2, T_m' = m U_{m-1}).  Per tile of 768 delays
+// (512 where two workgroups would not fit a CU's LDS) the workgroup builds g' around the
+// tile (the images of the tile see their whole window),
 // correlates it with the coefficient rows on the matrix cores (E; both operands as
 // FP16 pairs x = hi + lo 2^-11, three products hi hi, hi lo, lo hi per K step, 22 bits of
 // each operand), enumerates the images of the tile's delay range as the forward basis
@@ -2452,9 +2453,10 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     // the items of a RIR partition its images and each image is evaluated once, in the tile
     // holding its n0, against g' over its whole window.
     const int n0_end = it.t_hi >= R.nISM ? R.nISM - J0 : it.t_hi - J1;  // exclusive
-    for (int tl = it.t_lo; tl < it.t_hi; tl += BT) {
-        const int lo_n0 = tl - J1, hi_n0 = tl + BT >= it.t_hi ? n0_end - 1 : tl + BT - J1 - 1;
-        const int nE = hi_n0 - lo_n0 + 1;  // E positions of the tile (<= BT + nJ - 1 <= DL)
+    const int ABT = a.abt;
+    for (int tl = it.t_lo; tl < it.t_hi; tl += ABT) {
+        const int lo_n0 = tl - J1, hi_n0 = tl + ABT >= it.t_hi ? n0_end - 1 : tl + ABT - J1 - 1;
+        const int nE = hi_n0 - lo_n0 + 1;  // E positions of the tile (<= ABT + nJ - 1 <= DL)
         // g' around the tile: dL/dout plus the envelope-window term on the ISM part, zero
         // outside it (GL <= 4 WGB)
         float gv[4];
@@ -3991,7 +3993,7 @@ struct AdjLaunch {
     std::vector<int> order;
     bool basis = false;      // basis adjoint (ism_adjoint_basis_kernel)
     std::vector<unsigned short> achs;  // basis: coefficient FP16 pairs (make_adjoint_half_copies)
-    int adl = 0, agl = 0, aks = 0, acl = 0;
+    int abt = BT, adl = 0, agl = 0, aks = 0, acl = 0;
     size_t off_rooms = 0, off_items = 0, off_rr = 0, off_kt = 0, off_rib = 0, off_ord = 0, off_act = 0, plan_bytes = 0;
     std::vector<unsigned char> blob;
     size_t lds = 0;
@@ -4042,16 +4044,21 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
             tiles += (long long)h.n_src * h.n_rcv * ((n_samples_rule(h.Tdiff, opt.fs) + BT - 1) / BT);
             ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         }
-        A.adl = (int)align_up((size_t)(BT + A.T.nJ - 1), 16);
         A.aks = 8 * ((A.T.nJ + 15 + 31) / 32);  // K per lane group: 4 aks >= nJ + 15, steps of 8
         A.acl = 4 * A.aks + 16;
-        const int nota = (A.adl + 255) / 256;
-        A.agl = (int)align_up((size_t)std::max(A.adl + A.T.nJ - 1, 256 * nota - 16 + 4 * A.aks + 4), 16);
         const size_t nt = align_up((size_t)ntab, 4);
-        // axis tables, x runs and row starts, g' pairs, coefficient pairs (two copies), E,
-        // row arrays, queue, scan scratch
-        A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 2) + 8 * (size_t)ADJ_NB * (A.acl + 2) +
-                4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB + 4 * (size_t)QB + 128;
+        // tiles of 768 delays where two workgroups still fit a CU's LDS, else 512
+        for (int abt : {3 * BT / 2, BT}) {
+            A.abt = abt;
+            A.adl = (int)align_up((size_t)(abt + A.T.nJ - 1), 16);
+            const int nota = (A.adl + 255) / 256;
+            A.agl = (int)align_up((size_t)std::max(A.adl + A.T.nJ - 1, 256 * nota - 16 + 4 * A.aks + 4), 16);
+            // axis tables, x runs and row starts, g' pairs, coefficient pairs (two copies), E,
+            // row arrays, queue, scan scratch
+            A.lds = 20 * nt + 16 * (nt + 4) + 4 * ((size_t)A.agl + A.agl / 2) + 8 * (size_t)ADJ_NB * (A.acl + 2) +
+                    4 * (size_t)ADJ_NB * A.adl + 12 * (size_t)WGB + 4 * (size_t)QB + 128;
+            if (2 * A.lds <= 160 * 1024) break;
+        }
         if ((opt.method == METHOD_AUTO && tiles < AUTO_BASIS_MIN_TILES) || A.lds > 150 * 1024 || A.agl > 4 * WGB) {
             GPURIR_CHECK(opt.method == METHOD_AUTO, "basis adjoint: LDS budget exceeded or window too long");
             A.basis = false;
@@ -4062,7 +4069,7 @@ int adj_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt_in, Ad
         ob.tile_samples = 0;  // the planner's default tiles per item
         A.opt = opt;
         A.mask = (A.Tw % 2) != 0;
-        A.P = make_plan_basis(rooms, ob, A.T, false);
+        A.P = make_plan_basis(rooms, ob, A.T, false, A.abt);
         A.achs = make_adjoint_half_copies(make_adjoint_coefficients(A.T, A.Tw, ADJ_NB, A.acl), ADJ_NB, A.acl, 2);
     } else {
         if (opt.tile_samples <= 0) opt.tile_samples = std::max(1, std::min(nism_max, ADJ_TILE));
@@ -4204,6 +4211,7 @@ int adj_enqueue(const AdjLaunch& A, const void* d_plan, void* d_ws, const float*
         a.ntab_max = A.ntab;
         a.gp_len = A.gp_len;
         a.achs = reinterpret_cast<const unsigned short*>(dp + A.off_act);
+        a.abt = A.abt;
         a.adl = A.adl;
         a.agl = A.agl;
         a.aks = A.aks;

@@ -2397,7 +2397,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
     constexpr int NCP = 2;                               // shifted coefficient copies
     const int CLP = CL + NCP;
     unsigned short* chs = gph + 2 * GP;                  // [2][NCP][NB][CLP] coefficient FP16 pairs
-    float* E = reinterpret_cast<float*>(chs + 2 * NCP * NB * CLP);  // [NB][DL] E_m[lo_n0 + d]
+    float* E = reinterpret_cast<float*>(chs + 2 * NCP * NB * CLP);  // [DL][NB] E_m[lo_n0 + d] at d NB + m
     double* rb2 = reinterpret_cast<double*>(E + NB * DL);  // [WGB] row x^2 + y^2
     int* rxy = reinterpret_cast<int*>(rb2 + WGB);        // [WGB] row nx | ny << 16
     int* queue = rxy + WGB;                              // [QB] run marks
@@ -2561,8 +2561,8 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     if (d0 + i < nE) {
-                        E[mp * DL + d0 + i] = (h0[i] + x0[i] * (1.0f / 2048.0f)) * oscale;
-                        E[(mp + MH) * DL + d0 + i] = (h1[i] + x1[i] * (1.0f / 2048.0f)) * oscale;
+                        E[(d0 + i) * NB + mp] = (h0[i] + x0[i] * (1.0f / 2048.0f)) * oscale;
+                        E[(d0 + i) * NB + mp + MH] = (h1[i] + x1[i] * (1.0f / 2048.0f)) * oscale;
                     }
             }
         }
@@ -2688,15 +2688,23 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                         if (n0 < lo_n0 || n0 > hi_n0) continue;
                         const float u = fmaf(2.0f, (float)(tau - fl), -1.0f);
                         const float u2 = u + u;
-                        const float* ep = E + (n0 - lo_n0);
+                        // E_0 .. E_9 of the delay: five 8-byte reads
+                        const float2* ep = reinterpret_cast<const float2*>(E + (n0 - lo_n0) * NB);
+                        float e[NB];
+#pragma unroll
+                        for (int q = 0; q < NB / 2; q++) {
+                            const float2 v = ep[q];
+                            e[2 * q] = v.x;
+                            e[2 * q + 1] = v.y;
+                        }
                         // T_m (first kind) and m U_{m-1} (derivative) by their recurrences
                         float t0 = 1.0f, t1 = u, w0 = 1.0f, w1 = u2;  // T_0, T_1, U_0, U_1
-                        float G0 = fmaf(t1, ep[DL], ep[0]);
-                        float D1 = ep[DL];  // 1 * U_0 * E_1
+                        float G0 = fmaf(t1, e[1], e[0]);
+                        float D1 = e[1];  // 1 * U_0 * E_1
 #pragma unroll
                         for (int m = 2; m < NB; m++) {
                             const float t2 = fmaf(u2, t1, -t0);
-                            const float em = ep[m * DL];
+                            const float em = e[m];
                             G0 = fmaf(t2, em, G0);
                             D1 = fmaf((float)m * w1, em, D1);
                             const float w2 = fmaf(u2, w1, -w0);

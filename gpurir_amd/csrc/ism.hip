@@ -144,6 +144,17 @@ __device__ inline double rcp_normal(double x) {
     return fma(r, e, r);
 }
 
+// Square root of a positive normal double for the basis kernels' image delays: the v_rsq_f64
+// estimate (relative error below 2^-22) and one Goldschmidt step, which squares that error:
+// about 44 correct bits, so a delay of up to 2^13 samples is off by less than 1e-9 sample,
+// far below the tap expansion's 1e-6 (sqrt_normal takes two more residual corrections).
+__device__ inline double sqrt_delay(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    return fma(g, r, g);
+}
+
 __device__ inline double ipow(double b, int e) {
     double r = 1.0;
     while (e > 0) {
@@ -1231,7 +1242,7 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     const double vz = offs[Nx + Ny + nz];
                     double d2 = rb2[slot];
                     d2 = d2 + vz * vz;
-                    const double dist = sqrt_normal(d2);
+                    const double dist = sqrt_delay(d2);
                     const double tau = dist * fsc;
                     const double fl = floor(tau);
                     const int n0 = (int)fl;
@@ -2681,7 +2692,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                         const double vz = offs[Nx + Ny + nz];
                         double d2 = rb2[slot];
                         d2 = d2 + vz * vz;
-                        const double dist = sqrt_normal(d2);
+                        const double dist = sqrt_delay(d2);
                         const double tau = dist * fsc;
                         const double fl = floor(tau);
                         const int n0 = (int)fl;

@@ -896,8 +896,8 @@ __device__ inline int block_scan_excl1(int v, int* tmp, int* total) {
 }
 
 // sqrt of a non-negative double for the enumeration's distance bounds (widened by 1e-7 of
-// 2L, so values below 1e-200 may read as 0)
-__device__ inline double sqrt_bound(double x) { return x > 1e-200 ? sqrt_normal(x) : 0.0; }
+// 2L, far above sqrt_delay's 2^-44, so values below 1e-200 may read as 0)
+__device__ inline double sqrt_bound(double x) { return x > 1e-200 ? sqrt_delay(x) : 0.0; }
 
 // Image entries of one axis in closed form: entry n has k = n - N/2 and offset
 // 2 m L + c_q - (rounding) with m = ceil(k / 2), q = k & 1, c_0 = s - r, c_1 = -s - r

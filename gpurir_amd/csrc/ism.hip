@@ -155,6 +155,15 @@ __device__ inline double sqrt_delay(double x) {
     return fma(g, r, g);
 }
 
+// sqrt_delay and the reciprocal square root from the same Goldschmidt step (both ~44 bits)
+__device__ inline double sqrt_delay_inv(double x, double& inv) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    inv = 2.0 * fma(h, r, h);
+    return fma(g, r, g);
+}
+
 __device__ inline double ipow(double b, int e) {
     double r = 1.0;
     while (e > 0) {
@@ -2150,15 +2159,15 @@ __device__ __forceinline__ void adj_chain(const RoomK& R, int nx, int ny, int nz
     acc[11] -= gvz;
 }
 
-// adj_chain with omnidirectional source and receiver (no directivity, no orientation terms):
+// adj_chain with omnidirectional source and receiver (no directivity, no orientation terms;
+// id = 1 / d):
 // with amp = att / (4 pi d), d amp / d v = -amp v / d^2 and d tau / d v = (fs/c) v / d, the
 // position terms are g_v = -amp (G0 / d + G1 fs/c) v / d.
 __device__ __forceinline__ void adj_chain_omni(const RoomK& R, int nx, int ny, int nz, const double* offs,
                                                const float* F, const float* Flo, const float* Fhi, float G0, float G1,
-                                               double acc[12], double d) {
+                                               double acc[12], double id) {
     const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
     const int ey = Nx + ny, ez = Nx + Ny + nz;
-    const double id = rcp_normal(d);
     const double Fx = F[nx], Fy = F[ey], Fz = F[ez];
     const double g0 = (double)G0, g1 = (double)G1;
     const double a0 = id * 0.079577471545947668;  // 1 / (4 pi d)
@@ -2692,7 +2701,8 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                         const double vz = offs[Nx + Ny + nz];
                         double d2 = rb2[slot];
                         d2 = d2 + vz * vz;
-                        const double dist = sqrt_delay(d2);
+                        double idist;
+                        const double dist = sqrt_delay_inv(d2, idist);
                         const double tau = dist * fsc;
                         const double fl = floor(tau);
                         const int n0 = (int)fl;
@@ -2729,7 +2739,7 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                             adj_chain(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, orx, ory, orz, osx, osy, osz,
                                       spkr, mic, G0, -2.0f * D1, acc, dist);
                         else
-                            adj_chain_omni(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, G0, -2.0f * D1, acc, dist);
+                            adj_chain_omni(R, xy & 0xffff, xy >> 16, nz, offs, F, Flo, Fhi, G0, -2.0f * D1, acc, idist);
                     }
                 }
                 __syncthreads();

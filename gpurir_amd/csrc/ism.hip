@@ -2542,6 +2542,16 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                 v.w = q[3];
                 return __builtin_bit_cast(half8_t, v);
             };
+            // lanes c >= 8 of each row of 16 get v from lane c - 8, lanes c < 8 keep f
+            auto shr8_or = [](half8_t f, half8_t v) -> half8_t {
+                uint4 fu = __builtin_bit_cast(uint4, f);
+                const uint4 vu = __builtin_bit_cast(uint4, v);
+                fu.x = __builtin_amdgcn_update_dpp(fu.x, vu.x, 0x118, 0xf, 0xf, false);  // row_shr:8
+                fu.y = __builtin_amdgcn_update_dpp(fu.y, vu.y, 0x118, 0xf, 0xf, false);
+                fu.z = __builtin_amdgcn_update_dpp(fu.z, vu.z, 0x118, 0xf, 0xf, false);
+                fu.w = __builtin_amdgcn_update_dpp(fu.w, vu.w, 0x118, 0xf, 0xf, false);
+                return __builtin_bit_cast(half8_t, fu);
+            };
             for (int task = wave; task < MH * nota; task += NWB) {
                 const int mp = task / nota, ot = task - mp * nota;
                 const unsigned short* A0 = chs + (rho * NB + mp) * CLP + g * KS + 16 - c + rho;
@@ -2556,10 +2566,19 @@ __global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(4))) void i
                     half8_t n0h = a0h, n0l = a0l, n1h = a1h, n1l = a1l, nxh = xh, nxl = xl;
                     if (st + 1 < nst) {
                         const int o = 8 * (st + 1), dn = kb + o;
-                        n0h = ld_coef(A0 + o);
-                        n0l = ld_coef(A0 + LO + o);
-                        n1h = ld_coef(A1 + o);
-                        n1l = ld_coef(A1 + LO + o);
+                        // the Toeplitz operand of lane c at step st + 1 is lane c - 8's at step
+                        // st: lanes c >= 8 take it by a row shift, only lanes c < 8 read LDS
+                        half8_t f0h = a0h, f0l = a0l, f1h = a1h, f1l = a1l;
+                        if (c < 8) {
+                            f0h = ld_coef(A0 + o);
+                            f0l = ld_coef(A0 + LO + o);
+                            f1h = ld_coef(A1 + o);
+                            f1l = ld_coef(A1 + LO + o);
+                        }
+                        n0h = shr8_or(f0h, a0h);
+                        n0l = shr8_or(f0l, a0l);
+                        n1h = shr8_or(f1h, a1h);
+                        n1l = shr8_or(f1l, a1l);
                         nxh = *reinterpret_cast<const half8_t*>(gph + dn + 8 * (dn >> 4));
                         nxl = *reinterpret_cast<const half8_t*>(gph + GP + dn + 8 * (dn >> 4));
                     }

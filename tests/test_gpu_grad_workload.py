@@ -4,8 +4,8 @@ differences (h = 1e-6) of the C oracle.
 Workload: 256 rooms (4-8 x 3-6 x 2.5-3.5 m, T60 0.3-0.8 s, 1 source x 4 omni receivers,
 fs 16 kHz, Tmax 0.5 s, Tdiff at 20 dB); the loss is the MSE between the simulated RIRs
 and targets simulated from betas scaled by 1 + U(-0.2, 0.2) and sources moved by
-N(0, 0.1^2) m.  The adjoint runs with the default (basis) method over every 512-sample
-tile of each RIR (up to 9 tiles, 122,472 images per room).
+N(0, 0.1^2) m.  The adjoint runs with the default (basis) method over every 768-delay
+tile of each RIR (up to 6 tiles, 122,472 images per room).
 
 Bar: every gradient group (6 betas, source xyz, receiver xyz, level) of every checked
 room within 1e-4 relative L2 of the FP64 finite differences.  The target is the GPU's
@@ -71,19 +71,24 @@ def test_gradient_workload_matches_finite_differences():
     print(f"largest rel-L2 against finite differences: {worst:.3e}")
 
 
-@pytest.mark.parametrize("method", ["basis", "taps"])
-def test_multi_tile_adjoint_small_room(method):
-    """Three full 512-sample tiles and a partial one per RIR (Tdiff 0.1 s at 16 kHz, every image
-    of the early part): the basis adjoint's per-tile correlation across tile boundaries and the
-    tap adjoint's tiles, against FP64 central differences of the C oracle."""
+@pytest.mark.parametrize("method,wide", [("basis", False), ("taps", False), ("basis", True)],
+                         ids=["basis", "taps", "basis-long-axis-tables"])
+def test_multi_tile_adjoint_small_room(method, wide):
+    """1600 early samples per RIR (Tdiff 0.1 s at 16 kHz, every image of the early part): the
+    basis adjoint's delay tiles (two of 768 delays and a last one that runs to the last delay
+    with a tap in the early part) and the tap adjoint's tiles, against FP64 central differences
+    of the C oracle.  With 100 images per axis (wide: 300 axis-table entries, of which the early
+    part uses about a third) the basis adjoint's LDS no longer fits two workgroups per CU with
+    768-delay tiles and it takes tiles of 512."""
     fs = 16000.0
     L = np.array([4.0, 3.5, 2.7], np.float32)
     rng = np.random.default_rng(11)
-    room = dict(room_sz=L, nb_img=[int(np.ceil(2 * 0.1 / (x / 343.0))) for x in L], Tmax=0.13, Tdiff=0.1,
+    nb = [100, 100, 100] if wide else [int(np.ceil(2 * 0.1 / (x / 343.0))) for x in L]
+    room = dict(room_sz=L, nb_img=nb, Tmax=0.13, Tdiff=0.1,
                 beta=rng.uniform(0.7, 0.95, 6).astype(np.float32),
                 pos_src=(0.5 + rng.uniform(size=(1, 3)) * (L - 1.0)).astype(np.float32),
                 pos_rcv=(0.5 + rng.uniform(size=(2, 3)) * (L - 1.0)).astype(np.float32), level=0.8)
-    assert orc.n_samples(room["Tdiff"], fs) == 1600  # 3 full tiles + 64 samples
+    assert orc.n_samples(room["Tdiff"], fs) == 1600  # 2 x 768 + 64
     dev = torch.device("cuda", 0)
     b = torch.tensor(room["beta"], device=dev, requires_grad=True)
     s = torch.tensor(room["pos_src"], device=dev, requires_grad=True)
@@ -98,7 +103,7 @@ def test_multi_tile_adjoint_small_room(method):
            "rcv": rr.grad.double().cpu().numpy(), "level": float(lv.grad)}
     want = gc.room_mse_grads(dict(room, level=np.float32(0.8)), tgt, 1.0, fs)
     errs = gc.group_errors(got, want)
-    print(method, errs)
+    print(method, "wide" if wide else "", errs)
     for g, e in errs.items():
         assert e <= TOL, f"{method} d/d{g}: rel-L2 {e:.3e}"
 

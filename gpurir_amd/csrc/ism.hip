@@ -971,12 +971,14 @@ __device__ inline unsigned short half_bits(float x) {
 // overlaps a second tile half as often.
 template <int NB, int DLC, int H16, bool LUT, bool PK, int BTK = BT>
 __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned char* smem_raw) {
-    static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && !H16), "basis rows of a wave quarter");
+    static_assert((NB >= 4 && NB <= 8) || (LUT && NB == LUT_NB && (H16 == 0 || (H16 == 3 && DLC == 640))),
+                  "basis rows of a wave quarter");
     // SPL: FP32 filters from FP16 pairs (x = hi + lo 2^-11 for the coefficients and the rows,
     // hi lo' + lo hi' accumulated apart): three v_mfma_f32_16x16x16_f16 per 16 K values
     // instead of four v_mfma_f32_16x16x4_f32 per 4 K values
     constexpr bool SPL = H16 == 3;
-    static_assert(!SPL || (NB == 8 && !LUT), "FP16 pairs carry the 8-term FP32 expansion");
+    static_assert(!SPL || (NB == 8 && !LUT) || (LUT && NB == LUT_NB && DLC == 640),
+                  "FP16 pairs carry the 8-term FP32 expansion, or the table mode's 16 phase rows at 16 kHz");
     static_assert(!PK || (!LUT && NB % 2 == 0), "packed sums pair the Chebyshev rows");
     static_assert(BTK == BT || (BTK == 2 * BT && SPL && PK && DLC == 1152), "long tiles: FP16 pairs at 16 kHz");
     constexpr int NOTK = BTK / 256;        // 256-sample output blocks of a tile
@@ -984,7 +986,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     static_assert(UPW * NWB == 4 * NOTK, "whole products per wave");
     // 16-bit coefficient copies: NCP copies, copy rho shifted by rho entries (four: the four
     // operands of a lane start at a multiple of 4 halves, 8-byte reads; two: 4-byte reads)
-    constexpr int NCP = BTK == BT ? 4 : 2;
+    // (the table mode's FP16 pairs: one copy, rows of CL + 2 halves, read as three aligned
+    // words and a funnel shift: sixteen rows of copies would not leave two workgroups per CU)
+    constexpr int NCP = LUT && SPL ? 1 : BTK == BT ? 4 : 2;
+    constexpr int CPAD = NCP == 1 ? 2 : NCP;  // halves after each copy's row
     constexpr int QBK = LUT ? QB_LUT : QB;
     constexpr int QEPT = QBK / WGB;   // consecutive queue entries per thread
     constexpr int QSEG = 64 * QEPT;   // entries per wave
@@ -1008,9 +1013,9 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     const int dreg = (LUT || PK) ? max(NB * DL, (H16 == 1 || H16 == 2) ? 0 : NB * DLP) : 2 * NB * DL;  // floats
     const int nwords = PK ? NB / 2 * DL : NB * DL;  // 64-bit (or LUT 32-bit) words of the sums
     float* ctab = reinterpret_cast<float*>(D) + dreg;                 // [NB][CL] (H16: see chs)
-    // H16: NCP copies of the coefficients in 16 bits, [NCP][NB][CL + NCP] (SPL: hi, then lo)
+    // H16: NCP copies of the coefficients in 16 bits, [NCP][NB][CL + CPAD] (SPL: hi, then lo)
     unsigned short* chs = reinterpret_cast<unsigned short*>(ctab);
-    const int crw = H16 ? (SPL ? 2 : 1) * NCP * NB * (CL + NCP) / 2 : NB * CL;  // floats of the region
+    const int crw = H16 ? (SPL ? 2 : 1) * NCP * NB * (CL + CPAD) / 2 : NB * CL;  // floats of the region
     double* rb2 = reinterpret_cast<double*>(ctab + crw);              // [WGB] row x^2 + y^2
     float* ratt = reinterpret_cast<float*>(rb2 + WGB);                // [WGB] row att_x att_y
     int* rxy = reinterpret_cast<int*>(ratt + WGB);                    // [WGB] row nx | ny << 16
@@ -1483,9 +1488,9 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                 // steps each; the operand reads of step q + 1 are issued before the products
                 // of step q (three per block)
                 constexpr int KSC = 36, CLC = 4 * KSC + 16, NR = NB / 4, NSTEP = NR * KSC / 4;
-                constexpr int AHL = NCP * NB * (CLC + NCP);  // halves from the hi to the lo copies
+                constexpr int AHL = NCP * NB * (CLC + CPAD);  // halves from the hi to the lo copies
                 const int rho = c & (NCP - 1), m0 = mq * NR;
-                const unsigned short* Ap0 = chs + (rho * NB + m0) * (CLC + NCP) + g * KSC + 16 - c + rho;
+                const unsigned short* Ap0 = chs + (rho * NB + m0) * (CLC + CPAD) + g * KSC + 16 - c + rho;
                 const unsigned short* Bp0 = Dh + m0 * DLP;
                 int kbu[UPW];
 #pragma unroll
@@ -1497,21 +1502,30 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     ahh[u] = acc;
                     axx[u] = acc;
                 }
-                // four coefficients of a lane: 8-byte aligned with four copies, 4-byte with two
+                // four coefficients of a lane: 8-byte aligned with four copies, 4-byte with two;
+                // one copy: the three aligned words that hold them, funnel-shifted by the odd half
                 auto ld_coef = [&](const unsigned short* p) -> half4_t {
                     if constexpr (NCP == 4) {
                         return *reinterpret_cast<const half4_t*>(p);
-                    } else {
+                    } else if constexpr (NCP == 2) {
                         const unsigned* q = reinterpret_cast<const unsigned*>(p);
                         uint2 v;
                         v.x = q[0];
                         v.y = q[1];
                         return __builtin_bit_cast(half4_t, v);
+                    } else {
+                        const unsigned od = (unsigned)(p - chs) & 1u;
+                        const unsigned* q = reinterpret_cast<const unsigned*>(p - od);
+                        const unsigned w0 = q[0], w1 = q[1], w2 = q[2];
+                        uint2 v;
+                        v.x = __builtin_amdgcn_alignbit(w1, w0, 16u * od);
+                        v.y = __builtin_amdgcn_alignbit(w2, w1, 16u * od);
+                        return __builtin_bit_cast(half4_t, v);
                     }
                 };
                 auto load_step = [&](int q, int b) {
                     const int r = q / (KSC / 4), t4 = q % (KSC / 4);
-                    const unsigned short* ap = Ap0 + r * (CLC + NCP) + 4 * t4;
+                    const unsigned short* ap = Ap0 + r * (CLC + CPAD) + 4 * t4;
                     ah[b] = ld_coef(ap);
                     al[b] = ld_coef(ap + AHL);
 #pragma unroll
@@ -1546,6 +1560,7 @@ if (H16) {
                     // rows 2 mq, 2 mq + 1 (NB = 8) or mq, mq + 4 (NB = 6: waves 2 and 3 take one)
                     const int m = NB % 4 == 0 ? mq * (NB / 4) + mm : mq + 4 * mm;
                     if (NB % 4 != 0 && m >= NB) break;
+                    static_assert(NCP == 4, "the general 16-bit products read four copies");
                     const unsigned short* Ap = chs + (rho * NB + m) * (CL + 4) + g * KS + 16 - c + rho;
                     const unsigned short* Bp = Dh + m * DLP;
                     const int kb = 256 * ot + 16 * c + g * KS;
@@ -3210,12 +3225,13 @@ BasisTables make_lut_basis_tables(const TapTables& T) {
 // copies followed by the lo copies (the rounding error of hi times 2^11).  The kernel copies
 // the image into LDS unchanged.
 std::vector<unsigned short> make_half_copies(const BasisTables& B, int nb, int h16, int ncp = 4) {
-    const size_t per = (size_t)ncp * nb * (B.cl + ncp);
+    const int rs = B.cl + (ncp == 1 ? 2 : ncp);  // row stride: a single copy keeps rows word-aligned
+    const size_t per = (size_t)ncp * nb * rs;
     std::vector<unsigned short> img((h16 == 3 ? 2 : 1) * per, 0);
     for (size_t e = 0; e < per; e++) {
-        const int rho = (int)(e / ((size_t)nb * (B.cl + ncp)));
-        const int rm = (int)(e - (size_t)rho * nb * (B.cl + ncp));
-        const int m = rm / (B.cl + ncp), z = rm - m * (B.cl + ncp), src = z - rho;
+        const int rho = (int)(e / ((size_t)nb * rs));
+        const int rm = (int)(e - (size_t)rho * nb * rs);
+        const int m = rm / rs, z = rm - m * rs, src = z - rho;
         const float cv = (src >= 0 && src < B.cl) ? B.ctab[(size_t)m * B.cl + src] : 0.0f;
         if (h16 == 3) {
             const float vs = cv * 16384.0f;
@@ -3273,7 +3289,8 @@ std::vector<unsigned short> make_adjoint_half_copies(const std::vector<float>& c
 
 size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut, bool pack, int ncp = 4) {
     // 16-bit coefficients in ncp shifted copies (FP16 pairs: hi and lo), or FP32
-    const size_t coef = h16 ? (h16 == 3 ? 4 : 2) * (size_t)ncp * nb * (B.cl + ncp) : 4 * (size_t)nb * B.cl;
+    const size_t coef =
+        h16 ? (h16 == 3 ? 4 : 2) * (size_t)ncp * nb * (B.cl + (ncp == 1 ? 2 : ncp)) : 4 * (size_t)nb * B.cl;
     // the region holds the sums and, after the conversion, the FP32 (or 16-bit) rows
     const size_t f32rows = 4 * (size_t)nb * (B.dl + B.dl / 4);
     const size_t rows = lut    ? f32rows
@@ -3446,6 +3463,9 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
             return f->second;
         };
         S.Bt = tables(BT);
+        // the table mode's 16 phase rows: FP16-pair filters at 16 kHz (one coefficient copy),
+        // FP32 ones otherwise
+        if (S.lut) S.h16 = S.Bt.dl == 640 ? 3 : 0;
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         long long most_images = 0;
@@ -3465,7 +3485,8 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
             }
         }
         // (2 BT tiles convert with their own DLC / 128 words per thread)
-        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut, S.pack, S.bt == BT ? 4 : 2) > 160 * 1024 ||
+        if (basis_lds_bytes((ntab + 3) / 4 * 4, S.Bt, S.h16, S.nb, S.lut, S.pack, S.lut ? 1 : S.bt == BT ? 4 : 2) >
+                160 * 1024 ||
             (S.bt == BT && S.Bt.dl > BCS * WGB)) {
             // the tile's basis rows do not fit the conversion or the LDS (windows of more
             // than 513 taps, i.e. fs above 64 kHz, or very long axis tables): the tap
@@ -3481,7 +3502,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
             static std::mutex mu;
             static std::map<std::tuple<int, int, int, int>, std::vector<unsigned short>> cache;
             std::lock_guard<std::mutex> lk(mu);
-            const int ncp = S.bt == BT ? 4 : 2;
+            const int ncp = S.lut ? 1 : S.bt == BT ? 4 : 2;
             const auto key = std::make_tuple(S.Tw, S.nb, S.h16, ncp);
             auto f = cache.find(key);
             if (f == cache.end()) f = cache.emplace(key, make_half_copies(S.Bt, S.nb, S.h16, ncp)).first;
@@ -3522,7 +3543,7 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
     S.acc_len = (int)align_up((size_t)P.tt_max + (size_t)(T.nJ - 1) + (size_t)T.n_pass * 64, 4);
     S.lds = sizeof(double) * S.ntab + sizeof(float) * S.ntab + sizeof(long long) * S.acc_len +
             (size_t)(QCAP + QPAD) * 5 * 4 + 32 + (T.n_pass <= 12 ? sizeof(float2) * T.lut.size() : 0);
-    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut, S.pack, S.bt == BT ? 4 : 2);
+    if (S.basis) S.lds = basis_lds_bytes(S.ntab, S.Bt, S.h16, S.nb, S.lut, S.pack, S.lut ? 1 : S.bt == BT ? 4 : 2);
     GPURIR_CHECK(T.n_pass <= 48, "sampling frequency too high: at most 3072 window taps (fs 384 kHz) are supported");
     GPURIR_CHECK(S.lds <= 160 * 1024, "LDS budget exceeded: reduce the tile length or the image counts");
     return 0;
@@ -3628,13 +3649,15 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
 #define GPURIR_BASIS_FN(K, N, H) \
     (S.pack ? GPURIR_BASIS_FN1(K, N, H, false, true) : GPURIR_BASIS_FN1(K, N, H, false, false))
         const void* fn_once = S.bt != BT   ? reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>)
-                              : S.lut      ? GPURIR_BASIS_FN1(ism_basis_kernel, LUT_NB, 0, true, false)
+                              : S.lut      ? (dl640 ? reinterpret_cast<const void*>(&ism_basis_kernel<LUT_NB, 640, 3, true, false>)
+                                                    : reinterpret_cast<const void*>(&ism_basis_kernel<LUT_NB, 0, 0, true, false>))
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1)
                               : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 3)
                                            : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0);
         const void* fn_pers = S.bt != BT   ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>)
-                              : S.lut      ? GPURIR_BASIS_FN1(ism_basis_persistent_kernel, LUT_NB, 0, true, false)
+                              : S.lut      ? (dl640 ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<LUT_NB, 640, 3, true, false>)
+                                                    : reinterpret_cast<const void*>(&ism_basis_persistent_kernel<LUT_NB, 0, 0, true, false>))
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1)
                               : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 3)

@@ -1889,6 +1889,8 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
     float* tail = o + nISM;
     const int nblk = (nDiff + 3) >> 2;
     const int lane = tid & 63, wave = tid >> 6;
+    // the envelope's ratio between neighbouring samples: one exp2 per four samples
+    const float q1 = __builtin_amdgcn_exp2f(ha * 1.4426950408889634f), q2 = q1 * q1, q3 = q2 * q1;
     const int off = (4 - (int)((reinterpret_cast<unsigned long long>(tail) >> 2) & 3)) & 3;
     for (int b0 = 63 * wave; b0 <= nblk; b0 += 63 * NWAVE) {
         const int B = b0 + lane;
@@ -1911,12 +1913,13 @@ __global__ __launch_bounds__(WG) void ism_finalize_kernel(FinArgs a) {
             default: v[0] = p[3]; v[1] = z[0]; v[2] = z[1]; v[3] = z[2]; break;
         }
         const int sd0 = off + 4 * (B - 1);
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            // exp(x) = 2^(x log2 e) with the hardware exp2 (relative error ~1e-7)
-            const float env = __builtin_amdgcn_exp2f(fmaf((float)(sd0 + u), ha, base) * 1.4426950408889634f);
-            v[u] = gain * env * v[u];
-        }
+        // exp(x) = 2^(x log2 e) with the hardware exp2 (relative error ~1e-7) at the group's
+        // first sample, times the ratio's powers for the other three
+        const float ge = gain * __builtin_amdgcn_exp2f(fmaf((float)sd0, ha, base) * 1.4426950408889634f);
+        v[0] = ge * v[0];
+        v[1] = (ge * q1) * v[1];
+        v[2] = (ge * q2) * v[2];
+        v[3] = (ge * q3) * v[3];
         if (sd0 >= 0 && sd0 + 3 < nDiff) {
             *reinterpret_cast<float4*>(tail + sd0) = make_float4(v[0], v[1], v[2], v[3]);
         } else {
@@ -2870,21 +2873,28 @@ __global__ __launch_bounds__(WG) void tail_adjoint_kernel(TailAdjArgs a) {
     const unsigned k0 = (unsigned)a.seed, k1 = (unsigned)(a.seed >> 32);
     double s1 = 0.0, s2 = 0.0;
     const int nblk = (nDiff + 3) >> 2;
+    // sA exp(alpha ts / 2) at the thread's sample, advanced by its ratio per sample and per
+    // loop step (4 WG samples) instead of an FP64 exp per sample
+    const double r1 = exp(0.5 * alpha), rstep = exp(0.5 * alpha * (4.0 * WG));
+    double env_b = sA * exp(0.5 * alpha * ((double)(nISM + 4 * tid) - tau_dp));
     for (int blk = tid; blk < nblk; blk += WG) {
         unsigned w[4];
         philox4x32_10((unsigned)blk, (unsigned)m, R.room_id, a.call_offset, k0, k1, w);
+        double env = env_b;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int sd = 4 * blk + u;
             if (sd < nDiff) {
                 const double z = (double)tail_noise(w, u, a.noise);
                 const double ts = (double)(nISM + sd) - tau_dp;
-                const double q = sA * exp(0.5 * alpha * ts) * z;  // tail / level
+                const double q = env * z;  // tail / level
                 const double gq = (double)gy[nISM + sd] * q;
                 s1 += gq;
                 s2 += gq * ts;
             }
+            env *= r1;
         }
+        env_b *= rstep;
     }
     s1 = block_sum(s1, red);
     s2 = 0.5 * level * block_sum(s2, red);

@@ -1156,20 +1156,11 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             }
             return zc[0] + zc[1] + zc[2] + zc[3];
         };
-        // PK: the tile's image count bounds every field's sum, so it is counted before the
-        // first add (by a separate pass only when the rows take several rounds)
-        int tile_images = 0;
-        if (PK && nrows > WGB) {
-            for (int r0 = 0; r0 < nrows; r0 += WGB) {
-                int zf[4] = {0, 0, 0, 0}, zc[4] = {0, 0, 0, 0};
-                double b2;
-                int nx, ny;
-                const int cnt = r0 + tid < nrows ? row_runs(r0 + tid, zf, zc, b2, nx, ny) : 0;
-                int tot;
-                block_scan_excl1(cnt, misc + 16 + 8 * (scan_par++ & 1), &tot);
-                tile_images += tot;
-            }
-        }
+        // PK: the image count bounds every field's sum.  The scale is set from the images of
+        // the first row round; a later round whose images make the bound fall by k bits first
+        // halves the sums so far k times (each field rounded to nearest), so the sums end at
+        // the scale of the tile's whole count without a separate counting pass.
+        int seen = 0, E_cur = 0;
         float pscale = 1.0f;  // PK: the scale of row 0 (row m: pscale 2^-(F_0 - F_m))
         for (int r0 = 0; r0 < nrows; r0 += WGB) {
             const int row = r0 + tid;
@@ -1185,19 +1176,21 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
             }
             int tot;
             const int ex = block_scan_excl1(cnt, misc + 16 + 8 * (scan_par++ & 1), &tot);
-            if (PK && r0 == 0) {
-                if (nrows <= WGB) tile_images = tot;
+            if (PK) {
+                const int tile_images = seen + tot;
+                seen = tile_images;
                 // every image of the tile lies at d >= max(d_min, rlo): |amp| <= amax, and a
                 // field's sum is at most tile_images (amax scale + 1/2).  The scale of the
                 // narrowest field F_min keeps that below 2^(F_min - 1); a wider field's scale
                 // is larger by 2^(F - F_min); row 0 (widest) also stays below 2^30 per image
                 // for the int32 conversion.  The factor 1 + 2^-10 covers the FP32 rounding of
-                // the amplitudes, the recurrence and the bound itself.
+                // the amplitudes, the recurrence and the bound itself; one spare unit per row
+                // round, the rounding of a halving.
                 constexpr int F0 = pack_field_bits(NB, 0), FMIN = pack_lo_bits(NB, 0);
                 const float rt = fmaxf(dmin, (float)rlo);
                 const float amax = att_max * __builtin_amdgcn_rcpf(12.566370614359172f * rt) * (1.0f + 0x1p-10f);
                 const float tb = (float)tile_images * amax * (1.0f + 0x1p-10f);
-                const float room_f = 0x1p-1f * ldexpf(1.0f, FMIN) - (float)tile_images - 2.0f;
+                const float room_f = 0x1p-1f * ldexpf(1.0f, FMIN) - (float)tile_images - (float)(r0 / WGB) - 2.0f;
                 int E = 0;
                 if (tile_images > 0 && room_f > 0.0f && tb > 0.0f && amax < 3.0e38f) {
                     const float q = room_f / tb;
@@ -1205,8 +1198,27 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     E = min(E, 29 - ilogbf(amax));
                 }
                 E = min(max(E, -90), 120);
-                pscale = ldexpf(1.0f, E);
-                inv_scale = ldexpf(1.0f, -E);
+                if (r0 == 0) {
+                    E_cur = E;
+                } else if (E < E_cur) {
+                    // the rounds so far were added at 2^(E_cur - E) times this scale (the
+                    // previous queue round ended at a barrier; the next one's marks barrier
+                    // orders these writes before its adds)
+                    const int k = min(E_cur - E, 62);
+                    const long long half = 1LL << (k - 1);
+                    for (int e = tid; e < NB / 2 * DL; e += WGB) {
+                        const int w = e / DL;
+                        const int S = w == 0 ? pack_lo_bits(NB, 0) : w == 1 ? pack_lo_bits(NB, 1)
+                                    : w == 2 ? pack_lo_bits(NB, 2) : pack_lo_bits(NB, 3);
+                        const unsigned long long v = Dw[e];
+                        const long long lo = (long long)__builtin_amdgcn_sbfe((int)(unsigned)v, 0, S);
+                        const long long qh = ((long long)v - lo) >> S;
+                        Dw[e] = (unsigned long long)((((qh + half) >> k) << S) + ((lo + half) >> k));
+                    }
+                    E_cur = E;
+                }
+                pscale = ldexpf(1.0f, E_cur);
+                inv_scale = ldexpf(1.0f, -E_cur);
             }
             for (int q0 = 0; q0 < tot; q0 += QBK) {
                 // Entry p of the round (run g of row slot, run start s) has nz = base + 2 p with

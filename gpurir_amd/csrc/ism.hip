@@ -976,7 +976,10 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
     // SPL: FP32 filters from FP16 pairs (x = hi + lo 2^-11 for the coefficients and the rows,
     // hi lo' + lo hi' accumulated apart): three v_mfma_f32_16x16x16_f16 per 16 K values
     // instead of four v_mfma_f32_16x16x4_f32 per 4 K values
-    constexpr bool SPL = H16 == 3;
+    // H16 = 4 (the FP16 mode at 16 kHz): the same machinery with the rows and coefficients
+    // rounded to FP16 and one product per K step (the hi parts only)
+    constexpr bool SPL = H16 == 3 || H16 == 4;
+    constexpr bool HI1 = H16 == 4;
     static_assert(!SPL || (NB == 8 && !LUT) || (LUT && NB == LUT_NB && DLC == 640),
                   "FP16 pairs carry the 8-term FP32 expansion, or the table mode's 16 phase rows at 16 kHz");
     static_assert(!PK || (!LUT && NB % 2 == 0), "packed sums pair the Chebyshev rows");
@@ -1400,7 +1403,8 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                         const float xs = x * sc;
                         const _Float16 hi = (_Float16)xs;
                         Dh[m * DLP + col] = __builtin_bit_cast(unsigned short, hi);
-                        Dh[(NB + m) * DLP + col] = __builtin_bit_cast(unsigned short, (_Float16)((xs - (float)hi) * 2048.0f));
+                        if (!HI1)
+                            Dh[(NB + m) * DLP + col] = __builtin_bit_cast(unsigned short, (_Float16)((xs - (float)hi) * 2048.0f));
                     } else if (H16) {
                         Dh[m * DLP + col] = half_bits<H16>(x * sc);
                     } else {
@@ -1472,8 +1476,9 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                             const float xs = fv[m][sl] * sc;
                             const _Float16 hi = (_Float16)xs;
                             Dh[m * DLP + d + 4 * (d >> 4)] = __builtin_bit_cast(unsigned short, hi);
-                            Dh[(NB + m) * DLP + d + 4 * (d >> 4)] =
-                                __builtin_bit_cast(unsigned short, (_Float16)((xs - (float)hi) * 2048.0f));
+                            if (!HI1)
+                                Dh[(NB + m) * DLP + d + 4 * (d >> 4)] =
+                                    __builtin_bit_cast(unsigned short, (_Float16)((xs - (float)hi) * 2048.0f));
                         } else if (H16) {
                             Dh[m * DLP + d + 4 * (d >> 4)] = half_bits<H16>(fv[m][sl] * sc);
                         }
@@ -1539,13 +1544,13 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
                     const int r = q / (KSC / 4), t4 = q % (KSC / 4);
                     const unsigned short* ap = Ap0 + r * (CLC + CPAD) + 4 * t4;
                     ah[b] = ld_coef(ap);
-                    al[b] = ld_coef(ap + AHL);
+                    if (!HI1) al[b] = ld_coef(ap + AHL);
 #pragma unroll
                     for (int u = 0; u < UPW; u++) {
                         const int dl = kbu[u] + 4 * t4;
                         const unsigned short* xp = Bp0 + r * DLP + dl + 4 * (dl >> 4);
                         xh[b][u] = *reinterpret_cast<const half4_t*>(xp);
-                        xl[b][u] = *reinterpret_cast<const half4_t*>(xp + NB * DLP);
+                        if (!HI1) xl[b][u] = *reinterpret_cast<const half4_t*>(xp + NB * DLP);
                     }
                 };
                 load_step(0, 0);
@@ -1556,13 +1561,15 @@ __device__ __forceinline__ void basis_item(const AccArgs& a, int item, unsigned 
 #pragma unroll
                     for (int u = 0; u < UPW; u++) {
                         ahh[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1], xh[q & 1][u], ahh[u], 0, 0, 0);
-                        axx[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1], xl[q & 1][u], axx[u], 0, 0, 0);
-                        axx[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[q & 1], xh[q & 1][u], axx[u], 0, 0, 0);
+                        if (!HI1) {
+                            axx[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[q & 1], xl[q & 1][u], axx[u], 0, 0, 0);
+                            axx[u] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[q & 1], xh[q & 1][u], axx[u], 0, 0, 0);
+                        }
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
-                for (int u = 0; u < UPW; u++) accu[u] = (ahh[u] + axx[u] * (1.0f / 2048.0f)) * out_scale;
+                for (int u = 0; u < UPW; u++) accu[u] = (HI1 ? ahh[u] : ahh[u] + axx[u] * (1.0f / 2048.0f)) * out_scale;
             } else {
 if (H16) {
                 // four k per lane and instruction: K steps of 4 per lane group, KS / 4 products
@@ -1580,12 +1587,14 @@ if (H16) {
                         const int dl = kb + 4 * t4;
                         if (SPL) {
                             const half4_t xh = *reinterpret_cast<const half4_t*>(Bp + dl + 4 * (dl >> 4));
-                            const half4_t xl = *reinterpret_cast<const half4_t*>(Bp + NB * DLP + dl + 4 * (dl >> 4));
                             const half4_t ah = *reinterpret_cast<const half4_t*>(Ap + 4 * t4);
-                            const half4_t al = *reinterpret_cast<const half4_t*>(Ap + 4 * NB * (CL + 4) + 4 * t4);
                             acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xh, acc, 0, 0, 0);
-                            accx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xl, accx, 0, 0, 0);
-                            accx = __builtin_amdgcn_mfma_f32_16x16x16f16(al, xh, accx, 0, 0, 0);
+                            if (!HI1) {
+                                const half4_t xl = *reinterpret_cast<const half4_t*>(Bp + NB * DLP + dl + 4 * (dl >> 4));
+                                const half4_t al = *reinterpret_cast<const half4_t*>(Ap + 4 * NB * (CL + 4) + 4 * t4);
+                                accx = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, xl, accx, 0, 0, 0);
+                                accx = __builtin_amdgcn_mfma_f32_16x16x16f16(al, xh, accx, 0, 0, 0);
+                            }
                         } else if (H16 == 2) {
                             const short4_t xv = *reinterpret_cast<const short4_t*>(Bp + dl + 4 * (dl >> 4));
                             const short4_t av = *reinterpret_cast<const short4_t*>(Ap + 4 * t4);
@@ -1597,7 +1606,7 @@ if (H16) {
                         }
                     }
                 }
-                if (SPL) acc = acc + accx * (1.0f / 2048.0f);
+                if (SPL && !HI1) acc = acc + accx * (1.0f / 2048.0f);
                 acc = acc * out_scale;
             } else if constexpr (DLC == 640 && NB % 4 == 0) {
                 // The 16 kHz row length pins KS = 36: the wave's NB/4 rows are nine K steps each,
@@ -1698,7 +1707,7 @@ if (H16) {
 // The FP32 and table-mode kernels with the row length compiled in are held to 128 VGPRs
 // (two workgroups per CU); the same cap on the 16-bit-input kernels schedules them worse.
 template <int NB, int DLC, int H16, bool LUT, bool PK, int BTK = BT>
-__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3) ? 4 : 1)))
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3 || H16 == 4) ? 4 : 1)))
 void ism_basis_kernel(AccArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     basis_item<NB, DLC, H16, LUT, PK, BTK>(a, blockIdx.x, smem_raw);
@@ -1706,7 +1715,7 @@ void ism_basis_kernel(AccArgs a) {
 
 // Persistent form, items taken from a counter as in ism_accumulate_persistent_kernel.
 template <int NB, int DLC, int H16, bool LUT, bool PK, int BTK = BT>
-__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3) ? 4 : 1)))
+__global__ __launch_bounds__(WGB) __attribute__((amdgpu_waves_per_eu(DLC && (H16 == 0 || H16 == 3 || H16 == 4) ? 4 : 1)))
 void ism_basis_persistent_kernel(AccArgs a, int n_items, unsigned* counters) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int next;
@@ -3250,13 +3259,14 @@ BasisTables make_lut_basis_tables(const TapTables& T) {
 std::vector<unsigned short> make_half_copies(const BasisTables& B, int nb, int h16, int ncp = 4) {
     const int rs = B.cl + (ncp == 1 ? 2 : ncp);  // row stride: a single copy keeps rows word-aligned
     const size_t per = (size_t)ncp * nb * rs;
-    std::vector<unsigned short> img((h16 == 3 ? 2 : 1) * per, 0);
+    const bool pairs = h16 == 3 || h16 == 4;  // hi and lo halves (4 reads the hi ones only)
+    std::vector<unsigned short> img((pairs ? 2 : 1) * per, 0);
     for (size_t e = 0; e < per; e++) {
         const int rho = (int)(e / ((size_t)nb * rs));
         const int rm = (int)(e - (size_t)rho * nb * rs);
         const int m = rm / rs, z = rm - m * rs, src = z - rho;
         const float cv = (src >= 0 && src < B.cl) ? B.ctab[(size_t)m * B.cl + src] : 0.0f;
-        if (h16 == 3) {
+        if (pairs) {
             const float vs = cv * 16384.0f;
             const _Float16 hi = (_Float16)vs;
             img[e] = __builtin_bit_cast(unsigned short, hi);
@@ -3313,7 +3323,7 @@ std::vector<unsigned short> make_adjoint_half_copies(const std::vector<float>& c
 size_t basis_lds_bytes(int ntab, const BasisTables& B, int h16, int nb, bool lut, bool pack, int ncp = 4) {
     // 16-bit coefficients in ncp shifted copies (FP16 pairs: hi and lo), or FP32
     const size_t coef =
-        h16 ? (h16 == 3 ? 4 : 2) * (size_t)ncp * nb * (B.cl + (ncp == 1 ? 2 : ncp)) : 4 * (size_t)nb * B.cl;
+        h16 ? (h16 == 3 || h16 == 4 ? 4 : 2) * (size_t)ncp * nb * (B.cl + (ncp == 1 ? 2 : ncp)) : 4 * (size_t)nb * B.cl;
     // the region holds the sums and, after the conversion, the FP32 (or 16-bit) rows
     const size_t f32rows = 4 * (size_t)nb * (B.dl + B.dl / 4);
     const size_t rows = lut    ? f32rows
@@ -3489,12 +3499,25 @@ int sim_prepare(const std::vector<RoomHost>& rooms, const SimOptions& opt, SimLa
         // the table mode's 16 phase rows: FP16-pair filters at 16 kHz (one coefficient copy),
         // FP32 ones otherwise
         if (S.lut) S.h16 = S.Bt.dl == 640 ? 3 : 0;
+        // the FP16 mode at 16 kHz: the FP32 path's machinery (8 terms, packed sums, long tiles)
+        // with the rows and coefficients rounded to FP16 and one product per K step (4)
+        if (S.prec == PREC_FP16) {
+            const int nb_fp16 = S.nb;
+            S.nb = BASIS_NB;
+            const BasisTables B8 = tables(BT);
+            if (B8.dl == 640) {
+                S.h16 = 4;
+                S.Bt = B8;
+            } else {
+                S.nb = nb_fp16;
+            }
+        }
         int ntab = 0;
         for (const RoomHost& h : rooms) ntab = std::max(ntab, h.nb[0] + h.nb[1] + h.nb[2]);
         long long most_images = 0;
         for (const RoomHost& h : rooms) most_images = std::max(most_images, (long long)h.nb[0] * h.nb[1] * h.nb[2]);
-        S.pack = !S.lut && (S.h16 == 0 || S.h16 == 3) && most_images < PACK_MAX_IMAGES;
-        if (S.pack && S.h16 == 3 && S.Bt.dl == 640) {
+        S.pack = !S.lut && (S.h16 == 0 || S.h16 == 3 || S.h16 == 4) && most_images < PACK_MAX_IMAGES;
+        if (S.pack && (S.h16 == 3 || S.h16 == 4) && S.Bt.dl == 640) {
             // FP16 pairs at 16 kHz: tiles of 2 BT samples when the launch has enough of them
             // to fill the device and two workgroups still fit a CU's LDS
             long long tiles2 = 0;
@@ -3671,19 +3694,23 @@ int sim_enqueue_stage(const SimLaunch& S, int stage, const void* d_plan, float* 
     (dl640 ? reinterpret_cast<const void*>(&K<N, 640, H, U, P>) : reinterpret_cast<const void*>(&K<N, 0, H, U, P>))
 #define GPURIR_BASIS_FN(K, N, H) \
     (S.pack ? GPURIR_BASIS_FN1(K, N, H, false, true) : GPURIR_BASIS_FN1(K, N, H, false, false))
-        const void* fn_once = S.bt != BT   ? reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>)
+        const void* fn_once = S.bt != BT   ? (S.h16 == 4 ? reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 1152, 4, false, true, 2 * BT>)
+                                                        : reinterpret_cast<const void*>(&ism_basis_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>))
                               : S.lut      ? (dl640 ? reinterpret_cast<const void*>(&ism_basis_kernel<LUT_NB, 640, 3, true, false>)
                                                     : reinterpret_cast<const void*>(&ism_basis_kernel<LUT_NB, 0, 0, true, false>))
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB_FP16, 1)
                               : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 3)
+                              : S.h16 == 4 ? GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 4)
                                            : GPURIR_BASIS_FN(ism_basis_kernel, BASIS_NB, 0);
-        const void* fn_pers = S.bt != BT   ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>)
+        const void* fn_pers = S.bt != BT   ? (S.h16 == 4 ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 1152, 4, false, true, 2 * BT>)
+                                                        : reinterpret_cast<const void*>(&ism_basis_persistent_kernel<BASIS_NB, 1152, 3, false, true, 2 * BT>))
                               : S.lut      ? (dl640 ? reinterpret_cast<const void*>(&ism_basis_persistent_kernel<LUT_NB, 640, 3, true, false>)
                                                     : reinterpret_cast<const void*>(&ism_basis_persistent_kernel<LUT_NB, 0, 0, true, false>))
                               : S.h16 == 2 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 2)
                               : S.h16 == 1 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB_FP16, 1)
                               : S.h16 == 3 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 3)
+                              : S.h16 == 4 ? GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 4)
                                            : GPURIR_BASIS_FN(ism_basis_persistent_kernel, BASIS_NB, 0);
 #undef GPURIR_BASIS_FN
 #undef GPURIR_BASIS_FN1

@@ -800,7 +800,7 @@ constexpr int QB = 2048;    // queue entries per round
 constexpr int NOT = BT / 256;  // 256-sample output blocks of a tile (16 x 16 products)
 static_assert(NWB == 4 * NOT, "four basis quarters per output block");
 constexpr int BASIS_NB = 8; // Chebyshev terms (FP32: within 5e-7 of each tap)
-constexpr int BASIS_NB_FP16 = 6;  // mixed precision: 6 terms, within 6e-5 (1e-4 relative L2)
+constexpr int BASIS_NB_FP16 = 6;  // mixed precision away from 16 kHz: 6 terms, within 6e-5 (1e-4 relative L2)
 constexpr int BCS = 2;      // columns per thread of the in-place conversion: DL <= BCS * WGB
 // Packed basis sums: word w < NB/2 of a column holds basis row w in its high 64 - S bits
 // and row NB - 1 - w in its low S bits (lds_add_pair).  The low fields take the rows with

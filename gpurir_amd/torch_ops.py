@@ -73,7 +73,8 @@ def simulate_rir(room_sz, beta, pos_src, pos_rcv, nb_img, Tmax, fs, Tdiff=None, 
     Tdiff (0, the default, is the reference's hard switch).  method: "auto" (the default: "basis"
     when the launch has at least 256 tiles of 512 early samples and fs <= 64 kHz, else "taps"),
     "basis" (per-image sums on a few rows and matrix-core filters: a Chebyshev expansion of the
-    taps in the fractional delay, 8 terms in "fp32", 6 terms and FP16 filter inputs in "fp16";
+    taps in the fractional delay, 8 terms in "fp32", in "fp16" the same 8 terms with FP16 rows
+    and coefficients at 16 kHz (6 terms and FP16 filter inputs at other rates);
     16 phase rows of the table's 1/16-sample grid in "lut") or "taps" (every image scatters its
     windowed-sinc taps)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)

@@ -52,7 +52,8 @@ extern "C" {
 #define GPURIR_METHOD_BASIS 2 /* per-image sums on a few rows, then matrix-core FIRs:     */
                               /* FP32, 8-term Chebyshev expansion of the taps in the     */
                               /* fractional delay (<= 5e-7 per tap of 1); mixed precision */
-                              /* 6 terms and FP16 filter inputs (<= 6e-5); table mode,    */
+                              /* FP16 rows and coefficients (8 terms at 16 kHz, 6 terms,  */
+                              /* <= 6e-5 per tap, at other rates); table mode,            */
                               /* 16 phase rows of the 1/16-sample grid (the table's own   */
                               /* linear interpolation, exactly)                           */
 

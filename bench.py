@@ -116,46 +116,49 @@ def main():
             T.trajectory(sig5, rirs5)
         torch.cuda.synchronize()
         # device time of the convolution + overlap-add: a trajectory plan bound to the RIRs and the
-        # signal replays its kernels as one HIP graph (events on the stream they run on); the eager
-        # call (planning cached, three launches, Python argument handling) timed beside it
+        # signal launches its three kernels (events on the stream they run on); the eager call
+        # (planning cached, three launches, Python argument handling) timed beside it
+        # Each figure is the median of three rounds: these calls are short enough (tens of µs to
+        # 1 ms) for a stall of the host to inflate a single round's average several times over.
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def rounds(fn, n, k=3):
+            dev_ms, host_ms = [], []
+            for _ in range(k):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                e0.record()
+                for _ in range(n):
+                    fn()
+                e1.record()
+                torch.cuda.synchronize()
+                host_ms.append((time.perf_counter() - t0) * 1e3 / n)
+                dev_ms.append(e0.elapsed_time(e1) / n)
+            return float(np.median(dev_ms)), float(np.median(host_ms))
+
         n_tr = 20
         plan5 = T.TrajectoryPlan(sig5, rirs5)
         for _ in range(3):
             plan5.run()
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(n_tr):
-            y5 = plan5.run()
-        e1.record()
-        torch.cuda.synchronize()
-        ms_traj = e0.elapsed_time(e1) / n_tr
-        t0 = time.perf_counter()
-        e0.record()
-        for _ in range(n_tr):
-            y5e = T.trajectory(sig5, rirs5)
-        e1.record()
-        torch.cuda.synchronize()
-        ms_traj_host = (time.perf_counter() - t0) * 1e3 / n_tr
-        ms_traj_eager = e0.elapsed_time(e1) / n_tr
+        ms_traj, _ = rounds(plan5.run, n_tr)
+        y5 = plan5.run()
+        ms_traj_eager, ms_traj_host = rounds(lambda: T.trajectory(sig5, rirs5), n_tr)
+        y5e = T.trajectory(sig5, rirs5)
         assert torch.equal(y5, y5e), "trajectory plan and eager call differ"
         P5, R5, L5 = rirs5.shape
         n5 = sig5.numel()
         traj_bytes = 4 * P5 * R5 * L5 + 4 * n5 + 8 * (n5 + L5 - 1) * R5  # algorithmic HBM bytes of stage (d)
         traj_gbs = traj_bytes / (ms_traj / 1e3) / 1e9
-        # eager simulation of the 1024 trajectory RIRs (plan built on the host each call): 4 warm-up
-        # calls (the first ones after lighter work run up to 20% slower), then the mean of 10
+        # eager simulation of the 1024 trajectory RIRs (the plan cached by the library, positions
+        # from host arrays): 4 warm-up calls (the first ones after lighter work run up to 20%
+        # slower), then the median of three rounds of 10
+        def traj_rirs():
+            T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
+                           out=rirs5)
+
         for _ in range(4):
-            T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
-                           out=rirs5)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(10):
-            T.simulate_rir(c5["room_sz"], c5["beta"], c5["pos_src"], c5["pos_rcv"], c5["nb_img"], c5["Tmax"], fs,
-                           out=rirs5)
-        e1.record()
-        torch.cuda.synchronize()
-        ms_traj_rirs = e0.elapsed_time(e1) / 10
+            traj_rirs()
+        ms_traj_rirs, _ = rounds(traj_rirs, 10)
         log(f"[bench] trajectory: RIRs {ms_traj_rirs:.2f} ms; convolution+OLA {ms_traj:.3f} ms on the device "
             f"({traj_gbs:.0f} GB/s of {traj_bytes / 1e6:.1f} MB algorithmic); eager call {ms_traj_eager:.3f} ms on the "
             f"device clock, {ms_traj_host:.3f} ms on the host")

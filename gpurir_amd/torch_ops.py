@@ -32,7 +32,12 @@ def _pat(p):
 def _dev_f32(x, dev):
     if isinstance(x, torch.Tensor):
         return x.detach().to(device=dev, dtype=torch.float32).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float32), device=dev)
+    h = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32))
+    if torch.device(dev).type != "cuda":
+        return h.to(dev)
+    # through pinned memory without a wait: a copy from pageable memory would wait for the
+    # stream's earlier work, so each call would start only after the previous one had ended
+    return h.pin_memory().to(device=dev, non_blocking=True)
 
 
 def _stream(dev):
@@ -451,9 +456,9 @@ class TrajectoryPlan:
 
     signal [n] and rirs [P, R, L] are float32 device tensors that the plan keeps and reads at
     every run (refill them in place between runs); out [n + L - 1, R] float64 is the plan's
-    result buffer.  With simulateTrajectory's own segmentation (no timestamps) the plan replays
-    one HIP graph of its kernels, so a run costs one graph launch on the host; other
-    segmentations run the eager path."""
+    result buffer.  With simulateTrajectory's own segmentation (no timestamps) a run launches
+    the plan's three kernels and does no other host work; other segmentations run the eager
+    path."""
 
     def __init__(self, signal, rirs, timestamps=None, fs=None):
         if not (isinstance(signal, torch.Tensor) and isinstance(rirs, torch.Tensor)):

@@ -236,8 +236,9 @@ int gpurir_trajectory_device(const float* d_signal, int n_signal, const float* d
  * d_rirs, d_out, as above) and one segmentation w_ini (host, copied).  The plan owns its
  * tables and workspace; each run enqueues the simulateTrajectory of the buffers' current
  * contents on `stream`.  Segments of S or S + 1 samples (256 <= S <= 1536, the segmentation
- * of simulateTrajectory without timestamps, gpuRIR/__init__.py:204-208) replay one HIP graph
- * captured on the first run; any other segmentation runs gpurir_trajectory_device eagerly.
+ * of simulateTrajectory without timestamps, gpuRIR/__init__.py:204-208) launch the plan's
+ * three kernels directly (no host work per run; a caller may capture them in its own graph);
+ * any other segmentation runs gpurir_trajectory_device eagerly.
  * No counterpart in the reference: it serves device-resident pipelines that call
  * simulateTrajectory repeatedly on the same shapes. */
 typedef struct gpurir_traj_plan* gpurir_traj_plan_t;

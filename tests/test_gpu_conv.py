@@ -90,10 +90,11 @@ def test_trajectory_segment_aligned(n, P, R, L):
     assert rel_l2(got.cpu().numpy(), want) < 1e-5
 
 
-def test_trajectory_plan_graph_replays():
-    # the plan's captured graph against the eager path: bit-identical on every replay, with eager
-    # work of another segmentation (which replaces the library's cached tables and workspace)
-    # and synchronisations between replays, and after the RIR buffer is refilled in place
+def test_trajectory_plan_runs_and_graph_replays():
+    # the plan's runs against the eager path: bit-identical on every run, with eager work of
+    # another segmentation (which replaces the library's cached tables and workspace) and
+    # synchronisations between runs, and after the RIR buffer is refilled in place; then a run
+    # captured by the caller in a graph of its own, checked on every replay
     rng = np.random.default_rng(11)
     n, P, R, L = 16000, 16, 3, 2000
     sig = torch.as_tensor(rng.standard_normal(n).astype(np.float32), device="cuda")
@@ -113,6 +114,23 @@ def test_trajectory_plan_graph_replays():
     rirs.mul_(-0.5)
     got = plan.run().clone()
     assert torch.equal(got, T.trajectory(sig, rirs))
+    want = got
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.run()  # a run before capture, on the capturing stream
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run()
+    for _ in range(3):
+        plan.out.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(plan.out, want)
+        T.trajectory(other_sig, other_rirs)
+        torch.cuda.synchronize()
     # a segmentation outside the segment-aligned path runs eagerly inside the plan
     ts = np.array([0.0, 0.1, 0.15, 0.5, 0.55, 0.6, 0.7, 0.8, 0.82, 0.84, 0.9, 0.92, 0.94, 0.96, 0.98, 0.99])
     plan2 = T.TrajectoryPlan(sig, rirs, timestamps=ts, fs=n)
